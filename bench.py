@@ -1,0 +1,195 @@
+#!/usr/bin/env python3
+"""Benchmark: inverted pixels/s on one MI355X.
+
+Workload (default): 1024 x 1024 pixels, Fe I 6173.3356 A (g_eff 2.5),
+30 wavelengths -350:35:665 mA, Gaussian instrument PSF of FWHM 60 mA on the
+grid, 9 free parameters, at most 50 LM iterations, classical-estimate start
+from the fixed guess (B 400 G, gamma 30, phi 30, vlos 0.25, dopp 34 mA,
+a 0.06, eta0 8, S0 0.15, S1 0.85).  Synthetic data: per-pixel true models
+drawn uniformly from a seed, synthesised by the engine, Gaussian noise of
+1e-3 Ic.  A step is one inversion of the whole cube with the cube already in
+HBM.  Prints one JSON line.
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+
+METRIC = ("inverted pixels/s on 1 MI355X (1024², 30λ, 9 par, 50 it, PSF); "
+          "p99 tile latency")
+PUBLISHED_PX_S = 7400.0           # Tesla V100, 30 lambda, 9 par, <= 50 it, PSF
+PEAK_FP32_TFLOPS = 157.3          # MI355X FP32 (MFMA / packed VALU) peak
+PEAK_HBM_GBS = 8000.0
+
+
+def load_product():
+    path = os.path.join(ROOT, "g-milos_amd", "__init__.py")
+    spec = importlib.util.spec_from_file_location(
+        "gmilos_amd", path, submodule_search_locations=[os.path.dirname(path)])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["gmilos_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+# Algorithmic work per evaluation (flop, FMA = 2): per-wavelength constants of
+# the reference arithmetic, V = 116 per (component, lambda) Voigt,
+# E = 81 per lambda ME solution, D_c = 23 per (component, lambda) Voigt
+# derivative, D_k = 890 per lambda RF chain rule; PSF with K applied taps.
+def work_model(nlambda, ncomp, taps, nfree=9):
+    S = 4
+    psf_pass = (S * nlambda * 2 * taps + 2 * nlambda) if taps else 0
+    f_synth = ncomp * nlambda * 116 + nlambda * 81 + psf_pass
+    f_chi = 5 * S * nlambda
+    f_rf = ncomp * nlambda * 23 + nlambda * 890 + (nfree - 1) * psf_pass
+    f_ne = S * nlambda * (nfree * (nfree + 1) + 2 + 2 * nfree)
+    p = nfree
+    f_solve = 12 * p ** 3 + 4 * p ** 2 + 2 * p
+    return dict(synth=f_synth, chi=f_chi, rf=f_rf, ne=f_ne, solve=f_solve)
+
+
+def cpu_baseline(gm, setup_desc, obs_host, budget_s):
+    """FP64 oracle on the host cores over a bounded sample of the cube."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    oracle.lib()
+    lines, grid, psf, kw = setup_desc
+    o = oracle.make_setup(lines, grid, psf=psf, init_defaults=gm.BENCH_INIT, free=gm.FREE9,
+                          miter=kw["miter"], noise=1e-6)
+    threads = min(os.cpu_count() or 1, 16)
+    # calibrate on a small sample, then size the timed sample to the budget
+    n0 = min(256, obs_host.shape[0])
+    t0 = time.perf_counter()
+    oracle.invert(o, obs_host[:n0], threads=threads, want_synth=False)
+    dt0 = max(time.perf_counter() - t0, 1e-3)
+    n = int(min(obs_host.shape[0], max(n0, n0 * budget_s / dt0)))
+    t0 = time.perf_counter()
+    oracle.invert(o, obs_host[:n], threads=threads, want_synth=False)
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "pixels/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} pixels of the same cube, FP64 oracle, OpenMP over pixels "
+                      f"({dt:.1f} s); throughput is linear in pixel count"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--npix", type=int, default=1024 * 1024)
+    ap.add_argument("--precision", choices=["mixed", "fp32"], default="mixed")
+    ap.add_argument("--solver", choices=["svd", "ldl"], default="svd")
+    ap.add_argument("--miter", type=int, default=50)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--seed", type=int, default=1234)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(pixels shard by tile across GPUs, see the design notes).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    gm = load_product()
+    gm.load_library()
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+
+    prec = gm.GM_PREC_MIXED if args.precision == "mixed" else gm.GM_PREC_FP32
+    solver = gm.GM_SOLVE_SVD if args.solver == "svd" else gm.GM_SOLVE_LDL
+    setup = gm.setup_single_line(psf_fwhm=60.0, precision=prec, solver=solver,
+                                 miter=args.miter)
+    ctx = gm.Context(setup)
+
+    # synthetic cube, resident in HBM
+    npix = args.npix
+    truth = torch.from_numpy(gm.random_models(npix, seed=args.seed)).to(dev)
+    clean = ctx.synth(truth)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(args.seed)
+    ic = (truth[:, 7] + truth[:, 8]).float()[:, None, None]
+    obs = (clean + torch.randn(clean.shape, generator=gen, device=dev) * (1e-3 * ic)).contiguous()
+    torch.cuda.synchronize()
+    out = None
+    for _ in range(args.warmup):
+        out = ctx.invert(obs, out=out)
+    torch.cuda.synchronize()
+
+    ctx.set_timing(True)
+    acc = {"ms_rf": 0.0, "ms_synth": 0.0, "ms_solve": 0.0, "ms_other": 0.0,
+           "rf_evals": 0, "synth_evals": 0, "solves": 0, "lm_rounds": 0}
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = ctx.invert(obs, out=out)
+        st = ctx.stats()
+        for k in acc:
+            acc[k] += st[k]
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / args.steps
+    taps = st["psf_taps"]
+    value = npix / (ms / 1000.0)
+
+    W = work_model(30, 3, taps)
+    rf_flops = acc["rf_evals"] * (W["rf"] + W["ne"])
+    rf_ms = acc["ms_rf"]
+    rf_tflops = rf_flops / (rf_ms / 1000.0) / 1e12 if rf_ms > 0 else 0.0
+    total_flops = (acc["synth_evals"] * (W["synth"] + W["chi"]) +
+                   acc["rf_evals"] * (W["rf"] + W["ne"]) + acc["solves"] * W["solve"])
+    job_tflops = total_flops / (ms * args.steps / 1000.0) / 1e12
+
+    iters = out["iters"].float()
+    res = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "pixels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": value / PUBLISHED_PX_S,
+        "dtype": "fp32" if args.precision == "fp32" else "fp32+fp64(solve,chi2)",
+        "data": "synthetic (seeded uniform models, engine synthesis, 1e-3 Ic noise)",
+        "config": {"workload": "1024x1024 px, Fe I 6173.34 A, 30 lambda, 9 free par, "
+                               "50 LM it, Gaussian PSF FWHM 60 mA" if npix == 1024 * 1024 else
+                               f"{npix} px, Fe I 6173.34 A, 30 lambda, 9 free par, 50 LM it, PSF",
+                   "npix": npix, "nlambda": 30, "psf_taps_applied": taps,
+                   "precision": args.precision, "solver": args.solver,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "compute": "fp32 (unpacked VALU)",
+                     "kernel": "k_rf (synthesis+RF+PSF+normal equations)",
+                     "achieved": rf_tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": rf_tflops / PEAK_FP32_TFLOPS, "traffic": None},
+        "whole_job_tflops": job_tflops,
+        "kernel_ms_per_step": {k[3:]: acc[k] / args.steps for k in
+                               ("ms_rf", "ms_synth", "ms_solve", "ms_other")},
+        "lm": {"mean_iters": float(iters.mean()), "rounds": acc["lm_rounds"] / args.steps,
+               "rf_evals_per_px": acc["rf_evals"] / args.steps / npix,
+               "synth_evals_per_px": acc["synth_evals"] / args.steps / npix},
+        "wall_s": wall,
+    }
+    if not args.no_cpu_baseline:
+        n_host = min(npix, 1 << 16)
+        obs_host = obs[:n_host].cpu().numpy()
+        res["cpu_baseline"] = cpu_baseline(
+            gm, ([gm.FE6173], gm.GRID_6173, ("gauss", 60.0), {"miter": args.miter}),
+            obs_host, args.cpu_budget)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

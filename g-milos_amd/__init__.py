@@ -1,0 +1,462 @@
+"""MI355X-native G-MILOS inversion engine: Python host mirror of the C ABI.
+
+The reference is driven as ``gmilos <file>.mtrol`` (gmilos.cu:86-1341); its
+engine is the per-pixel Levenberg-Marquardt kernel ``lm_mils``
+(milosUtils.cu:574-710) fed by constant symbols (gmilos.cu:1184-1198).  This
+module mirrors that surface over ``include/gmilos.h``:
+
+* :func:`make_setup` builds the constant set (line, grid, PSF, weights, LM
+  constants, free parameters, initial model) from the same ingredients the
+  reference reads from LINES / malla.grid / .psf / .mod / .mtrol files;
+* :class:`Context` owns a ``gm_ctx`` and runs the inversion or its pieces on
+  device buffers (torch tensors are used only as device memory).
+
+The HIP library must be built (``build()``); every entry point raises when it
+is missing -- there is no CPU fallback.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libgmilos.so")
+
+GM_MAX_LAMBDA = 256
+GM_MAX_LINES = 4
+GM_MAX_COMP = 10
+GM_NMODEL = 11
+
+GM_PREC_FP32, GM_PREC_MIXED = 0, 1
+GM_SOLVE_SVD, GM_SOLVE_LDL = 0, 1
+GM_INIT_CE, GM_INIT_MODEL, GM_INIT_PER_PIXEL = 0, 1, 2
+
+# Init_Model order (defines.h:94-106)
+MODEL_FIELDS = ("eta0", "B", "vlos", "dopp", "aa", "gm", "az", "S0", "S1", "mac", "alfa")
+
+STATUS = {0: "ok", 1: "invalid argument", 2: "HIP runtime error", 3: "out of device memory",
+          4: "unsupported configuration", 5: "not found"}
+
+
+class GmError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__(f"{what}: {STATUS.get(code, code)} ({code})")
+        self.code = code
+
+
+class gm_zeeman(ctypes.Structure):
+    _fields_ = [("n_pi", ctypes.c_int32), ("n_sig", ctypes.c_int32),
+                ("nub", ctypes.c_double * GM_MAX_COMP), ("nup", ctypes.c_double * GM_MAX_COMP),
+                ("nur", ctypes.c_double * GM_MAX_COMP), ("web", ctypes.c_double * GM_MAX_COMP),
+                ("wep", ctypes.c_double * GM_MAX_COMP), ("wer", ctypes.c_double * GM_MAX_COMP),
+                ("gl", ctypes.c_double), ("gu", ctypes.c_double), ("geff", ctypes.c_double),
+                ("fo", ctypes.c_double)]
+
+
+class gm_setup(ctypes.Structure):
+    _fields_ = [("nlambda", ctypes.c_int32), ("lambda_", ctypes.c_double * GM_MAX_LAMBDA),
+                ("nlines", ctypes.c_int32), ("lambda0", ctypes.c_double * GM_MAX_LINES),
+                ("zee", gm_zeeman * GM_MAX_LINES), ("use_psf", ctypes.c_int32),
+                ("psf", ctypes.c_double * GM_MAX_LAMBDA), ("weight", ctypes.c_double * 4),
+                ("noise", ctypes.c_double), ("ilambda", ctypes.c_double),
+                ("toplim", ctypes.c_double), ("miter", ctypes.c_int32), ("mu", ctypes.c_double),
+                ("logclambda", ctypes.c_int32), ("fix", ctypes.c_int32 * 11),
+                ("init_model", ctypes.c_double * GM_NMODEL), ("init_mode", ctypes.c_int32),
+                ("ce_line", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("solver", ctypes.c_int32), ("rf_reference_rules", ctypes.c_int32),
+                ("psf_tail_tol", ctypes.c_double)]
+
+
+class gm_out(ctypes.Structure):
+    _fields_ = [("model", ctypes.c_void_p), ("chisqr", ctypes.c_void_p),
+                ("iters", ctypes.c_void_p), ("synth", ctypes.c_void_p),
+                ("naccs", ctypes.c_void_p)]
+
+
+class gm_stats(ctypes.Structure):
+    _fields_ = [("npix", ctypes.c_int64), ("synth_evals", ctypes.c_int64),
+                ("rf_evals", ctypes.c_int64), ("solves", ctypes.c_int64),
+                ("lm_rounds", ctypes.c_int32), ("psf_taps", ctypes.c_int32),
+                ("ms_total", ctypes.c_float), ("ms_synth", ctypes.c_float),
+                ("ms_rf", ctypes.c_float), ("ms_solve", ctypes.c_float),
+                ("ms_other", ctypes.c_float)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+EXPORTED = ("gm_ctx_create", "gm_ctx_destroy", "gm_status_string", "gm_ctx_set_timing",
+            "gm_last_stats", "gm_invert", "gm_synth", "gm_synth_rf", "gm_normal_equations",
+            "gm_step_solve", "gm_classical_estimates", "gm_zeeman_from_quantum",
+            "gm_parse_lines_file", "gm_psf_gaussian", "gm_psf_from_table", "gm_setup_defaults")
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Loads libgmilos.so; raises when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build the HIP library first")
+    lib = ctypes.CDLL(path)
+    vp, i64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32
+    P = ctypes.POINTER
+    sig = {
+        "gm_ctx_create": ([P(gm_setup), P(vp)], i32),
+        "gm_ctx_destroy": ([vp], i32),
+        "gm_status_string": ([i32], ctypes.c_char_p),
+        "gm_ctx_set_timing": ([vp, i32], i32),
+        "gm_last_stats": ([vp, P(gm_stats)], i32),
+        "gm_invert": ([vp, vp, i64, vp, P(gm_out), vp], i32),
+        "gm_synth": ([vp, vp, i64, vp, vp], i32),
+        "gm_synth_rf": ([vp, vp, i64, vp, vp, vp], i32),
+        "gm_normal_equations": ([vp, vp, vp, i64, vp, vp, vp, vp], i32),
+        "gm_step_solve": ([vp, vp, vp, vp, i64, vp, vp], i32),
+        "gm_classical_estimates": ([vp, vp, i64, vp, vp], i32),
+        "gm_zeeman_from_quantum": ([ctypes.c_double] * 7 + [P(gm_zeeman)], i32),
+        "gm_parse_lines_file": ([ctypes.c_char_p, i32, P(ctypes.c_double),
+                                 P(ctypes.c_double)], i32),
+        "gm_psf_gaussian": ([ctypes.c_double, P(ctypes.c_double), i32, P(ctypes.c_double)], i32),
+        "gm_psf_from_table": ([P(ctypes.c_double), P(ctypes.c_double), i32, P(ctypes.c_double),
+                               i32, ctypes.c_double, P(ctypes.c_double)], i32),
+        "gm_setup_defaults": ([P(gm_setup)], None),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def _check(code, what):
+    if code != 0:
+        raise GmError(code, what)
+
+
+def _dptr(a):
+    return (ctypes.c_double * len(a))(*[float(x) for x in a])
+
+
+# ---------------------------------------------------------------------------
+# set-up construction
+def zeeman(qn, fo=1.0):
+    """Zeeman pattern from (S_lo, L_lo, J_lo, S_up, L_up, J_up)."""
+    lib = load_library()
+    z = gm_zeeman()
+    _check(lib.gm_zeeman_from_quantum(*[float(x) for x in qn], float(fo), ctypes.byref(z)),
+           "gm_zeeman_from_quantum")
+    return z
+
+
+def parse_lines(path, index):
+    """(lambda0, quantum numbers) of one LINES record (readConfig.c:388-492)."""
+    lib = load_library()
+    l0 = ctypes.c_double()
+    qn = (ctypes.c_double * 6)()
+    _check(lib.gm_parse_lines_file(path.encode(), int(index), ctypes.byref(l0), qn),
+           "gm_parse_lines_file")
+    return l0.value, tuple(qn)
+
+
+def psf_gaussian(fwhm_mA, lam):
+    lib = load_library()
+    n = len(lam)
+    out = (ctypes.c_double * n)()
+    _check(lib.gm_psf_gaussian(float(fwhm_mA), _dptr(lam), n, out), "gm_psf_gaussian")
+    return np.array(out[:], dtype=np.float64)
+
+
+def psf_from_table(dl_mA, val, offsets_mA, step_mA):
+    lib = load_library()
+    n = len(offsets_mA)
+    out = (ctypes.c_double * n)()
+    _check(lib.gm_psf_from_table(_dptr(dl_mA), _dptr(val), len(dl_mA), _dptr(offsets_mA), n,
+                                 float(step_mA), out), "gm_psf_from_table")
+    return np.array(out[:], dtype=np.float64)
+
+
+# Fixed initial guess of the benchmark workloads (init.mod style).
+BENCH_INIT = dict(eta0=8.0, B=400.0, vlos=0.25, dopp=0.034, aa=0.06, gm=30.0, az=30.0,
+                  S0=0.15, S1=0.85, mac=0.0, alfa=1.0)
+FREE9 = ("eta0", "B", "vlos", "dopp", "aa", "gm", "az", "S0", "S1")
+
+
+def make_setup(lines, offsets_mA, psf=None, init=None, free=FREE9, miter=50, noise=1e-6,
+               weights=(1.0, 1.0, 1.0, 1.0), ilambda=0.1, toplim=1e-12, mu=1.0,
+               precision=GM_PREC_MIXED, solver=GM_SOLVE_SVD, init_mode=GM_INIT_CE,
+               ce_line=0, rf_reference_rules=1, psf_tail_tol=1e-9, logclambda=0):
+    """Builds a gm_setup.
+
+    lines: list of (lambda0 [A], (S_lo, L_lo, J_lo, S_up, L_up, J_up), fo).
+    offsets_mA: grid offsets from lines[0]'s centre in mA (malla.grid style).
+    psf: None, ("gauss", fwhm_mA) or ("table", dl_mA, values).
+    """
+    lib = load_library()
+    s = gm_setup()
+    lib.gm_setup_defaults(ctypes.byref(s))
+    offsets = np.asarray(offsets_mA, dtype=np.float64)
+    n = len(offsets)
+    if n > GM_MAX_LAMBDA or len(lines) > GM_MAX_LINES:
+        raise ValueError("grid or line count beyond the engine's limits")
+    lam0 = float(lines[0][0])
+    # absolute grid as gmilos.cu:190-195 (cumulative steps from the first
+    # offset) when the grid is uniform, else offsets added directly
+    steps = np.diff(offsets)
+    if n > 1 and np.allclose(steps, steps[0]):
+        lam = [lam0 + offsets[0] / 1000.0]
+        for _ in range(1, n):
+            lam.append(lam[-1] + steps[0] / 1000.0)
+    else:
+        lam = list(lam0 + offsets / 1000.0)
+    s.nlambda = n
+    for i, v in enumerate(lam):
+        s.lambda_[i] = v
+    s.nlines = len(lines)
+    for L, (l0, qn, fo) in enumerate(lines):
+        s.lambda0[L] = float(l0)
+        s.zee[L] = zeeman(qn, fo)
+    if psf is None:
+        s.use_psf = 0
+    else:
+        s.use_psf = 1
+        if psf[0] == "gauss":
+            p = psf_gaussian(psf[1], lam)
+        elif psf[0] == "table":
+            p = psf_from_table(psf[1], psf[2], offsets, float(steps[0]) if n > 1 else 1.0)
+        elif psf[0] == "taps":
+            p = np.asarray(psf[1], dtype=np.float64)
+        else:
+            raise ValueError(f"unknown psf kind {psf[0]}")
+        for i, v in enumerate(p):
+            s.psf[i] = float(v)
+    for i in range(4):
+        s.weight[i] = float(weights[i])
+    s.noise = float(noise)
+    s.ilambda = float(ilambda)
+    s.toplim = float(toplim)
+    s.miter = int(miter)
+    s.mu = float(mu)
+    s.logclambda = int(logclambda)
+    for i in range(11):
+        s.fix[i] = 0
+    for name in free:
+        s.fix[MODEL_FIELDS.index(name)] = 1
+    model = dict(BENCH_INIT)
+    if init:
+        model.update(init)
+    for i, f in enumerate(MODEL_FIELDS):
+        s.init_model[i] = float(model[f])
+    s.init_mode = int(init_mode)
+    s.ce_line = int(ce_line)
+    s.precision = int(precision)
+    s.solver = int(solver)
+    s.rf_reference_rules = int(rf_reference_rules)
+    s.psf_tail_tol = float(psf_tail_tol)
+    return s
+
+
+# Workload descriptions (synthetic benchmark and test configurations) ------
+FE6173 = (6173.3356, (2, 1, 1.0, 2, 2, 0.0), 1.0)     # 5P1-5D0, g_eff 2.5
+FE6301 = (6301.515, (2, 1, 2.0, 2, 2, 2.0), 1.0)      # 5P2-5D2, g_eff 1.67
+FE6302 = (6302.494, (2, 1, 1.0, 2, 2, 0.0), 1.0)      # 5P1-5D0, g_eff 2.5
+# in a 6301.5 + 6302.5 pair the 6302.5 strength relative to 6301.5 follows
+# their log(gf) difference in run/LINES (-1.16 vs -0.59): 10**-0.57
+FE6302_PAIR = (6302.494, (2, 1, 1.0, 2, 2, 0.0), 0.27)
+GRID_6173 = np.arange(-350.0, 665.0 + 1e-9, 35.0)       # 30 samples
+GRID_6302 = -716.0 + 21.5 * np.arange(112)              # 112 samples over both lines
+
+
+def setup_single_line(psf_fwhm=None, **kw):
+    """Fe I 6173.3356 A, -350:35:665 mA, optional Gaussian PSF."""
+    psf = ("gauss", psf_fwhm) if psf_fwhm else None
+    return make_setup([FE6173], GRID_6173, psf=psf, **kw)
+
+
+def setup_two_lines(psf_fwhm=None, **kw):
+    """Fe I 6301.5 + 6302.5 A on 112 samples of 21.5 mA; classical
+    estimates use the g_eff = 2.5 line."""
+    psf = ("gauss", psf_fwhm) if psf_fwhm else None
+    kw.setdefault("ce_line", 1)
+    return make_setup([FE6301, FE6302_PAIR], GRID_6302, psf=psf, **kw)
+
+
+def random_models(n, seed, weak_fraction=0.0):
+    """Per-pixel true models drawn uniformly: B 0-3000 G, gamma/phi 0-180 deg,
+    vlos -3..3 km/s, dopp 20-60 mA, a 0.01-0.5, eta0 1-20, S0 0.1-0.4,
+    S1 0.5-0.9, mac 0, alpha 1; a fraction of pixels with B < 100 G."""
+    rng = np.random.default_rng(seed)
+    m = np.zeros((n, GM_NMODEL), dtype=np.float64)
+    m[:, 0] = rng.uniform(1.0, 20.0, n)
+    m[:, 1] = rng.uniform(0.0, 3000.0, n)
+    m[:, 2] = rng.uniform(-3.0, 3.0, n)
+    m[:, 3] = rng.uniform(0.020, 0.060, n)
+    m[:, 4] = rng.uniform(0.01, 0.5, n)
+    m[:, 5] = rng.uniform(0.0, 180.0, n)
+    m[:, 6] = rng.uniform(0.0, 180.0, n)
+    m[:, 7] = rng.uniform(0.1, 0.4, n)
+    m[:, 8] = rng.uniform(0.5, 0.9, n)
+    m[:, 9] = 0.0
+    m[:, 10] = 1.0
+    if weak_fraction > 0:
+        weak = rng.random(n) < weak_fraction
+        m[weak, 1] = rng.uniform(0.0, 100.0, int(weak.sum()))
+    return m
+
+
+def noise_sigma(models, sigma=1e-3, seed=0, high_fraction=0.0, high_sigma=3e-3):
+    """Per-pixel noise sigma in units of Ic = S0 + S1 (mu = 1)."""
+    rng = np.random.default_rng(seed + 7919)
+    sig = np.full(models.shape[0], sigma)
+    if high_fraction > 0:
+        sig[rng.random(models.shape[0]) < high_fraction] = high_sigma
+    return sig * (models[:, 7] + models[:, 8])
+
+
+# ---------------------------------------------------------------------------
+class Context:
+    """Owns one gm_ctx.  Buffers are torch tensors on the current device."""
+
+    def __init__(self, setup):
+        self.lib = load_library()
+        self.setup = setup
+        h = ctypes.c_void_p()
+        _check(self.lib.gm_ctx_create(ctypes.byref(setup), ctypes.byref(h)), "gm_ctx_create")
+        self.h = h
+        self.nlambda = setup.nlambda
+
+    def close(self):
+        if self.h:
+            self.lib.gm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _stream():
+        import torch
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def set_timing(self, on):
+        _check(self.lib.gm_ctx_set_timing(self.h, 1 if on else 0), "gm_ctx_set_timing")
+
+    def stats(self):
+        st = gm_stats()
+        _check(self.lib.gm_last_stats(self.h, ctypes.byref(st)), "gm_last_stats")
+        return st.as_dict()
+
+    def invert(self, obs, init=None, want_synth=True, out=None):
+        """obs: float32 [npix, 4, nlambda] CUDA tensor.  Returns a dict of
+        tensors: model [npix, 11] f64, chisqr, iters, naccs, synth."""
+        import torch
+        npix = obs.shape[0]
+        dev = obs.device
+        if out is None:
+            out = {
+                "model": torch.empty((npix, GM_NMODEL), dtype=torch.float64, device=dev),
+                "chisqr": torch.empty(npix, dtype=torch.float32, device=dev),
+                "iters": torch.empty(npix, dtype=torch.int32, device=dev),
+                "naccs": torch.empty(npix, dtype=torch.int32, device=dev),
+            }
+            if want_synth:
+                out["synth"] = torch.empty_like(obs)
+        o = gm_out(out["model"].data_ptr(), out["chisqr"].data_ptr(), out["iters"].data_ptr(),
+                   out["synth"].data_ptr() if out.get("synth") is not None else None,
+                   out["naccs"].data_ptr())
+        assert obs.dtype == torch.float32 and obs.is_contiguous()
+        assert obs.shape[1] == 4 and obs.shape[2] == self.nlambda
+        ip = None
+        if init is not None:
+            assert init.dtype == torch.float64 and init.shape == (npix, GM_NMODEL)
+            ip = ctypes.c_void_p(init.data_ptr())
+        _check(self.lib.gm_invert(self.h, ctypes.c_void_p(obs.data_ptr()), npix, ip,
+                                  ctypes.byref(o), self._stream()), "gm_invert")
+        return out
+
+    def synth(self, models):
+        import torch
+        npix = models.shape[0]
+        sp = torch.empty((npix, 4, self.nlambda), dtype=torch.float32, device=models.device)
+        _check(self.lib.gm_synth(self.h, ctypes.c_void_p(models.data_ptr()), npix,
+                                 ctypes.c_void_p(sp.data_ptr()), self._stream()), "gm_synth")
+        return sp
+
+    def synth_rf(self, models):
+        import torch
+        npix = models.shape[0]
+        sp = torch.empty((npix, 4, self.nlambda), dtype=torch.float32, device=models.device)
+        rf = torch.empty((npix, 4, 10, self.nlambda), dtype=torch.float32, device=models.device)
+        _check(self.lib.gm_synth_rf(self.h, ctypes.c_void_p(models.data_ptr()), npix,
+                                    ctypes.c_void_p(sp.data_ptr()), ctypes.c_void_p(rf.data_ptr()),
+                                    self._stream()), "gm_synth_rf")
+        return sp, rf
+
+    def normal_equations(self, models, obs):
+        import torch
+        npix = models.shape[0]
+        dev = models.device
+        a = torch.empty((npix, 45), dtype=torch.float32, device=dev)
+        b = torch.empty((npix, 9), dtype=torch.float32, device=dev)
+        c = torch.empty(npix, dtype=torch.float64, device=dev)
+        _check(self.lib.gm_normal_equations(self.h, ctypes.c_void_p(models.data_ptr()),
+                                            ctypes.c_void_p(obs.data_ptr()), npix,
+                                            ctypes.c_void_p(a.data_ptr()),
+                                            ctypes.c_void_p(b.data_ptr()),
+                                            ctypes.c_void_p(c.data_ptr()), self._stream()),
+               "gm_normal_equations")
+        return a, b, c
+
+    def step_solve(self, alpha, beta, flambda):
+        import torch
+        npix = alpha.shape[0]
+        d = torch.empty((npix, 9), dtype=torch.float64, device=alpha.device)
+        _check(self.lib.gm_step_solve(self.h, ctypes.c_void_p(alpha.data_ptr()),
+                                      ctypes.c_void_p(beta.data_ptr()),
+                                      ctypes.c_void_p(flambda.data_ptr()), npix,
+                                      ctypes.c_void_p(d.data_ptr()), self._stream()),
+               "gm_step_solve")
+        return d
+
+    def classical_estimates(self, obs):
+        import torch
+        npix = obs.shape[0]
+        m = torch.empty((npix, GM_NMODEL), dtype=torch.float64, device=obs.device)
+        _check(self.lib.gm_classical_estimates(self.h, ctypes.c_void_p(obs.data_ptr()), npix,
+                                               ctypes.c_void_p(m.data_ptr()), self._stream()),
+               "gm_classical_estimates")
+        return m
+
+
+def packed_index(p, q):
+    """Index of (p, q) in the packed 9x9 upper triangle."""
+    if p > q:
+        p, q = q, p
+    return p * 9 - (p * (p - 1)) // 2 + (q - p)
+
+
+def unpack_alpha(a):
+    """[..., 45] packed upper triangle -> [..., 9, 9] symmetric."""
+    a = np.asarray(a)
+    out = np.zeros(a.shape[:-1] + (9, 9), dtype=a.dtype)
+    for p in range(9):
+        for q in range(p, 9):
+            out[..., p, q] = a[..., packed_index(p, q)]
+            out[..., q, p] = a[..., packed_index(p, q)]
+    return out
+
+
+def build(verbose=False):
+    """Compiles libgmilos.so (and the CLI when its sources exist)."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("gmilos_amd_build",
+                                                  os.path.join(HERE, "build.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod.build(verbose)

@@ -1,0 +1,493 @@
+/*
+ * gm_capi.cpp -- C ABI of the inversion engine (gmilos.h): contexts, device
+ * set-up tables, workspace and the host-side LM round loop.
+ *
+ * The reference drives its engine from gmilos.cu:914-1322: constants go to
+ * __constant__ symbols, the cube is split over streams and threads, and one
+ * kernel runs the whole per-pixel LM with device-side child launches.  Here a
+ * context holds the set-up in device memory, the LM state lives in HBM as
+ * structure-of-arrays, and each LM round is four stream-ordered launches over
+ * compacted pixel lists: step solve -> synthesis + chi^2 -> accept/reject ->
+ * response functions + normal equations for accepted pixels.
+ */
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "gm_internal.h"
+
+namespace {
+
+constexpr int kMaxRounds = 1024;
+
+struct Events {
+    hipEvent_t a = nullptr, b = nullptr;
+};
+
+} // namespace
+
+struct gm_ctx {
+    gm_setup setup;
+    gm_dsetup host_ds;
+    gm_dsetup *dS = nullptr;
+    gm_lanecfg lc;
+    int device = 0;
+    /* LM state workspace */
+    gm_state st;
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+    /* scratch for the piecewise API */
+    float *scr_beta = nullptr;
+    int64_t scr_cap = 0;
+    /* round bookkeeping */
+    int32_t *h_counts = nullptr;     /* pinned [kMaxRounds][3]                */
+    hipEvent_t ev_round[kMaxRounds];
+    int ev_round_n = 0;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    int timing = 0;
+    std::vector<Events> kev;         /* per-kernel events when timing is on   */
+    gm_stats stats;
+};
+
+static gm_status hip_ok(hipError_t e)
+{
+    if (e == hipSuccess) return GM_OK;
+    return (e == hipErrorOutOfMemory) ? GM_ERR_NOMEM : GM_ERR_HIP;
+}
+
+#define GM_TRY(x)                                                              \
+    do {                                                                       \
+        gm_status _s = (x);                                                    \
+        if (_s != GM_OK) return _s;                                            \
+    } while (0)
+#define GM_HIP(x) GM_TRY(hip_ok(x))
+
+extern "C" const char *gm_status_string(gm_status st)
+{
+    switch (st) {
+    case GM_OK: return "ok";
+    case GM_ERR_INVALID: return "invalid argument";
+    case GM_ERR_HIP: return "HIP runtime error";
+    case GM_ERR_NOMEM: return "out of device memory";
+    case GM_ERR_UNSUPPORTED: return "unsupported configuration";
+    case GM_ERR_NOT_FOUND: return "not found";
+    }
+    return "unknown status";
+}
+
+gm_lanecfg gm_lane_config(const gm_dsetup &h)
+{
+    gm_lanecfg lc;
+    const int n = h.nlambda;
+    if (n <= 32) { lc.lpp = 32; lc.nlpl = 1; }
+    else if (n <= 64) { lc.lpp = 64; lc.nlpl = 1; }
+    else if (n <= 128) { lc.lpp = 64; lc.nlpl = 2; }
+    else { lc.lpp = 64; lc.nlpl = 4; }
+    lc.ppw = 64 / lc.lpp;
+    const int padlo = h.psf_dlo < 0 ? -h.psf_dlo : 0;
+    const int padhi = h.psf_dhi > 0 ? h.psf_dhi : 0;
+    lc.padlo = padlo;
+    lc.rows = lc.lpp * lc.nlpl + padlo + padhi;
+    return lc;
+}
+
+/* Builds the device tables.  The PSF correlation of the reference
+ * (convolution.cu:153-241) applies all nlambda taps centred at nlambda/2;
+ * taps whose combined tail mass is below psf_tail_tol are dropped, which
+ * changes results by less than that fraction of the signal.               */
+static gm_status build_dsetup(const gm_setup &s, gm_dsetup &d)
+{
+    std::memset(&d, 0, sizeof d);
+    const int n = s.nlambda;
+    if (n < 2 || n > GM_MAX_LAMBDA) return GM_ERR_INVALID;
+    if (s.nlines < 1 || s.nlines > GM_MAX_LINES) return GM_ERR_INVALID;
+    if (!(s.noise > 0)) return GM_ERR_INVALID;
+    if (s.miter < 0 || s.miter > kMaxRounds) return GM_ERR_INVALID;
+    if (s.fix[9] || s.fix[10]) return GM_ERR_UNSUPPORTED;  /* mac / stray light */
+    if (s.init_model[9] > 0.0001) return GM_ERR_UNSUPPORTED; /* macroturbulence */
+    if (s.init_model[10] < 1.0) return GM_ERR_UNSUPPORTED;   /* filling factor  */
+    if (s.ce_line < 0 || s.ce_line >= s.nlines) return GM_ERR_INVALID;
+    if (s.precision != GM_PREC_FP32 && s.precision != GM_PREC_MIXED) return GM_ERR_INVALID;
+    if (s.solver != GM_SOLVE_SVD && s.solver != GM_SOLVE_LDL) return GM_ERR_INVALID;
+    d.nlambda = n;
+    d.nlines = s.nlines;
+    for (int L = 0; L < s.nlines; ++L) {
+        const gm_zeeman &z = s.zee[L];
+        if (z.n_pi < 0 || z.n_pi > GM_MAX_COMP || z.n_sig < 0 || z.n_sig > GM_MAX_COMP)
+            return GM_ERR_INVALID;
+        d.ncomp[L][0] = z.n_pi;
+        d.ncomp[L][1] = z.n_sig;
+        d.ncomp[L][2] = z.n_sig;
+        for (int i = 0; i < z.n_pi; ++i) {
+            d.comp_nu[L][0][i] = (float)z.nup[i];
+            d.comp_w[L][0][i] = (float)z.wep[i];
+        }
+        for (int i = 0; i < z.n_sig; ++i) {
+            d.comp_nu[L][1][i] = (float)z.nub[i];
+            d.comp_w[L][1][i] = (float)z.web[i];
+            /* red shifts: the reference mirrors the blue ones
+             * (nurB[i] = -nubB[N_SIG-1-i], mil_sinrf.cu:74-76) */
+            d.comp_nu[L][2][i] = (float)(-z.nub[z.n_sig - 1 - i]);
+            d.comp_w[L][2][i] = (float)z.wer[i];
+        }
+        d.l0[L] = (float)s.lambda0[L];
+        d.fo[L] = (float)z.fo;
+        for (int k = 0; k < n; ++k) d.dl[L][k] = (float)(s.lambda[k] - s.lambda0[L]);
+    }
+    d.use_psf = s.use_psf ? 1 : 0;
+    const int c = n / 2;
+    if (d.use_psf) {
+        double tot = 0;
+        for (int j = 0; j < n; ++j) tot += std::fabs(s.psf[j]);
+        if (!(tot > 0)) return GM_ERR_INVALID;
+        int jlo = 0, jhi = n - 1;
+        double budget = s.psf_tail_tol > 0 ? s.psf_tail_tol * tot : 0.0;
+        double cut = 0;
+        while (jlo < jhi) {
+            const double lo = std::fabs(s.psf[jlo]), hi = std::fabs(s.psf[jhi]);
+            const double take = lo <= hi ? lo : hi;
+            if (cut + take > budget) break;
+            cut += take;
+            if (lo <= hi) ++jlo; else --jhi;
+        }
+        d.psf_dlo = jlo - c;
+        d.psf_dhi = jhi - c;
+        for (int j = jlo; j <= jhi; ++j) d.psf_tap[j - jlo] = (float)s.psf[j];
+        for (int k = 0; k < n; ++k) {
+            double r = 0;
+            for (int j = 0; j < n; ++j) {
+                const int m = k + j - c;
+                if (m >= 0 && m < n) r += s.psf[j];
+            }
+            d.psf_edge[k] = (float)(1.0 - r);
+        }
+    } else {
+        d.psf_dlo = d.psf_dhi = 0;
+    }
+    for (int i = 0; i < 4; ++i) {
+        d.weight[i] = (float)s.weight[i];
+        d.wsig[i] = (float)(s.weight[i] / s.noise);
+    }
+    d.mu = (float)s.mu;
+    d.shi_scale = s.rf_reference_rules ? 1.0f : 2.0f;
+    d.freemask = 0;
+    for (int p = 0; p < 9; ++p)
+        if (s.fix[p]) d.freemask |= (1u << p);
+    for (int p = 0; p < 11; ++p) d.fix[p] = s.fix[p];
+    d.nfree_chi = 4 * n - 10;   /* nfree = nlambda*NPARMS - NTERMS, milosUtils.cu:585 */
+    d.noise = s.noise;
+    d.ilambda = s.ilambda;
+    d.toplim = s.toplim;
+    d.miter = s.miter;
+    d.logclambda = s.logclambda;
+    for (int f = 0; f < GM_NMODEL; ++f) d.init_model[f] = s.init_model[f];
+    for (int k = 0; k < n; ++k) d.lambda[k] = s.lambda[k];
+    /* classical-estimate range: the whole grid for one line, else the
+     * samples nearest to the chosen line */
+    d.ce_lo = 0;
+    d.ce_hi = n;
+    if (s.nlines > 1) {
+        int first = -1, last = -1;
+        for (int k = 0; k < n; ++k) {
+            int best = 0;
+            for (int L = 1; L < s.nlines; ++L)
+                if (std::fabs(s.lambda[k] - s.lambda0[L]) < std::fabs(s.lambda[k] - s.lambda0[best]))
+                    best = L;
+            if (best == s.ce_line) {
+                if (first < 0) first = k;
+                last = k;
+            }
+        }
+        if (first >= 0) { d.ce_lo = first; d.ce_hi = last + 1; }
+    }
+    d.ce_l0 = s.lambda0[s.ce_line];
+    d.ce_geff = s.zee[s.ce_line].geff;
+    return GM_OK;
+}
+
+extern "C" gm_status gm_ctx_create(const gm_setup *setup, gm_ctx **out)
+{
+    if (!setup || !out) return GM_ERR_INVALID;
+    *out = nullptr;
+    gm_ctx *c = new (std::nothrow) gm_ctx();
+    if (!c) return GM_ERR_NOMEM;
+    c->setup = *setup;
+    gm_status s = build_dsetup(*setup, c->host_ds);
+    if (s != GM_OK) { delete c; return s; }
+    c->lc = gm_lane_config(c->host_ds);
+    if (hipGetDevice(&c->device) != hipSuccess) { delete c; return GM_ERR_HIP; }
+    if (hipMalloc(&c->dS, sizeof(gm_dsetup)) != hipSuccess) { delete c; return GM_ERR_NOMEM; }
+    if (hipMemcpy(c->dS, &c->host_ds, sizeof(gm_dsetup), hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(c->dS);
+        delete c;
+        return GM_ERR_HIP;
+    }
+    if (hipHostMalloc((void **)&c->h_counts, sizeof(int32_t) * 3 * (kMaxRounds + 1)) != hipSuccess) {
+        hipFree(c->dS);
+        delete c;
+        return GM_ERR_NOMEM;
+    }
+    for (int i = 0; i < kMaxRounds; ++i) c->ev_round[i] = nullptr;
+    hipEventCreate(&c->ev_start);
+    hipEventCreate(&c->ev_stop);
+    std::memset(&c->st, 0, sizeof c->st);
+    std::memset(&c->stats, 0, sizeof c->stats);
+    *out = c;
+    return GM_OK;
+}
+
+extern "C" gm_status gm_ctx_destroy(gm_ctx *c)
+{
+    if (!c) return GM_ERR_INVALID;
+    hipDeviceSynchronize();
+    if (c->ws) hipFree(c->ws);
+    if (c->scr_beta) hipFree(c->scr_beta);
+    if (c->dS) hipFree(c->dS);
+    if (c->h_counts) hipHostFree(c->h_counts);
+    for (int i = 0; i < c->ev_round_n; ++i) hipEventDestroy(c->ev_round[i]);
+    for (auto &e : c->kev) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
+    if (c->ev_start) hipEventDestroy(c->ev_start);
+    if (c->ev_stop) hipEventDestroy(c->ev_stop);
+    delete c;
+    return GM_OK;
+}
+
+extern "C" gm_status gm_ctx_set_timing(gm_ctx *c, int32_t enable)
+{
+    if (!c) return GM_ERR_INVALID;
+    c->timing = enable ? 1 : 0;
+    return GM_OK;
+}
+
+extern "C" gm_status gm_last_stats(const gm_ctx *c, gm_stats *s)
+{
+    if (!c || !s) return GM_ERR_INVALID;
+    *s = c->stats;
+    return GM_OK;
+}
+
+/* Carves the SoA state out of one allocation sized for `cap` pixels. */
+static gm_status ensure_state(gm_ctx *c, int64_t npix)
+{
+    if (npix <= c->st.cap) return GM_OK;
+    int64_t cap = ((npix + 255) / 256) * 256;
+    const size_t per = sizeof(double) * (2 * GM_NMODEL + 5) + sizeof(float) * (GM_ALPHA_N + 9) +
+                       sizeof(int32_t) * (2 + 3);
+    const size_t bytes = per * (size_t)cap + 256 * 16 + 64;
+    if (c->ws) {
+        hipDeviceSynchronize();
+        hipFree(c->ws);
+        c->ws = nullptr;
+        c->st.cap = 0;
+    }
+    GM_HIP(hipMalloc(&c->ws, bytes));
+    c->ws_bytes = bytes;
+    char *p = (char *)c->ws;
+    auto take = [&](size_t b) {
+        char *r = p;
+        p += (b + 255) & ~(size_t)255;
+        return (void *)r;
+    };
+    gm_state &st = c->st;
+    st.cap = cap;
+    st.best = (double *)take(sizeof(double) * GM_NMODEL * cap);
+    st.trial = (double *)take(sizeof(double) * GM_NMODEL * cap);
+    st.alpha = (float *)take(sizeof(float) * GM_ALPHA_N * cap);
+    st.beta = (float *)take(sizeof(float) * 9 * cap);
+    st.chi_trial = (double *)take(sizeof(double) * cap);
+    st.ochisqr = (double *)take(sizeof(double) * cap);
+    st.chisqr0 = (double *)take(sizeof(double) * cap);
+    st.flambda = (double *)take(sizeof(double) * cap);
+    st.factor = (double *)take(sizeof(double) * cap);
+    st.iter = (int32_t *)take(sizeof(int32_t) * cap);
+    st.nacc = (int32_t *)take(sizeof(int32_t) * cap);
+    for (int i = 0; i < 3; ++i) st.list[i] = (int32_t *)take(sizeof(int32_t) * cap);
+    st.counts = (int32_t *)take(sizeof(int32_t) * 4);
+    return GM_OK;
+}
+
+static hipEvent_t round_event(gm_ctx *c, int r)
+{
+    while (c->ev_round_n <= r) {
+        hipEventCreateWithFlags(&c->ev_round[c->ev_round_n], hipEventDisableTiming);
+        c->ev_round_n++;
+    }
+    return c->ev_round[r];
+}
+
+static Events &kernel_events(gm_ctx *c, size_t i)
+{
+    while (c->kev.size() <= i) {
+        Events e;
+        hipEventCreate(&e.a);
+        hipEventCreate(&e.b);
+        c->kev.push_back(e);
+    }
+    return c->kev[i];
+}
+
+extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
+                               const double *init, const gm_out *out, void *stream)
+{
+    if (!c || !obs || !out || npix < 0) return GM_ERR_INVALID;
+    if (npix > (int64_t)0x7fffffff) return GM_ERR_INVALID;
+    if (c->setup.init_mode == GM_INIT_PER_PIXEL && !init) return GM_ERR_INVALID;
+    std::memset(&c->stats, 0, sizeof c->stats);
+    c->stats.npix = npix;
+    c->stats.psf_taps = c->host_ds.use_psf ? (c->host_ds.psf_dhi - c->host_ds.psf_dlo + 1) : 0;
+    if (npix == 0) return GM_OK;
+    GM_TRY(ensure_state(c, npix));
+    hipStream_t s = (hipStream_t)stream;
+    const int mixed = c->setup.precision == GM_PREC_MIXED;
+    const int solver = c->setup.solver;
+    gm_state st = c->st;
+    const gm_lanecfg &lc = c->lc;
+    int32_t *cnt = st.counts;
+    std::vector<int> kclass;   /* 0 synth, 1 rf, 2 solve, 3 other */
+    size_t kidx = 0;
+    auto kbegin = [&](int cls) {
+        if (!c->timing) return;
+        hipEventRecord(kernel_events(c, kidx).a, s);
+        kclass.push_back(cls);
+    };
+    auto kend = [&]() {
+        if (!c->timing) return;
+        hipEventRecord(kernel_events(c, kidx).b, s);
+        ++kidx;
+    };
+
+    GM_HIP(hipEventRecord(c->ev_start, s));
+    kbegin(3);
+    gm_launch_init_state(c->dS, st, obs, init, c->setup.init_mode, npix, st.list[0], mixed, s);
+    gm_launch_set_count(&cnt[0], (int32_t)npix, s);
+    kend();
+    kbegin(1);
+    gm_launch_rf(lc, c->dS, st.best, st.cap, obs, st.list[0], &cnt[0], 0, npix, st.alpha,
+                 st.beta, st.cap, st.ochisqr, out->synth, nullptr, mixed, s);
+    kend();
+    kbegin(3);
+    gm_launch_init_chi(st, npix, s);
+    kend();
+    int cur = 0, rounds = 0;
+    const int miter = c->setup.miter > 0 ? c->setup.miter : 1;  /* do-while */
+    for (int r = 0; r < miter; ++r) {
+        const int nxt = 1 - cur;
+        kbegin(2);
+        gm_launch_solve(c->dS, st, st.list[cur], &cnt[cur], npix, mixed, solver, &cnt[nxt],
+                        &cnt[2], s);
+        kend();
+        kbegin(0);
+        gm_launch_synth(lc, c->dS, st.trial, st.cap, obs, st.list[cur], &cnt[cur], 0, npix,
+                        st.chi_trial, out->synth, mixed, s);
+        kend();
+        kbegin(3);
+        gm_launch_decide(c->dS, st, st.list[cur], &cnt[cur], npix, st.list[nxt], &cnt[nxt],
+                         st.list[2], &cnt[2], mixed, s);
+        kend();
+        kbegin(1);
+        gm_launch_rf(lc, c->dS, st.best, st.cap, obs, st.list[2], &cnt[2], 0, npix, st.alpha,
+                     st.beta, st.cap, nullptr, nullptr, nullptr, mixed, s);
+        kend();
+        /* per-round counts: [active this round, active next round, accepted] */
+        int32_t *hc = c->h_counts + 3 * r;
+        GM_HIP(hipMemcpyAsync(hc, &cnt[cur], sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        GM_HIP(hipMemcpyAsync(hc + 1, &cnt[nxt], sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        GM_HIP(hipMemcpyAsync(hc + 2, &cnt[2], sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        GM_HIP(hipEventRecord(round_event(c, r), s));
+        cur = nxt;
+        rounds = r + 1;
+        /* stop issuing rounds once a completed round reports no survivors */
+        if (r >= 2) {
+            hipEvent_t e = c->ev_round[r - 2];
+            if (hipEventQuery(e) == hipSuccess && c->h_counts[3 * (r - 2) + 1] == 0) break;
+        }
+    }
+    kbegin(3);
+    gm_launch_finalize(st, npix, *out, s);
+    kend();
+    GM_HIP(hipEventRecord(c->ev_stop, s));
+    GM_HIP(hipGetLastError());
+    GM_HIP(hipEventSynchronize(c->ev_stop));
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->ev_start, c->ev_stop);
+    c->stats.ms_total = ms;
+    c->stats.lm_rounds = rounds;
+    c->stats.synth_evals = npix;
+    c->stats.rf_evals = npix;
+    for (int r = 0; r < rounds; ++r) {
+        const int32_t *hc = c->h_counts + 3 * r;
+        c->stats.solves += hc[0];
+        c->stats.synth_evals += hc[0];
+        c->stats.rf_evals += hc[2];
+    }
+    if (c->timing) {
+        for (size_t i = 0; i < kidx; ++i) {
+            float t = 0;
+            hipEventElapsedTime(&t, c->kev[i].a, c->kev[i].b);
+            switch (kclass[i]) {
+            case 0: c->stats.ms_synth += t; break;
+            case 1: c->stats.ms_rf += t; break;
+            case 2: c->stats.ms_solve += t; break;
+            default: c->stats.ms_other += t; break;
+            }
+        }
+    }
+    return GM_OK;
+}
+
+/* ---- path pieces ---------------------------------------------------------- */
+extern "C" gm_status gm_synth(gm_ctx *c, const double *models, int64_t npix, float *spectra,
+                              void *stream)
+{
+    if (!c || !models || !spectra || npix < 0) return GM_ERR_INVALID;
+    if (npix == 0) return GM_OK;
+    gm_launch_synth(c->lc, c->dS, models, 0, nullptr, nullptr, nullptr, npix, npix, nullptr,
+                    spectra, 0, (hipStream_t)stream);
+    return hip_ok(hipGetLastError());
+}
+
+extern "C" gm_status gm_synth_rf(gm_ctx *c, const double *models, int64_t npix, float *spectra,
+                                 float *rf, void *stream)
+{
+    if (!c || !models || npix < 0) return GM_ERR_INVALID;
+    if (npix == 0) return GM_OK;
+    gm_launch_rf(c->lc, c->dS, models, 0, nullptr, nullptr, nullptr, npix, npix, nullptr, nullptr,
+                 0, nullptr, spectra, rf, 0, (hipStream_t)stream);
+    return hip_ok(hipGetLastError());
+}
+
+extern "C" gm_status gm_normal_equations(gm_ctx *c, const double *models, const float *obs,
+                                         int64_t npix, float *alpha, float *beta, double *chisqr,
+                                         void *stream)
+{
+    if (!c || !models || !obs || npix < 0) return GM_ERR_INVALID;
+    if (npix == 0) return GM_OK;
+    const int mixed = c->setup.precision == GM_PREC_MIXED;
+    gm_launch_rf(c->lc, c->dS, models, 0, obs, nullptr, nullptr, npix, npix, alpha, beta, 0,
+                 chisqr, nullptr, nullptr, mixed, (hipStream_t)stream);
+    return hip_ok(hipGetLastError());
+}
+
+extern "C" gm_status gm_step_solve(gm_ctx *c, const float *alpha, const float *beta,
+                                   const double *flambda, int64_t npix, double *delta,
+                                   void *stream)
+{
+    if (!c || !alpha || !beta || !flambda || !delta || npix < 0) return GM_ERR_INVALID;
+    if (npix == 0) return GM_OK;
+    gm_launch_step_solve(alpha, beta, flambda, npix, delta, c->setup.precision == GM_PREC_MIXED,
+                         c->setup.solver, (hipStream_t)stream);
+    return hip_ok(hipGetLastError());
+}
+
+extern "C" gm_status gm_classical_estimates(gm_ctx *c, const float *obs, int64_t npix,
+                                            double *models, void *stream)
+{
+    if (!c || !obs || !models || npix < 0) return GM_ERR_INVALID;
+    if (npix == 0) return GM_OK;
+    gm_launch_ce(c->dS, obs, npix, models, (hipStream_t)stream);
+    return hip_ok(hipGetLastError());
+}

@@ -1,0 +1,110 @@
+/*
+ * gm_internal.h -- device-side set-up tables and kernel launchers shared by
+ * the C ABI (gm_capi.cpp) and the kernels (gm_kernels.hip).
+ */
+#ifndef GM_INTERNAL_H
+#define GM_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "gmilos.h"
+
+#define GM_ALPHA_N   45          /* packed upper triangle of the 9x9 system */
+#define GM_RED_N     55          /* 45 alpha + 9 beta + 1 chi^2             */
+#define GM_SLOTS     36          /* staged values per wavelength: per Stokes
+                                    [synth, eta0, B, vlos, dopp, a, gamma, phi,
+                                    S1]                                       */
+#define GM_CPT       (GM_MAX_LINES * GM_MAX_COMP)
+
+/* Device copy of the set-up, built once per context (read-only). */
+struct gm_dsetup {
+    int32_t nlambda;
+    int32_t nlines;
+    int32_t ncomp[GM_MAX_LINES][3];          /* sub-components per type     */
+    float   comp_nu[GM_MAX_LINES][3][GM_MAX_COMP];
+    float   comp_w[GM_MAX_LINES][3][GM_MAX_COMP];
+    float   l0[GM_MAX_LINES];                /* line centres (A)            */
+    float   fo[GM_MAX_LINES];                /* relative line strengths     */
+    float   dl[GM_MAX_LINES][GM_MAX_LAMBDA]; /* lambda - lambda0 (A)        */
+    int32_t use_psf;
+    int32_t psf_dlo, psf_dhi;                /* applied tap offsets d=j-c   */
+    float   psf_tap[GM_MAX_LAMBDA];          /* tap for offset d: [d-dlo]   */
+    float   psf_edge[GM_MAX_LAMBDA];         /* 1 - sum of in-range taps    */
+    float   weight[4];
+    float   wsig[4];                         /* weight / noise              */
+    float   mu;
+    float   shi_scale;                       /* 1 reference rule, 2 exact   */
+    uint32_t freemask;                       /* bit p: RF slot p free (0-8) */
+    int32_t fix[11];
+    int32_t nfree_chi;                       /* 4*nlambda - 10              */
+    double  noise;
+    double  ilambda, toplim;
+    int32_t miter, logclambda;
+    double  init_model[GM_NMODEL];
+    /* classical estimates */
+    int32_t ce_lo, ce_hi;
+    double  ce_l0, ce_geff;
+    double  lambda[GM_MAX_LAMBDA];           /* absolute wavelengths (A)    */
+};
+
+/* Per-pixel LM state, structure-of-arrays over the pixel capacity. */
+struct gm_state {
+    int64_t  cap;
+    double  *best;        /* [11][cap] best accepted model                  */
+    double  *trial;       /* [11][cap] last proposed model                  */
+    float   *alpha;       /* [45][cap] normal matrix at best                */
+    float   *beta;        /* [9][cap]                                        */
+    double  *chi_trial;   /* [cap] chi^2 of trial                           */
+    double  *ochisqr;     /* [cap] chi^2 of best                            */
+    double  *chisqr0;     /* [cap] initial chi^2                            */
+    double  *flambda;     /* [cap] Marquardt lambda                          */
+    double  *factor;      /* [cap] PARBETA_FACTOR (logclambda)              */
+    int32_t *iter;        /* [cap]                                           */
+    int32_t *nacc;        /* [cap]                                           */
+    int32_t *list[3];     /* two active lists (ping-pong) + accepted list    */
+    int32_t *counts;      /* [4]: list counts                                */
+};
+
+/* Lane mapping of the wavelength-parallel kernels. */
+struct gm_lanecfg {
+    int lpp;   /* lanes per pixel (32 or 64)                                 */
+    int nlpl;  /* wavelengths per lane                                       */
+    int ppw;   /* pixels per 64-lane workgroup                               */
+    int rows;  /* staged rows per pixel incl. zero padding                   */
+    int padlo; /* zero rows before wavelength 0                              */
+};
+
+gm_lanecfg gm_lane_config(const gm_dsetup &h);
+
+/* Launchers (gm_kernels.hip).  `list` may be NULL (identity) with the item
+ * count given by *count (device) or by n_direct when count is NULL.          */
+void gm_launch_rf(const gm_lanecfg &lc, const gm_dsetup *dS, const double *models,
+                  int64_t model_stride, const float *obs, const int32_t *list,
+                  const int32_t *count, int64_t n_direct, int64_t grid_items,
+                  float *alpha, float *beta, int64_t ab_stride, double *chi_out,
+                  float *synth_out, float *rf_out, int mixed, hipStream_t st);
+void gm_launch_synth(const gm_lanecfg &lc, const gm_dsetup *dS, const double *models,
+                     int64_t model_stride, const float *obs, const int32_t *list,
+                     const int32_t *count, int64_t n_direct, int64_t grid_items,
+                     double *chi_out, float *synth_out, int mixed, hipStream_t st);
+void gm_launch_solve(const gm_dsetup *dS, gm_state st, const int32_t *list,
+                     const int32_t *count, int64_t grid_items, int mixed, int solver,
+                     int32_t *reset0, int32_t *reset1, hipStream_t stream);
+void gm_launch_step_solve(const float *alpha, const float *beta, const double *flambda,
+                          int64_t n, double *delta, int mixed, int solver,
+                          hipStream_t stream);
+void gm_launch_decide(const gm_dsetup *dS, gm_state st, const int32_t *list,
+                      const int32_t *count, int64_t grid_items, int32_t *next_list,
+                      int32_t *next_count, int32_t *acc_list, int32_t *acc_count,
+                      int mixed, hipStream_t stream);
+void gm_launch_init_state(const gm_dsetup *dS, gm_state st, const float *obs,
+                          const double *init, int init_mode, int64_t npix,
+                          int32_t *list0, int mixed, hipStream_t stream);
+void gm_launch_init_chi(gm_state st, int64_t npix, hipStream_t stream);
+void gm_launch_finalize(gm_state st, int64_t npix, const gm_out &out,
+                        hipStream_t stream);
+void gm_launch_ce(const gm_dsetup *dS, const float *obs, int64_t npix,
+                  double *models, hipStream_t stream);
+void gm_launch_set_count(int32_t *count, int32_t value, hipStream_t stream);
+
+#endif

@@ -1,0 +1,1213 @@
+/*
+ * gm_kernels.hip -- gfx950 kernels of the per-pixel Levenberg-Marquardt path.
+ *
+ * Two lane mappings:
+ *  - wavelength-parallel ("lambda lanes"): 32 or 64 lanes own one pixel, each
+ *    lane one or more wavelengths.  Used for synthesis, response functions,
+ *    PSF correlation and normal equations, whose per-wavelength state is large
+ *    and whose cross-wavelength coupling (PSF, sums) goes through LDS.
+ *  - pixel-parallel ("pixel lanes"): one lane owns one pixel.  Used for the
+ *    step solve, the accept/reject decision, the classical estimates and state
+ *    bookkeeping, which are serial per pixel.
+ * LM state lives in HBM as structure-of-arrays between kernels; active pixels
+ * are carried in compacted lists so retired pixels cost nothing.
+ *
+ * Reference routines restated here: mil_sinrf.cu:27-433 (synthesis),
+ * me_der.cu:35-376 (response functions), convolution.cu:153-241 (PSF),
+ * lib.cu:65-155 (normal equations, chi^2), milosUtils.cu:30-238 (step
+ * application and clamps), milosUtils.cu:280-365 (pseudo-inverse solve),
+ * milosUtils.cu:386-563 (classical estimates), milosUtils.cu:574-710 (LM).
+ */
+#include "gm_internal.h"
+#include <type_traits>
+
+namespace {
+
+constexpr float kVLIGHT = 2.99792458e+5f;   /* defines.h:80 */
+constexpr float kCTE = 4.6686411e-13f;      /* defines.h:82 */
+constexpr float kRR = 0.5641895836f;        /* defines.h:78 */
+constexpr double kPI = 3.14159265358979323846264338327950288;
+
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+/* ------------------------------------------------------------------------ */
+/* Voigt and Faraday-Voigt functions, FP32: Hui, Armstrong & Wray (1978)
+ * rational approximation w(z) ~ P(Z)/Q(Z), Z = a - i u, evaluated by complex
+ * Horner and a scaled complex division (no |Q|^2 overflow for large |u|).   */
+__device__ __forceinline__ void voigt32(float a, float u, float &H, float &F)
+{
+    constexpr float P[7] = {122.607931777104326f, 214.382388694706425f,
+                            181.928533092181549f, 93.155580458138441f,
+                            30.180142196210589f, 5.912626209773153f,
+                            0.564189583562615f};
+    constexpr float Q[8] = {122.60793177387535f, 352.730625110963558f,
+                            457.334478783897737f, 348.703917719495792f,
+                            170.354001821091472f, 53.992906912940207f,
+                            10.479857114260399f, 1.0f};
+    const float zr = a, zi = -u;
+    float pr = P[6], pi = 0.f;
+#pragma unroll
+    for (int k = 5; k >= 0; --k) {
+        const float tr = pr * zr - pi * zi + P[k];
+        const float ti = pr * zi + pi * zr;
+        pr = tr;
+        pi = ti;
+    }
+    float qr = Q[7], qi = 0.f;
+#pragma unroll
+    for (int k = 6; k >= 0; --k) {
+        const float tr = qr * zr - qi * zi + Q[k];
+        const float ti = qr * zi + qi * zr;
+        qr = tr;
+        qi = ti;
+    }
+    const float s = frcp(fmaxf(fabsf(qr), fabsf(qi)));
+    const float qrs = qr * s, qis = qi * s;
+    const float inv = frcp(qrs * qrs + qis * qis) * s;
+    H = (pr * qrs + pi * qis) * inv;
+    F = 0.5f * (pi * qrs - pr * qis) * inv;
+}
+
+/* Model-dependent constants shared by all wavelengths of a pixel
+ * (mil_sinrf.cu:38-60).  gamma/phi derivatives are per degree.              */
+struct PixPre {
+    float B, aa, inv_dopp, vlos, S0, S1, E0, E0_2;
+    float sinis, cosi, cosis_2, sina, cosa, sinis_cosa, sinis_sina;
+    float sinda, cosda, sindi, cosdi;
+};
+
+__device__ __forceinline__ PixPre make_pre(const double *m)
+{
+    PixPre p;
+    const float CC = (float)(kPI / 180.0);
+    float sin_gm, cosi, sina, cosa;
+    sincosf((float)(m[5] * (kPI / 180.0)), &sin_gm, &cosi);
+    sincosf((float)(2.0 * m[6] * (kPI / 180.0)), &sina, &cosa);
+    p.B = (float)m[1];
+    p.aa = (float)m[4];
+    p.inv_dopp = (float)(1.0 / m[3]);
+    p.vlos = (float)m[2];
+    p.S0 = (float)m[7];
+    p.S1 = (float)m[8];
+    p.E0 = (float)m[0];
+    p.E0_2 = p.E0 * 0.5f;
+    p.sinis = sin_gm * sin_gm;
+    p.cosi = cosi;
+    p.cosis_2 = (1.f + cosi * cosi) * 0.5f;
+    p.sina = sina;
+    p.cosa = cosa;
+    p.sinis_cosa = p.sinis * cosa;
+    p.sinis_sina = p.sinis * sina;
+    p.sinda = cosa * (2.f * CC);
+    p.cosda = -sina * (2.f * CC);
+    p.sindi = cosi * sin_gm * (2.f * CC);
+    p.cosdi = -sin_gm * CC;
+    return p;
+}
+
+/* Unconvolved synthesis (and response functions) at one wavelength k.
+ * out[s*9 + j], s = Stokes, j = 0 synth, 1..6 d/d{eta0,B,vlos,dopp,a,gamma},
+ * 7 d/dphi, 8 d/dS1.  Follows mil_sinrf.cu:144-433 and me_der.cu:35-376; the
+ * Voigt derivatives are summed over the sub-components of each type.        */
+template <bool RF>
+__device__ __forceinline__ void eval_lambda(const gm_dsetup *__restrict__ S,
+                                            const PixPre &pp, int k, float *out)
+{
+    float fi[3] = {0.f, 0.f, 0.f}, shi[3] = {0.f, 0.f, 0.f};
+    float dfi[3][4], dshi[3][4];
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) { dfi[t][j] = 0.f; dshi[t][j] = 0.f; }
+    const float shs = S->shi_scale;
+    const int nlines = S->nlines;
+    for (int L = 0; L < nlines; ++L) {
+        const float l0 = S->l0[L], fo = S->fo[L];
+        const float ulos = pp.vlos * l0 * pp.inv_dopp * (1.f / kVLIGHT);
+        const float parcial = l0 * l0 * pp.inv_dopp * kCTE;
+        const float auxCte = -l0 * pp.inv_dopp * (1.f / kVLIGHT);
+        const float ub = S->dl[L][k] * pp.inv_dopp - ulos;
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            const int nc = S->ncomp[L][t];
+            for (int i = 0; i < nc; ++i) {
+                const float nu = parcial * S->comp_nu[L][t][i];
+                const float w = S->comp_w[L][t][i] * fo;
+                const float u = ub - nu * pp.B;
+                float H, F;
+                voigt32(pp.aa, u, H, F);
+                fi[t] += w * H;
+                shi[t] += 2.f * w * F;
+                if (RF) {
+                    const float dH_u = (4.f * pp.aa * F - 2.f * u * H) * w;
+                    const float dF_u = (kRR - pp.aa * H - 2.f * u * F) * w;
+                    const float uu = -u * pp.inv_dopp;
+                    dfi[t][0] += dH_u * (-nu);
+                    dfi[t][1] += dH_u * auxCte;
+                    dfi[t][2] += dH_u * uu;
+                    dfi[t][3] += -2.f * dF_u;
+                    dshi[t][0] += shs * dF_u * (-nu);
+                    dshi[t][1] += shs * dF_u * auxCte;
+                    dshi[t][2] += shs * dF_u * uu;
+                    dshi[t][3] += shs * dH_u * 0.5f;
+                }
+            }
+        }
+    }
+    const float E0_2 = pp.E0_2;
+    const float parcial1 = fi[1] + fi[2];
+    const float parcial2 = E0_2 * (fi[0] - 0.5f * parcial1);
+    const float parcial3 = E0_2 * (shi[0] - 0.5f * (shi[1] + shi[2]));
+    const float etain = E0_2 * (fi[0] * pp.sinis + parcial1 * pp.cosis_2);
+    const float etaq = parcial2 * pp.sinis_cosa;
+    const float etau = parcial2 * pp.sinis_sina;
+    const float etav = (fi[2] - fi[1]) * (E0_2 * pp.cosi);
+    const float rhoq = parcial3 * pp.sinis_cosa;
+    const float rhou = parcial3 * pp.sinis_sina;
+    const float rhov = (shi[2] - shi[1]) * (E0_2 * pp.cosi);
+    const float etai = 1.f + etain;
+    const float etai_2 = etai * etai;
+    const float auxq = rhoq * rhoq, auxu = rhou * rhou, auxv = rhov * rhov;
+    const float gp1 = etai_2 - etaq * etaq - etau * etau - etav * etav + auxq + auxu + auxv;
+    const float gp3 = etai_2 + auxq + auxu + auxv;
+    const float gp2 = etaq * rhoq + etau * rhou + etav * rhov;
+    const float dt = etai_2 * gp1 - gp2 * gp2;
+    const float dti = frcp(dt);
+    const float gp4 = etai_2 * etaq + etai * (etav * rhou - etau * rhov);
+    const float gp5 = etai_2 * etau + etai * (etaq * rhov - etav * rhoq);
+    const float gp6 = etai_2 * etav + etai * (etau * rhoq - etaq * rhou);
+    const float g4 = gp4 + rhoq * gp2;
+    const float g5 = gp5 + rhou * gp2;
+    const float g6 = gp6 + rhov * gp2;
+    const float ah = S->mu;
+    const float dtiaux = dti * (-(pp.S1 * ah));
+    out[0] = pp.S0 - dtiaux * etai * gp3;
+    out[9] = dtiaux * g4;
+    out[18] = dtiaux * g5;
+    out[27] = dtiaux * g6;
+    if (!RF) return;
+
+    /* d(eta, rho)/dp: me_der.cu:254-312 */
+    float d_ei[7], d_eq[7], d_eu[7], d_ev[7], d_rq[7], d_ru[7], d_rv[7];
+    const float E0 = pp.E0;
+    const float iE0 = frcp(E0);
+    d_ei[0] = etain * iE0; d_eq[0] = etaq * iE0; d_eu[0] = etau * iE0;
+    d_ev[0] = etav * iE0; d_rq[0] = rhoq * iE0; d_ru[0] = rhou * iE0;
+    d_rv[0] = rhov * iE0;
+    const float sinis_E0_2 = pp.sinis * E0_2, cosis_2_E0_2 = pp.cosis_2 * E0_2;
+    const float sc = pp.sinis_cosa * E0_2, ss = pp.sinis_sina * E0_2;
+    const float cosi_2_E0 = pp.cosi * E0_2;
+#pragma unroll
+    for (int j = 1; j < 5; ++j) {
+        const float dfisum = dfi[1][j - 1] + dfi[2][j - 1];
+        d_ei[j] = dfi[0][j - 1] * sinis_E0_2 + dfisum * cosis_2_E0_2;
+        const float aux = dfi[0][j - 1] - 0.5f * dfisum;
+        d_eq[j] = aux * sc;
+        d_eu[j] = aux * ss;
+        d_ev[j] = (dfi[2][j - 1] - dfi[1][j - 1]) * cosi_2_E0;
+        const float aux2 = dshi[0][j - 1] - 0.5f * (dshi[1][j - 1] + dshi[2][j - 1]);
+        d_rq[j] = aux2 * sc;
+        d_ru[j] = aux2 * ss;
+        d_rv[j] = (dshi[2][j - 1] - dshi[1][j - 1]) * cosi_2_E0;
+    }
+    const float cosi_cosdi = pp.cosi * pp.cosdi * E0_2, sindi_E0_2 = pp.sindi * E0_2;
+    const float cosdi_E0_2 = E0_2 * pp.cosdi;
+    d_ei[5] = fi[0] * sindi_E0_2 + parcial1 * cosi_cosdi;
+    d_eq[5] = parcial2 * (pp.sindi * pp.cosa);
+    d_eu[5] = parcial2 * (pp.sindi * pp.sina);
+    d_ev[5] = (fi[2] - fi[1]) * cosdi_E0_2;
+    d_rq[5] = parcial3 * (pp.sindi * pp.cosa);
+    d_ru[5] = parcial3 * (pp.sindi * pp.sina);
+    d_rv[5] = (shi[2] - shi[1]) * cosdi_E0_2;
+    d_ei[6] = 0.f;
+    d_eq[6] = parcial2 * (pp.sinis * pp.cosda);
+    d_eu[6] = parcial2 * (pp.sinis * pp.sinda);
+    d_ev[6] = 0.f;
+    d_rq[6] = parcial3 * (pp.sinis * pp.cosda);
+    d_ru[6] = parcial3 * (pp.sinis * pp.sinda);
+    d_rv[6] = 0.f;
+
+    /* chain rule through the ME solution: me_der.cu:317-373 */
+    const float dtaux = (-(pp.S1 * ah)) * dti * dti;
+    const float etai_gp3 = etai * gp3;
+    const float a2 = 2.f * etai;
+    const float ext1 = a2 * etaq + etav * rhou - etau * rhov;
+    const float ext2 = a2 * etau + etaq * rhov - etav * rhoq;
+    const float ext3 = a2 * etav + etau * rhoq - etaq * rhou;
+    const float ext4 = a2 * gp1;
+#pragma unroll
+    for (int p = 0; p < 7; ++p) {
+        const float ei = d_ei[p], eq = d_eq[p], eu = d_eu[p], ev = d_ev[p];
+        const float rq = d_rq[p], ru = d_ru[p], rv = d_rv[p];
+        const float dgp1 = 2.f * (etai * ei - etaq * eq - etau * eu - etav * ev +
+                                  rhoq * rq + rhou * ru + rhov * rv);
+        const float dgp2 = rhoq * eq + etaq * rq + rhou * eu + etau * ru + rhov * ev + etav * rv;
+        const float d_dt = ext4 * ei + etai_2 * dgp1 - 2.f * gp2 * dgp2;
+        const float dgp3 = 2.f * (etai * ei + rhoq * rq + rhou * ru + rhov * rv);
+        const float dgp4 = ei * ext1 + etai_2 * eq + etai * (rhou * ev + etav * ru - rhov * eu - etau * rv);
+        const float dgp5 = ei * ext2 + etai_2 * eu + etai * (rhov * eq + etaq * rv - rhoq * ev - etav * rq);
+        const float dgp6 = ei * ext3 + etai_2 * ev + etai * (rhoq * eu + etau * rq - rhou * eq - etaq * ru);
+        out[0 + 1 + p] = -(((ei * gp3 + etai * dgp3) * dt - d_dt * etai_gp3) * dtaux);
+        out[9 + 1 + p] = ((dgp4 + rq * gp2 + rhoq * dgp2) * dt - d_dt * g4) * dtaux;
+        out[18 + 1 + p] = ((dgp5 + ru * gp2 + rhou * dgp2) * dt - d_dt * g5) * dtaux;
+        out[27 + 1 + p] = ((dgp6 + rv * gp2 + rhov * dgp2) * dt - d_dt * g6) * dtaux;
+    }
+    const float dti2 = -(dti * ah);
+    out[0 + 8] = -dti2 * etai_gp3;
+    out[9 + 8] = dti2 * g4;
+    out[18 + 8] = dti2 * g5;
+    out[27 + 8] = dti2 * g6;
+    out[0 + 7] = 0.f;   /* d I / d phi = 0 (me_der.cu:372) */
+    out[27 + 7] = 0.f;  /* d V / d phi = 0 (me_der.cu:373) */
+}
+
+__device__ __forceinline__ void load_model(const double *__restrict__ models,
+                                           int64_t fstride, int64_t pstride,
+                                           int64_t pix, double *m)
+{
+#pragma unroll
+    for (int f = 0; f < GM_NMODEL; ++f) m[f] = models[f * fstride + pix * pstride];
+}
+
+/* Reference continuum rule for the I-variant of the PSF: the larger end
+ * sample (mil_sinrf.cu:120-123, me_der.cu:171-174).                         */
+__device__ __forceinline__ float end_max(float x0, float xn) { return (x0 > xn) ? x0 : xn; }
+
+template <typename T, int W>
+__device__ __forceinline__ T group_sum(T v)
+{
+#pragma unroll
+    for (int m = W / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+/* RF slot (reference layout) -> staged slot j within a Stokes block; -1 is
+ * the unconvolved S0 response, -2 the always-zero slot 9.                   */
+__device__ __forceinline__ int staged_slot(int p)
+{
+    return (p <= 6) ? p + 1 : (p == 7 ? -1 : (p == 8 ? 8 : -2));
+}
+
+/* ------------------------------------------------------------------------ */
+/* Synthesis + response functions + PSF + normal equations + chi^2 for a list
+ * of pixels; wavelength-parallel mapping.                                   */
+template <int LPP, int NLPL, bool MIXED>
+__global__ __launch_bounds__(64) void k_rf(
+    const gm_dsetup *__restrict__ S, const double *__restrict__ models,
+    int64_t mfs, int64_t mps, const float *__restrict__ obs,
+    const int32_t *__restrict__ list, const int32_t *__restrict__ count,
+    int64_t n_direct, float *__restrict__ alpha, float *__restrict__ beta,
+    int64_t aes, int64_t aps, int64_t bes, int64_t bps, double *__restrict__ chi_out,
+    float *__restrict__ synth_out, float *__restrict__ rf_out, int rows, int padlo)
+{
+    constexpr int PPW = 64 / LPP;
+    constexpr int RSTR = 60;                   /* reduction row stride (floats) */
+    extern __shared__ float4 lds4[];
+    const int lane = threadIdx.x;
+    const int h = lane / LPP;
+    const int k0 = lane % LPP;
+    const int n = S->nlambda;
+    const int64_t cnt = count ? (int64_t)(*count) : n_direct;
+    const int region4 = max(rows * 9, LPP * (RSTR / 4));
+    float4 *conv = lds4 + h * region4;
+    float *red = reinterpret_cast<float *>(conv);
+    const int use_psf = S->use_psf;
+    const int dlo = S->psf_dlo, dhi = S->psf_dhi;
+    const uint32_t fm = S->freemask;
+    const int nfree = S->nfree_chi;
+
+    for (int64_t base = (int64_t)blockIdx.x * PPW; base < cnt;
+         base += (int64_t)gridDim.x * PPW) {
+        const int64_t item = base + h;
+        const bool valid = item < cnt;
+        const int64_t pix = list ? (int64_t)list[valid ? item : base] : (valid ? item : base);
+        double m[GM_NMODEL];
+        load_model(models, mfs, mps, pix, m);
+        const PixPre pp = make_pre(m);
+
+        /* zero padding rows (the reduction buffer aliases them) */
+        for (int r = k0; r < rows; r += LPP)
+            if (r < padlo || r >= padlo + LPP * NLPL)
+#pragma unroll
+                for (int j = 0; j < 9; ++j) conv[r * 9 + j] = make_float4(0.f, 0.f, 0.f, 0.f);
+        /* stage unconvolved values, one row per wavelength */
+#pragma unroll
+        for (int s = 0; s < NLPL; ++s) {
+            const int k = k0 + s * LPP;
+            float v[GM_SLOTS];
+            eval_lambda<true>(S, pp, k < n ? k : n - 1, v);
+            if (k >= n)
+#pragma unroll
+                for (int j = 0; j < GM_SLOTS; ++j) v[j] = 0.f;
+            float4 *row = conv + (padlo + k) * 9;
+#pragma unroll
+            for (int j = 0; j < 9; ++j)
+                row[j] = make_float4(v[4 * j], v[4 * j + 1], v[4 * j + 2], v[4 * j + 3]);
+        }
+        __syncthreads();
+
+        float acc[GM_RED_N];
+#pragma unroll
+        for (int e = 0; e < GM_RED_N; ++e) acc[e] = 0.f;
+        double chi = 0.0;
+#pragma unroll
+        for (int s = 0; s < NLPL; ++s) {
+            const int k = k0 + s * LPP;
+            float y[GM_SLOTS];
+            if (use_psf) {
+#pragma unroll
+                for (int j = 0; j < GM_SLOTS; ++j) y[j] = 0.f;
+                for (int d = dlo; d <= dhi; ++d) {
+                    const float tap = S->psf_tap[d - dlo];
+                    const float4 *row = conv + (padlo + k + d) * 9;
+#pragma unroll
+                    for (int j = 0; j < 9; ++j) {
+                        const float4 x = row[j];
+                        y[4 * j] += tap * x.x;
+                        y[4 * j + 1] += tap * x.y;
+                        y[4 * j + 2] += tap * x.z;
+                        y[4 * j + 3] += tap * x.w;
+                    }
+                }
+                /* I-variant (Ic trick) on the Stokes-I block: adds
+                 * Ic * (1 - sum of in-range taps), convolution.cu:200-241 */
+                const float edge = S->psf_edge[k < n ? k : 0];
+                const float4 *r0 = conv + padlo * 9;
+                const float4 *rn = conv + (padlo + n - 1) * 9;
+                const float4 a0 = r0[0], a1 = r0[1], a2 = r0[2];
+                const float4 b0 = rn[0], b1 = rn[1], b2 = rn[2];
+                y[0] += end_max(a0.x, b0.x) * edge;
+                y[1] += end_max(a0.y, b0.y) * edge;
+                y[2] += end_max(a0.z, b0.z) * edge;
+                y[3] += end_max(a0.w, b0.w) * edge;
+                y[4] += end_max(a1.x, b1.x) * edge;
+                y[5] += end_max(a1.y, b1.y) * edge;
+                y[6] += end_max(a1.z, b1.z) * edge;
+                y[7] += end_max(a1.w, b1.w) * edge;
+                y[8] += end_max(a2.x, b2.x) * edge;
+            } else {
+                const float4 *row = conv + (padlo + k) * 9;
+#pragma unroll
+                for (int j = 0; j < 9; ++j) {
+                    const float4 x = row[j];
+                    y[4 * j] = x.x; y[4 * j + 1] = x.y; y[4 * j + 2] = x.z; y[4 * j + 3] = x.w;
+                }
+            }
+            if (k < n) {
+                float g[4][9];
+                float r[4];
+#pragma unroll
+                for (int st = 0; st < 4; ++st) {
+                    r[st] = obs ? y[st * 9] - obs[pix * 4 * n + st * n + k] : 0.f;
+#pragma unroll
+                    for (int p = 0; p < 9; ++p) {
+                        const int js = staged_slot(p);
+                        float gv = (js >= 0) ? y[st * 9 + js] : (st == 0 ? 1.f : 0.f);
+                        g[st][p] = ((fm >> p) & 1u) ? gv : 0.f;
+                    }
+                }
+#pragma unroll
+                for (int st = 0; st < 4 && obs; ++st) {
+                    const float ws = S->wsig[st];
+                    if (MIXED)
+                        chi += (double)ws * ((double)r[st] * (double)r[st]);
+                    else
+                        acc[54] += (r[st] * r[st]) * ws;
+                    int e = 0;
+#pragma unroll
+                    for (int p = 0; p < 9; ++p) {
+                        const float wg = ws * g[st][p];
+#pragma unroll
+                        for (int q = p; q < 9; ++q) acc[e++] += wg * g[st][q];
+                    }
+#pragma unroll
+                    for (int p = 0; p < 9; ++p) acc[45 + p] += (ws * r[st]) * g[st][p];
+                }
+                if (synth_out && valid) {
+                    float *so = synth_out + pix * 4 * n + k;
+#pragma unroll
+                    for (int st = 0; st < 4; ++st) so[st * n] = y[st * 9];
+                }
+                if (rf_out && valid) {
+                    float *ro = rf_out + pix * 4 * 10 * n + k;
+#pragma unroll
+                    for (int st = 0; st < 4; ++st)
+#pragma unroll
+                        for (int p = 0; p < 10; ++p) ro[(st * 10 + p) * n] = (p < 9) ? g[st][p] : 0.f;
+                }
+            }
+        }
+        __syncthreads();
+        /* cross-lane reduction of the 55 partial sums through LDS */
+        float4 *rrow = reinterpret_cast<float4 *>(red + k0 * RSTR);
+#pragma unroll
+        for (int j = 0; j < 14; ++j)
+            rrow[j] = make_float4(acc[4 * j], acc[4 * j + 1], acc[4 * j + 2],
+                                  (4 * j + 3 < GM_RED_N) ? acc[4 * j + 3] : 0.f);
+        __syncthreads();
+        for (int e = k0; e < GM_RED_N - 1; e += LPP) {
+            float sum = 0.f;
+#pragma unroll 8
+            for (int l = 0; l < LPP; ++l) sum += red[l * RSTR + e];
+            if (valid) {
+                if (e < 45) {
+                    if (alpha) alpha[e * aes + pix * aps] = sum;
+                } else {
+                    if (beta) beta[(e - 45) * bes + pix * bps] = sum;
+                }
+            }
+        }
+        if (MIXED) {
+            chi = group_sum<double, LPP>(chi);
+        } else {
+            float c = acc[54];
+            c = group_sum<float, LPP>(c);
+            chi = (double)(c / (float)nfree);
+        }
+        if (MIXED) chi = chi / (double)nfree;
+        if (chi_out && valid && k0 == 0) chi_out[pix] = chi;
+        __syncthreads();
+    }
+}
+
+/* Synthesis + PSF + chi^2 only (rejected steps need nothing more). */
+template <int LPP, int NLPL, bool MIXED>
+__global__ __launch_bounds__(64) void k_synth(
+    const gm_dsetup *__restrict__ S, const double *__restrict__ models,
+    int64_t mfs, int64_t mps, const float *__restrict__ obs,
+    const int32_t *__restrict__ list, const int32_t *__restrict__ count,
+    int64_t n_direct, double *__restrict__ chi_out, float *__restrict__ synth_out,
+    int rows, int padlo)
+{
+    constexpr int PPW = 64 / LPP;
+    extern __shared__ float4 lds4[];
+    const int lane = threadIdx.x;
+    const int h = lane / LPP;
+    const int k0 = lane % LPP;
+    const int n = S->nlambda;
+    const int64_t cnt = count ? (int64_t)(*count) : n_direct;
+    float4 *conv = lds4 + h * rows;
+    const int use_psf = S->use_psf;
+    const int dlo = S->psf_dlo, dhi = S->psf_dhi;
+    const int nfree = S->nfree_chi;
+
+    for (int r = k0; r < rows; r += LPP)
+        if (r < padlo || r >= padlo + LPP * NLPL) conv[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+
+    for (int64_t base = (int64_t)blockIdx.x * PPW; base < cnt;
+         base += (int64_t)gridDim.x * PPW) {
+        const int64_t item = base + h;
+        const bool valid = item < cnt;
+        const int64_t pix = list ? (int64_t)list[valid ? item : base] : (valid ? item : base);
+        double m[GM_NMODEL];
+        load_model(models, mfs, mps, pix, m);
+        const PixPre pp = make_pre(m);
+#pragma unroll
+        for (int s = 0; s < NLPL; ++s) {
+            const int k = k0 + s * LPP;
+            float v[GM_SLOTS];
+            eval_lambda<false>(S, pp, k < n ? k : n - 1, v);
+            conv[padlo + k] = (k < n) ? make_float4(v[0], v[9], v[18], v[27])
+                                      : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        __syncthreads();
+        double chi = 0.0;
+        float chif = 0.f;
+#pragma unroll
+        for (int s = 0; s < NLPL; ++s) {
+            const int k = k0 + s * LPP;
+            float4 y;
+            if (use_psf) {
+                y = make_float4(0.f, 0.f, 0.f, 0.f);
+                for (int d = dlo; d <= dhi; ++d) {
+                    const float tap = S->psf_tap[d - dlo];
+                    const float4 x = conv[padlo + k + d];
+                    y.x += tap * x.x;
+                    y.y += tap * x.y;
+                    y.z += tap * x.z;
+                    y.w += tap * x.w;
+                }
+                const float edge = S->psf_edge[k < n ? k : 0];
+                y.x += end_max(conv[padlo].x, conv[padlo + n - 1].x) * edge;
+            } else {
+                y = conv[padlo + k];
+            }
+            if (k < n && obs) {
+                const float *ob = obs + pix * 4 * n + k;
+                const float r0 = y.x - ob[0], r1 = y.y - ob[n], r2 = y.z - ob[2 * n], r3 = y.w - ob[3 * n];
+                if (MIXED) {
+                    chi += (double)S->wsig[0] * ((double)r0 * r0) + (double)S->wsig[1] * ((double)r1 * r1) +
+                           (double)S->wsig[2] * ((double)r2 * r2) + (double)S->wsig[3] * ((double)r3 * r3);
+                } else {
+                    chif += (r0 * r0) * S->wsig[0] + (r1 * r1) * S->wsig[1] +
+                            (r2 * r2) * S->wsig[2] + (r3 * r3) * S->wsig[3];
+                }
+            }
+            if (k < n && synth_out && valid) {
+                float *so = synth_out + pix * 4 * n + k;
+                so[0] = y.x; so[n] = y.y; so[2 * n] = y.z; so[3 * n] = y.w;
+            }
+        }
+        if (MIXED) {
+            chi = group_sum<double, LPP>(chi) / (double)nfree;
+        } else {
+            chif = group_sum<float, LPP>(chif);
+            chi = (double)(chif / (float)nfree);
+        }
+        if (chi_out && valid && k0 == 0) chi_out[pix] = chi;
+        __syncthreads();
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* 9x9 symmetric step solves, pixel-parallel, all indices static.           */
+__host__ __device__ constexpr int ut(int i, int j)
+{
+    return (i <= j) ? (i * 9 - (i * (i - 1)) / 2 + (j - i))
+                    : (j * 9 - (j * (j - 1)) / 2 + (i - j));
+}
+
+template <typename T> __device__ __forceinline__ T rsq(T x);
+template <> __device__ __forceinline__ float rsq<float>(float x) { return __builtin_amdgcn_rsqf(x); }
+template <> __device__ __forceinline__ double rsq<double>(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    return y * (1.5 - 0.5 * x * y * y);      /* one Newton step */
+}
+
+/* SVD of the symmetric damped matrix by cyclic Jacobi rotations; the
+ * singular values are |eigenvalues| and the solution is the reference's
+ * delta = V diag(1/w, |w| > 1e-12) V^T beta (milosUtils.cu:296-322).        */
+template <typename T, int MAXSWEEP>
+__device__ __forceinline__ void solve_svd(T (&a)[45], const T (&b)[9], T (&x)[9])
+{
+    T v[81];
+#pragma unroll
+    for (int i = 0; i < 81; ++i) v[i] = (i % 10 == 0) ? T(1) : T(0);
+    const T eps2 = (sizeof(T) == 8) ? T(1e-30) : T(1e-14);
+    bool done = false;
+    for (int sweep = 0; sweep < MAXSWEEP; ++sweep) {
+        T off = 0, dg = 0;
+#pragma unroll
+        for (int p = 0; p < 9; ++p) {
+            dg += a[ut(p, p)] * a[ut(p, p)];
+#pragma unroll
+            for (int q = p + 1; q < 9; ++q) off += a[ut(p, q)] * a[ut(p, q)];
+        }
+        done = done || (off <= eps2 * dg);
+        if (__all(done)) break;
+#pragma unroll
+        for (int p = 0; p < 8; ++p) {
+#pragma unroll
+            for (int q = p + 1; q < 9; ++q) {
+                const T apq = a[ut(p, q)];
+                const T app = a[ut(p, p)], aqq = a[ut(q, q)];
+                const T d = aqq - app;
+                const T den = fabs(d) + sqrt(d * d + T(4) * apq * apq);
+                T t = (den > T(0)) ? (T(2) * apq) / den : T(0);
+                t = (d < T(0)) ? -t : t;
+                t = done ? T(0) : t;
+                const T c = rsq<T>(T(1) + t * t);
+                const T s = t * c;
+                a[ut(p, p)] = app - t * apq;
+                a[ut(q, q)] = aqq + t * apq;
+                a[ut(p, q)] = T(0);
+#pragma unroll
+                for (int r = 0; r < 9; ++r) {
+                    if (r == p || r == q) continue;
+                    const T arp = a[ut(r, p)], arq = a[ut(r, q)];
+                    a[ut(r, p)] = c * arp - s * arq;
+                    a[ut(r, q)] = s * arp + c * arq;
+                }
+#pragma unroll
+                for (int r = 0; r < 9; ++r) {
+                    const T vrp = v[r * 9 + p], vrq = v[r * 9 + q];
+                    v[r * 9 + p] = c * vrp - s * vrq;
+                    v[r * 9 + q] = s * vrp + c * vrq;
+                }
+            }
+        }
+    }
+    T aux[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        T sum = 0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) sum += b[k] * v[k * 9 + j];
+        const T w = fabs(a[ut(j, j)]);
+        aux[j] = (w > T(1e-12)) ? sum / w : T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        T sum = 0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) sum += v[i * 9 + k] * aux[k];
+        x[i] = sum;
+    }
+}
+
+/* LDL^T of the damped (symmetric positive semi-definite) matrix; pivots
+ * <= 1e-12 drop their component, as the pseudo-inverse drops |w| <= 1e-12. */
+template <typename T>
+__device__ __forceinline__ void solve_ldl(const T (&a)[45], const T (&b)[9], T (&x)[9])
+{
+    T L[45], d[9], id[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        T dj = a[ut(j, j)];
+#pragma unroll
+        for (int k = 0; k < j; ++k) dj -= L[ut(j, k)] * L[ut(j, k)] * d[k];
+        const bool ok = dj > T(1e-12);
+        d[j] = ok ? dj : T(0);
+        id[j] = ok ? T(1) / dj : T(0);
+#pragma unroll
+        for (int i = j + 1; i < 9; ++i) {
+            T lij = a[ut(i, j)];
+#pragma unroll
+            for (int k = 0; k < j; ++k) lij -= L[ut(i, k)] * L[ut(j, k)] * d[k];
+            L[ut(i, j)] = lij * id[j];
+        }
+    }
+    T z[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        T zi = b[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) zi -= L[ut(i, k)] * z[k];
+        z[i] = zi;
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) z[i] *= id[i];
+#pragma unroll
+    for (int i = 8; i >= 0; --i) {
+        T xi = z[i];
+#pragma unroll
+        for (int k = i + 1; k < 9; ++k) xi -= L[ut(k, i)] * x[k];
+        x[i] = xi;
+    }
+}
+
+template <typename T, int SOLVER>
+__device__ __forceinline__ void damped_solve(const float *__restrict__ alpha,
+                                             const float *__restrict__ beta,
+                                             int64_t es, T flambda, T (&x)[9])
+{
+    T a[45], b[9];
+#pragma unroll
+    for (int e = 0; e < 45; ++e) a[e] = (T)alpha[e * es];
+#pragma unroll
+    for (int p = 0; p < 9; ++p) {
+        b[p] = (T)beta[p * es];
+        a[ut(p, p)] = a[ut(p, p)] * (T(1) + flambda);   /* milosUtils.cu:648-652 */
+    }
+    if (SOLVER == GM_SOLVE_SVD)
+        solve_svd<T, (sizeof(T) == 8 ? 12 : 10)>(a, b, x);
+    else
+        solve_ldl<T>(a, b, x);
+}
+
+/* AplicaDelta + check (milosUtils.cu:30-238), double model. */
+template <typename T>
+__device__ __forceinline__ void apply_delta_check(const gm_dsetup *__restrict__ S,
+                                                  const double *best, const T (&d)[9],
+                                                  double *m)
+{
+    const int32_t *fix = S->fix;
+    T aux;
+    if (fix[0]) m[0] = best[0] - d[0];
+    if (fix[1]) {
+        aux = d[1];
+        aux = aux < T(-300) ? T(-300) : (aux > T(300) ? T(300) : aux);
+        m[1] = best[1] - aux;
+    }
+    if (fix[2]) m[2] = best[2] - d[2];
+    if (fix[3]) m[3] = best[3] - d[3];
+    if (fix[4]) m[4] = best[4] - d[4];
+    if (fix[5]) {
+        aux = d[5];
+        aux = aux < T(-30) ? T(-30) : (aux > T(30) ? T(30) : aux);
+        m[5] = best[5] - aux;
+    }
+    if (fix[6]) {
+        aux = d[6];
+        aux = aux < T(-30) ? T(-30) : (aux > T(30) ? T(30) : aux);
+        m[6] = best[6] - aux;
+    }
+    if (fix[7]) m[7] = best[7] - d[7];
+    if (fix[8]) m[8] = best[8] - d[8];
+    /* check(): milosUtils.cu:160-238 */
+    if (m[1] < 0) { m[1] = -m[1]; m[5] = 180.0 - m[5]; }
+    if (m[1] > 5000) m[1] = 5000;
+    if (m[5] < 0) m[5] = -m[5];
+    if (m[5] > 180) m[5] = 360.0 - m[5];
+    if (m[6] < 0) m[6] = 180 + m[6];
+    if (m[6] > 180) m[6] = m[6] - 180.0;
+    m[0] = m[0] < 1 ? 1 : (m[0] > 2500 ? 2500 : m[0]);
+    m[2] = m[2] < -20 ? -20 : (m[2] > 20 ? 20 : m[2]);
+    m[3] = m[3] < 0.0001 ? 0.0001 : (m[3] > 0.6 ? 0.6 : m[3]);
+    m[4] = m[4] < 0.0001 ? 0.0001 : (m[4] > 10.0 ? 10.0 : m[4]);
+    m[7] = m[7] < 0.0001 ? 0.0001 : (m[7] > 2.0 ? 2.0 : m[7]);
+    m[8] = m[8] < 0.0001 ? 0.0001 : (m[8] > 2.0 ? 2.0 : m[8]);
+    m[9] = m[9] < 0 ? 0 : (m[9] > 4 ? 4 : m[9]);
+    m[10] = m[10] < 0 ? 0.0 : (m[10] > 1.0 ? 1.0 : m[10]);
+}
+
+template <bool MIXED, int SOLVER>
+__global__ __launch_bounds__(64) void k_solve(const gm_dsetup *__restrict__ S,
+                                              gm_state st, const int32_t *__restrict__ list,
+                                              const int32_t *__restrict__ count,
+                                              int32_t *__restrict__ reset0,
+                                              int32_t *__restrict__ reset1)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (reset0) *reset0 = 0;
+        if (reset1) *reset1 = 0;
+    }
+    using T = typename std::conditional<MIXED, double, float>::type;
+    const int64_t cnt = *count;
+    const int64_t cap = st.cap;
+    for (int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x; i < cnt;
+         i += (int64_t)gridDim.x * 64) {
+        const int64_t pix = list[i];
+        T x[9];
+        damped_solve<T, SOLVER>(st.alpha + pix, st.beta + pix, cap, (T)st.flambda[pix], x);
+        double best[GM_NMODEL], m[GM_NMODEL];
+#pragma unroll
+        for (int f = 0; f < GM_NMODEL; ++f) {
+            best[f] = st.best[f * cap + pix];
+            m[f] = st.trial[f * cap + pix];
+        }
+        apply_delta_check<T>(S, best, x, m);
+#pragma unroll
+        for (int f = 0; f < GM_NMODEL; ++f) st.trial[f * cap + pix] = m[f];
+    }
+}
+
+template <bool MIXED, int SOLVER>
+__global__ __launch_bounds__(64) void k_step_solve(const float *__restrict__ alpha,
+                                                   const float *__restrict__ beta,
+                                                   const double *__restrict__ flambda,
+                                                   int64_t n, double *__restrict__ delta)
+{
+    using T = typename std::conditional<MIXED, double, float>::type;
+    for (int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * 64) {
+        /* caller layout: alpha [n][45], beta [n][9] */
+        T x[9];
+        damped_solve<T, SOLVER>(alpha + i * 45, beta + i * 9, 1, (T)flambda[i], x);
+#pragma unroll
+        for (int p = 0; p < 9; ++p) delta[i * 9 + p] = (double)x[p];
+    }
+}
+
+/* Accept / reject and LM bookkeeping (milosUtils.cu:660-698); builds the
+ * next active list and the list of accepted pixels that need new normal
+ * equations.  One atomic per wave per list.                                */
+template <bool MIXED>
+__global__ __launch_bounds__(64) void k_decide(const gm_dsetup *__restrict__ S, gm_state st,
+                                               const int32_t *__restrict__ list,
+                                               const int32_t *__restrict__ count,
+                                               int32_t *__restrict__ next_list,
+                                               int32_t *__restrict__ next_count,
+                                               int32_t *__restrict__ acc_list,
+                                               int32_t *__restrict__ acc_count)
+{
+    using T = typename std::conditional<MIXED, double, float>::type;
+    const int64_t cnt = *count;
+    const int64_t cap = st.cap;
+    const int lane = threadIdx.x;
+    for (int64_t base = (int64_t)blockIdx.x * 64; base < cnt; base += (int64_t)gridDim.x * 64) {
+        const int64_t i = base + lane;
+        const bool valid = i < cnt;
+        int32_t pix = valid ? list[i] : 0;
+        bool cont = false, accn = false;
+        if (valid) {
+            const T chisqr = (T)st.chi_trial[pix];
+            T ochisqr = (T)st.ochisqr[pix];
+            T flambda = (T)st.flambda[pix];
+            const T factor = (T)st.factor[pix];
+            int it = st.iter[pix];
+            bool clanda = (fabs(((ochisqr) - (chisqr)) * T(100) / (chisqr)) < (T)S->toplim) ||
+                          ((chisqr) < T(0.0001));
+            bool accept = (chisqr) - (ochisqr) < T(0);
+            if (accept) {
+                flambda = flambda / (T(5.0) * factor);
+#pragma unroll
+                for (int f = 0; f < GM_NMODEL; ++f) st.best[f * cap + pix] = st.trial[f * cap + pix];
+                ochisqr = chisqr;
+            } else {
+                flambda = flambda * T(10.0) * factor;
+            }
+            if ((flambda > T(1e+7)) || (flambda < T(1e-25))) clanda = true;
+            it++;
+            if (S->logclambda) {
+                const T c0 = (T)st.chisqr0[pix];
+                st.factor[pix] = (double)(log10(chisqr) / log10(c0));
+            }
+            st.flambda[pix] = (double)flambda;
+            st.ochisqr[pix] = (double)ochisqr;
+            st.iter[pix] = it;
+            cont = (it < S->miter) && !clanda;
+            accn = accept && cont;
+            if (accn) st.nacc[pix] += 1;
+        }
+        const uint64_t mc = __ballot(cont);
+        const uint64_t ma = __ballot(accn);
+        const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        int bc = 0, ba = 0;
+        if (lane == 0) {
+            if (mc) bc = atomicAdd(next_count, (int)__popcll(mc));
+            if (ma) ba = atomicAdd(acc_count, (int)__popcll(ma));
+        }
+        bc = __shfl(bc, 0, 64);
+        ba = __shfl(ba, 0, 64);
+        if (cont) next_list[bc + __popcll(mc & below)] = pix;
+        if (accn) acc_list[ba + __popcll(ma & below)] = pix;
+    }
+}
+
+/* Classical estimates, FP64 (milosUtils.cu:386-563), pixel-parallel. */
+__device__ void classical_estimates(const gm_dsetup *__restrict__ S, const float *sp,
+                                    double *model)
+{
+    const int n = S->nlambda;
+    const int lo = S->ce_lo, hi = S->ce_hi;
+    const double lambda_0 = S->ce_l0;
+    const float *sI = sp, *sQ = sp + n, *sU = sp + 2 * n, *sV = sp + 3 * n;
+    const double *lambda = S->lambda;
+    int i;
+    int begin = lo;
+    for (i = lo; i < hi; i++) { if (sI[i] < 0) begin++; else break; }
+    int end = hi;
+    for (i = hi - 1; i >= lo; i--) { if (sI[i] < 0) end--; else break; }
+    /* all-negative I: the reference would read I[-1]; use the first sample */
+    const double Ic = (end > lo) ? sI[end - 1] : sI[lo];
+    double x = 0, y = 0, x_vlos = 0, y_vlos = 0;
+    for (i = begin; i < end - 1; i++) {
+        if (sI[i] > -1 && sV[i] > -1) {
+            const double aux = (Ic - (double)(sI[i] + sV[i]));
+            const double aux_vlos = (Ic - sI[i]);
+            x += aux * (lambda[i] - lambda_0);
+            x_vlos += aux_vlos * (lambda[i] - lambda_0);
+            y += aux;
+            y_vlos += aux_vlos;
+        }
+    }
+    const double lp = (fabs(y) > 1e-15) ? x / y : 0;
+    x = 0; y = 0;
+    for (i = begin; i < end - 1; i++) {
+        if (sI[i] > -1 && sV[i] > -1) {
+            const double aux = (Ic - (double)(sI[i] - sV[i]));
+            x += aux * (lambda[i] - lambda_0);
+            y += aux;
+        }
+    }
+    const double lm = (fabs(y) > 1e-15) ? x / y : 0;
+    const double C = (4.6686411e-13 * (lambda_0 * lambda_0) * S->ce_geff);
+    const double Blos = (1 / C) * ((lp - lm) / 2);
+    double Vlos = (2.99792458e+5 / lambda_0) * ((x_vlos / y_vlos) / 2);
+    x = 0; y = 0;
+    for (i = begin; i < end - 1; i++) {
+        if (sQ[i] > -1 && sU[i] > -1 && sV[i] > -1) {
+            const double L = fabs(sqrtf(sQ[i] * sQ[i] + sU[i] * sU[i]));
+            const double mm = fabs((4 * (lambda[i] - lambda_0) * L));
+            x = x + fabsf(sV[i]) * mm;
+            y = y + fabsf(sV[i]) * fabsf(sV[i]);
+        }
+    }
+    y = y * fabs((3 * C * Blos));
+    const double tan_gamma = fabs(sqrt(x / y));
+    double gamma_rad = atan(tan_gamma);
+    double gamma = gamma_rad * (180 / kPI);
+    if (gamma >= 85 && gamma <= 90) gamma_rad = 85 * (kPI / 180);
+    if (gamma > 90 && gamma <= 95) gamma_rad = 95 * (kPI / 180);
+    if (Blos < 0) gamma = (180) - gamma;
+    double sum_u = 0.0, sum_q = 0.0;
+    for (i = lo; i < hi; i++) {
+        if (sU[i] > -1 && sQ[i] > -1) {
+            if (fabs(sU[i]) > 0.0001 || fabs(sQ[i]) > 0.0001) {
+                sum_u += sU[i];
+                sum_q += sQ[i];
+            }
+        }
+    }
+    const double tan2phi = sum_u / sum_q;
+    double phi = (atan(tan2phi) * 180 / kPI) / 2;
+    if (sum_u > 0 && sum_q > 0) phi = phi;
+    else if (sum_u < 0 && sum_q > 0) phi = phi + 180;
+    else if (sum_u < 0 && sum_q < 0) phi = phi + 90;
+    else if (sum_u > 0 && sum_q < 0) phi = phi + 90;
+    const double B_aux = fabs(Blos / cos(gamma_rad));
+    if (Vlos < (-4)) Vlos = -4;
+    if (Vlos > (4)) Vlos = 4;
+    model[1] = (B_aux > 4000 ? 4000 : B_aux);
+    model[2] = Vlos;
+    model[5] = gamma;
+    model[6] = phi;
+    /* NaN guards: milosUtils.cu:621-628 */
+    if (isnan(model[1])) model[1] = 1;
+    if (isnan(model[2])) model[2] = 1e-3;
+    if (isnan(model[5])) model[5] = 1;
+    if (isnan(model[6])) model[6] = 1;
+}
+
+__global__ __launch_bounds__(64) void k_ce(const gm_dsetup *__restrict__ S,
+                                           const float *__restrict__ obs, int64_t npix,
+                                           double *__restrict__ models)
+{
+    const int n = S->nlambda;
+    for (int64_t p = (int64_t)blockIdx.x * 64 + threadIdx.x; p < npix;
+         p += (int64_t)gridDim.x * 64) {
+        double m[GM_NMODEL];
+#pragma unroll
+        for (int f = 0; f < GM_NMODEL; ++f) m[f] = S->init_model[f];
+        classical_estimates(S, obs + p * 4 * n, m);
+#pragma unroll
+        for (int f = 0; f < GM_NMODEL; ++f) models[p * GM_NMODEL + f] = m[f];
+    }
+}
+
+__global__ __launch_bounds__(64) void k_init_state(const gm_dsetup *__restrict__ S, gm_state st,
+                                                   const float *__restrict__ obs,
+                                                   const double *__restrict__ init,
+                                                   int init_mode, int64_t npix,
+                                                   int32_t *__restrict__ list0)
+{
+    const int n = S->nlambda;
+    const int64_t cap = st.cap;
+    for (int64_t p = (int64_t)blockIdx.x * 64 + threadIdx.x; p < npix;
+         p += (int64_t)gridDim.x * 64) {
+        double m[GM_NMODEL];
+        if (init_mode == GM_INIT_PER_PIXEL) {
+#pragma unroll
+            for (int f = 0; f < GM_NMODEL; ++f) m[f] = init[p * GM_NMODEL + f];
+        } else {
+#pragma unroll
+            for (int f = 0; f < GM_NMODEL; ++f) m[f] = S->init_model[f];
+            if (init_mode == GM_INIT_CE) classical_estimates(S, obs + p * 4 * n, m);
+        }
+#pragma unroll
+        for (int f = 0; f < GM_NMODEL; ++f) {
+            st.best[f * cap + p] = m[f];
+            st.trial[f * cap + p] = m[f];
+        }
+        st.flambda[p] = S->ilambda;
+        st.factor[p] = 1.0;
+        st.iter[p] = 0;
+        st.nacc[p] = 0;
+        list0[p] = (int32_t)p;
+    }
+}
+
+__global__ void k_init_chi(gm_state st, int64_t npix)
+{
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npix;
+         p += (int64_t)gridDim.x * blockDim.x)
+        st.chisqr0[p] = st.ochisqr[p];
+}
+
+__global__ void k_finalize(gm_state st, int64_t npix, double *__restrict__ model,
+                           float *__restrict__ chisqr, int32_t *__restrict__ iters,
+                           int32_t *__restrict__ naccs)
+{
+    const int64_t cap = st.cap;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npix;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        if (model)
+            for (int f = 0; f < GM_NMODEL; ++f) model[p * GM_NMODEL + f] = st.best[f * cap + p];
+        if (chisqr) chisqr[p] = (float)st.ochisqr[p];
+        if (iters) iters[p] = st.iter[p];
+        if (naccs) naccs[p] = st.nacc[p];
+    }
+}
+
+__global__ void k_set_count(int32_t *c, int32_t v) { *c = v; }
+
+inline int grid_for(int64_t items, int per_block, int cap_blocks)
+{
+    int64_t g = (items + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > cap_blocks) g = cap_blocks;
+    return (int)g;
+}
+
+constexpr int kMaxBlocks = 8192;
+
+template <int LPP, int NLPL>
+void launch_rf_t(const gm_lanecfg &lc, const gm_dsetup *dS, const double *models,
+                 int64_t mfs, int64_t mps, const float *obs, const int32_t *list,
+                 const int32_t *count, int64_t n_direct, int64_t grid_items, float *alpha,
+                 float *beta, int64_t aes, int64_t aps, int64_t bes, int64_t bps, double *chi,
+                 float *synth, float *rf, int mixed, hipStream_t s)
+{
+    const int region4 = (lc.rows * 9 > LPP * 15) ? lc.rows * 9 : LPP * 15;
+    const size_t shm = (size_t)lc.ppw * region4 * sizeof(float4);
+    const int g = grid_for(grid_items, lc.ppw, kMaxBlocks);
+    if (mixed)
+        hipLaunchKernelGGL((k_rf<LPP, NLPL, true>), dim3(g), dim3(64), shm, s, dS, models, mfs,
+                           mps, obs, list, count, n_direct, alpha, beta, aes, aps, bes, bps, chi, synth, rf,
+                           lc.rows, lc.padlo);
+    else
+        hipLaunchKernelGGL((k_rf<LPP, NLPL, false>), dim3(g), dim3(64), shm, s, dS, models, mfs,
+                           mps, obs, list, count, n_direct, alpha, beta, aes, aps, bes, bps, chi, synth, rf,
+                           lc.rows, lc.padlo);
+}
+
+template <int LPP, int NLPL>
+void launch_synth_t(const gm_lanecfg &lc, const gm_dsetup *dS, const double *models,
+                    int64_t mfs, int64_t mps, const float *obs, const int32_t *list,
+                    const int32_t *count, int64_t n_direct, int64_t grid_items, double *chi,
+                    float *synth, int mixed, hipStream_t s)
+{
+    const size_t shm = (size_t)lc.ppw * lc.rows * sizeof(float4);
+    const int g = grid_for(grid_items, lc.ppw, kMaxBlocks);
+    if (mixed)
+        hipLaunchKernelGGL((k_synth<LPP, NLPL, true>), dim3(g), dim3(64), shm, s, dS, models, mfs,
+                           mps, obs, list, count, n_direct, chi, synth, lc.rows, lc.padlo);
+    else
+        hipLaunchKernelGGL((k_synth<LPP, NLPL, false>), dim3(g), dim3(64), shm, s, dS, models,
+                           mfs, mps, obs, list, count, n_direct, chi, synth, lc.rows, lc.padlo);
+}
+
+} // namespace
+
+/* Model addressing: the LM state is SoA [11][cap] (field stride cap, pixel
+ * stride 1); caller arrays are AoS [npix][11] (field stride 1, pixel 11).   */
+static void split_strides(int64_t model_stride, int64_t &mfs, int64_t &mps)
+{
+    if (model_stride > 0) { mfs = model_stride; mps = 1; }
+    else { mfs = 1; mps = GM_NMODEL; }
+}
+
+void gm_launch_rf(const gm_lanecfg &lc, const gm_dsetup *dS, const double *models,
+                  int64_t model_stride, const float *obs, const int32_t *list,
+                  const int32_t *count, int64_t n_direct, int64_t grid_items, float *alpha,
+                  float *beta, int64_t ab_stride, double *chi_out, float *synth_out,
+                  float *rf_out, int mixed, hipStream_t st)
+{
+    int64_t mfs, mps;
+    split_strides(model_stride, mfs, mps);
+    /* alpha/beta: SoA over the state capacity (entry stride ab_stride, pixel
+     * stride 1) or, with ab_stride 0, the caller's AoS [npix][45] / [npix][9] */
+    const int64_t aes = ab_stride > 0 ? ab_stride : 1;
+    const int64_t aps = ab_stride > 0 ? 1 : 45;
+    const int64_t bes = ab_stride > 0 ? ab_stride : 1;
+    const int64_t bps = ab_stride > 0 ? 1 : 9;
+    if (lc.lpp == 32 && lc.nlpl == 1)
+        launch_rf_t<32, 1>(lc, dS, models, mfs, mps, obs, list, count, n_direct, grid_items, alpha,
+                           beta, aes, aps, bes, bps, chi_out, synth_out, rf_out, mixed, st);
+    else if (lc.lpp == 64 && lc.nlpl == 1)
+        launch_rf_t<64, 1>(lc, dS, models, mfs, mps, obs, list, count, n_direct, grid_items, alpha,
+                           beta, aes, aps, bes, bps, chi_out, synth_out, rf_out, mixed, st);
+    else if (lc.lpp == 64 && lc.nlpl == 2)
+        launch_rf_t<64, 2>(lc, dS, models, mfs, mps, obs, list, count, n_direct, grid_items, alpha,
+                           beta, aes, aps, bes, bps, chi_out, synth_out, rf_out, mixed, st);
+    else
+        launch_rf_t<64, 4>(lc, dS, models, mfs, mps, obs, list, count, n_direct, grid_items, alpha,
+                           beta, aes, aps, bes, bps, chi_out, synth_out, rf_out, mixed, st);
+}
+
+void gm_launch_synth(const gm_lanecfg &lc, const gm_dsetup *dS, const double *models,
+                     int64_t model_stride, const float *obs, const int32_t *list,
+                     const int32_t *count, int64_t n_direct, int64_t grid_items,
+                     double *chi_out, float *synth_out, int mixed, hipStream_t st)
+{
+    int64_t mfs, mps;
+    split_strides(model_stride, mfs, mps);
+    if (lc.lpp == 32 && lc.nlpl == 1)
+        launch_synth_t<32, 1>(lc, dS, models, mfs, mps, obs, list, count, n_direct, grid_items,
+                              chi_out, synth_out, mixed, st);
+    else if (lc.lpp == 64 && lc.nlpl == 1)
+        launch_synth_t<64, 1>(lc, dS, models, mfs, mps, obs, list, count, n_direct, grid_items,
+                              chi_out, synth_out, mixed, st);
+    else if (lc.lpp == 64 && lc.nlpl == 2)
+        launch_synth_t<64, 2>(lc, dS, models, mfs, mps, obs, list, count, n_direct, grid_items,
+                              chi_out, synth_out, mixed, st);
+    else
+        launch_synth_t<64, 4>(lc, dS, models, mfs, mps, obs, list, count, n_direct, grid_items,
+                              chi_out, synth_out, mixed, st);
+}
+
+void gm_launch_solve(const gm_dsetup *dS, gm_state st, const int32_t *list,
+                     const int32_t *count, int64_t grid_items, int mixed, int solver,
+                     int32_t *reset0, int32_t *reset1, hipStream_t s)
+{
+    const int g = grid_for(grid_items, 64, kMaxBlocks);
+    if (mixed) {
+        if (solver == GM_SOLVE_SVD)
+            hipLaunchKernelGGL((k_solve<true, GM_SOLVE_SVD>), dim3(g), dim3(64), 0, s, dS, st, list, count, reset0, reset1);
+        else
+            hipLaunchKernelGGL((k_solve<true, GM_SOLVE_LDL>), dim3(g), dim3(64), 0, s, dS, st, list, count, reset0, reset1);
+    } else {
+        if (solver == GM_SOLVE_SVD)
+            hipLaunchKernelGGL((k_solve<false, GM_SOLVE_SVD>), dim3(g), dim3(64), 0, s, dS, st, list, count, reset0, reset1);
+        else
+            hipLaunchKernelGGL((k_solve<false, GM_SOLVE_LDL>), dim3(g), dim3(64), 0, s, dS, st, list, count, reset0, reset1);
+    }
+}
+
+void gm_launch_step_solve(const float *alpha, const float *beta, const double *flambda,
+                          int64_t n, double *delta, int mixed, int solver, hipStream_t s)
+{
+    const int g = grid_for(n, 64, kMaxBlocks);
+    if (mixed) {
+        if (solver == GM_SOLVE_SVD)
+            hipLaunchKernelGGL((k_step_solve<true, GM_SOLVE_SVD>), dim3(g), dim3(64), 0, s, alpha, beta, flambda, n, delta);
+        else
+            hipLaunchKernelGGL((k_step_solve<true, GM_SOLVE_LDL>), dim3(g), dim3(64), 0, s, alpha, beta, flambda, n, delta);
+    } else {
+        if (solver == GM_SOLVE_SVD)
+            hipLaunchKernelGGL((k_step_solve<false, GM_SOLVE_SVD>), dim3(g), dim3(64), 0, s, alpha, beta, flambda, n, delta);
+        else
+            hipLaunchKernelGGL((k_step_solve<false, GM_SOLVE_LDL>), dim3(g), dim3(64), 0, s, alpha, beta, flambda, n, delta);
+    }
+}
+
+void gm_launch_decide(const gm_dsetup *dS, gm_state st, const int32_t *list,
+                      const int32_t *count, int64_t grid_items, int32_t *next_list,
+                      int32_t *next_count, int32_t *acc_list, int32_t *acc_count, int mixed,
+                      hipStream_t s)
+{
+    const int g = grid_for(grid_items, 64, kMaxBlocks);
+    if (mixed)
+        hipLaunchKernelGGL((k_decide<true>), dim3(g), dim3(64), 0, s, dS, st, list, count,
+                           next_list, next_count, acc_list, acc_count);
+    else
+        hipLaunchKernelGGL((k_decide<false>), dim3(g), dim3(64), 0, s, dS, st, list, count,
+                           next_list, next_count, acc_list, acc_count);
+}
+
+void gm_launch_init_state(const gm_dsetup *dS, gm_state st, const float *obs,
+                          const double *init, int init_mode, int64_t npix, int32_t *list0,
+                          int mixed, hipStream_t s)
+{
+    (void)mixed;
+    const int g = grid_for(npix, 64, kMaxBlocks);
+    hipLaunchKernelGGL(k_init_state, dim3(g), dim3(64), 0, s, dS, st, obs, init, init_mode, npix,
+                       list0);
+}
+
+void gm_launch_init_chi(gm_state st, int64_t npix, hipStream_t s)
+{
+    const int g = grid_for(npix, 256, kMaxBlocks);
+    hipLaunchKernelGGL(k_init_chi, dim3(g), dim3(256), 0, s, st, npix);
+}
+
+void gm_launch_finalize(gm_state st, int64_t npix, const gm_out &out, hipStream_t s)
+{
+    const int g = grid_for(npix, 256, kMaxBlocks);
+    hipLaunchKernelGGL(k_finalize, dim3(g), dim3(256), 0, s, st, npix, out.model, out.chisqr,
+                       out.iters, out.naccs);
+}
+
+void gm_launch_ce(const gm_dsetup *dS, const float *obs, int64_t npix, double *models,
+                  hipStream_t s)
+{
+    const int g = grid_for(npix, 64, kMaxBlocks);
+    hipLaunchKernelGGL(k_ce, dim3(g), dim3(64), 0, s, dS, obs, npix, models);
+}
+
+void gm_launch_set_count(int32_t *count, int32_t value, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, s, count, value);
+}

@@ -1,0 +1,197 @@
+/*
+ * gmilos.h -- C ABI of the MI355X-native G-MILOS inversion engine.
+ *
+ * The reference exposes its per-pixel Levenberg-Marquardt engine as the CUDA
+ * kernel lm_mils / lm_mils_11 (milosUtils.cuh:54-60) plus sixteen __constant__
+ * symbols set with cudaMemcpyToSymbol (gmilos.cu:46-68, 1184-1198), driven by
+ * the host code of gmilos.cu:914-1322.  This header replaces that surface with
+ * a context object that carries the same constant set (gm_setup) and entry
+ * points taking plain device pointers and sizes.
+ *
+ * Conventions
+ *  - Every buffer is caller-owned device memory unless stated otherwise.
+ *  - Every call is stream-ordered on the hipStream_t passed as `void *stream`
+ *    (NULL = the default stream) and returns a gm_status; nothing calls exit().
+ *  - A context is reentrant with respect to other contexts; calls on one
+ *    context must not overlap in time (they share its workspace).
+ *  - Layouts follow the reference: observed / synthetic spectra are float
+ *    [npix][4][nlambda] (I block, Q, U, V; utilsFits.c:605-608), models are
+ *    double [npix][11] in Init_Model order eta0, B, vlos, dopp, aa, gm, az,
+ *    S0, S1, mac, alfa (defines.h:94-106).
+ */
+#ifndef GMILOS_H
+#define GMILOS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GM_MAX_LAMBDA  256   /* wavelengths per pixel                       */
+#define GM_MAX_LINES   4     /* spectral lines sharing one atmosphere        */
+#define GM_MAX_COMP    10    /* Zeeman sub-components per type and line      */
+#define GM_NMODEL      11    /* Init_Model fields                            */
+#define GM_NFREE       9     /* response-function slots solved (0..8)        */
+
+typedef enum {
+    GM_OK = 0,
+    GM_ERR_INVALID = 1,      /* bad argument or inconsistent setup          */
+    GM_ERR_HIP = 2,          /* a HIP runtime call failed                   */
+    GM_ERR_NOMEM = 3,        /* device allocation failed                    */
+    GM_ERR_UNSUPPORTED = 4,  /* valid for the reference, not built here     */
+    GM_ERR_NOT_FOUND = 5     /* requested item absent (e.g. LINES index)     */
+} gm_status;
+
+typedef enum {
+    GM_PREC_FP32 = 0,        /* everything FP32 incl. solve and models      */
+    GM_PREC_MIXED = 1        /* FP32 synthesis/RF/PSF/normal equations; FP64
+                                solve, chi^2 accumulation and accept test,
+                                FP64 models                                  */
+} gm_precision;
+
+typedef enum {
+    GM_SOLVE_SVD = 0,        /* symmetric SVD (Jacobi), pseudo-inverse with the
+                                reference's |w| > 1e-12 rule                */
+    GM_SOLVE_LDL = 1         /* pivot-thresholded LDL^T of the damped matrix */
+} gm_solver;
+
+typedef enum {
+    GM_INIT_CE = 0,          /* init model, then classical estimates overwrite
+                                B, vlos, gamma, phi (milosUtils.cu:615-628)  */
+    GM_INIT_MODEL = 1,       /* init model as given for every pixel          */
+    GM_INIT_PER_PIXEL = 2    /* caller passes a per-pixel start model        */
+} gm_init_mode;
+
+/* Zeeman pattern of one line, the reference's Cuantic (defines.h:108-123). */
+typedef struct {
+    int32_t n_pi, n_sig;
+    double  nub[GM_MAX_COMP], nup[GM_MAX_COMP], nur[GM_MAX_COMP];
+    double  web[GM_MAX_COMP], wep[GM_MAX_COMP], wer[GM_MAX_COMP];
+    double  gl, gu, geff, fo;
+} gm_zeeman;
+
+/* The constant set the reference uploads with cudaMemcpyToSymbol
+ * (gmilos.cu:1184-1198) plus the new engine's choices.                      */
+typedef struct {
+    int32_t nlambda;
+    double  lambda[GM_MAX_LAMBDA];   /* absolute wavelengths (A)  d_lambda_const */
+    int32_t nlines;
+    double  lambda0[GM_MAX_LINES];   /* line centres (A)         d_wlines_const */
+    gm_zeeman zee[GM_MAX_LINES];     /*                           d_cuantic_const */
+    int32_t use_psf;                 /*                    d_use_convolution_const */
+    double  psf[GM_MAX_LAMBDA];      /* nlambda taps, centre at nlambda/2       */
+    double  weight[4];               /*                           d_weight_const */
+    double  noise;                   /* sigma^2                   d_sigma_const  */
+    double  ilambda;                 /* initial Marquardt lambda  d_ilambda_const */
+    double  toplim;                  /*                           d_toplim_const */
+    int32_t miter;                   /* max LM iterations         d_miter_const  */
+    double  mu;                      /* cos(theta)                d_ah_const     */
+    int32_t logclambda;              /*                           d_logclambda_const */
+    int32_t fix[11];                 /* 1 = free, Init_Model order d_fix_const  */
+    double  init_model[GM_NMODEL];   /*                           d_initModel_const */
+    int32_t init_mode;               /* gm_init_mode                             */
+    int32_t ce_line;                 /* line used by the classical estimates     */
+    int32_t precision;               /* gm_precision                             */
+    int32_t solver;                  /* gm_solver                                */
+    int32_t rf_reference_rules;      /* 1 = reference response-function scaling  */
+    double  psf_tail_tol;            /* drop PSF taps whose total tail mass is
+                                        below this fraction (0 keeps all taps)  */
+} gm_setup;
+
+/* Outputs of an inversion (caller-owned device memory; synth and naccs may
+ * be NULL).  Same content as the reference's d_vModels, d_vChisqrf,
+ * d_vNumIter and d_spectraAdjusted (gmilos.cu:1176-1179).                   */
+typedef struct {
+    double  *model;                  /* [npix][11] best model                   */
+    float   *chisqr;                 /* [npix] best chi^2                        */
+    int32_t *iters;                  /* [npix] LM iterations                     */
+    float   *synth;                  /* [npix][4][nlambda] last evaluated synth  */
+    int32_t *naccs;                  /* [npix] accepted steps that rebuilt the
+                                        normal equations (for work accounting)  */
+} gm_out;
+
+/* Per-call statistics (host memory) for measurement. */
+typedef struct {
+    int64_t npix;
+    int64_t synth_evals;             /* pixel-evaluations of synth+chi^2        */
+    int64_t rf_evals;                /* pixel-evaluations of RF+normal eqs       */
+    int64_t solves;                  /* pixel step solves                        */
+    int32_t lm_rounds;               /* LM rounds executed                       */
+    int32_t psf_taps;                /* PSF taps applied per sample              */
+    float   ms_total;                /* wall time of the device work (events)    */
+    float   ms_synth, ms_rf, ms_solve, ms_other; /* per-kernel-class totals
+                                        (only when timing is enabled)            */
+} gm_stats;
+
+typedef struct gm_ctx gm_ctx;
+
+/* ---- context ------------------------------------------------------------ */
+/* Replaces the constant uploads of gmilos.cu:1184-1198 and the occupancy /
+ * heap set-up of gmilos.cu:1008-1050.                                       */
+gm_status gm_ctx_create(const gm_setup *setup, gm_ctx **ctx);
+gm_status gm_ctx_destroy(gm_ctx *ctx);
+const char *gm_status_string(gm_status st);
+/* Enables per-kernel-class event timing in gm_stats (adds syncs). */
+gm_status gm_ctx_set_timing(gm_ctx *ctx, int32_t enable);
+gm_status gm_last_stats(const gm_ctx *ctx, gm_stats *stats);
+
+/* ---- inversion ---------------------------------------------------------- */
+/* Inverts npix pixels: the work of lm_mils (milosUtils.cu:574-710) launched
+ * by gmilos.cu:1233-1240.  init is [npix][11] when init_mode is
+ * GM_INIT_PER_PIXEL, otherwise ignored (may be NULL).                      */
+gm_status gm_invert(gm_ctx *ctx, const float *obs, int64_t npix,
+                    const double *init, const gm_out *out, void *stream);
+
+/* ---- path pieces, exposed for parity tests and the serving loop --------- */
+/* Synthesis of npix models (mil_sinrf, mil_sinrf.cu:27-142, with the PSF). */
+gm_status gm_synth(gm_ctx *ctx, const double *models, int64_t npix,
+                   float *spectra, void *stream);
+/* Synthesis + response functions (me_der, me_der.cu:81-215) in the
+ * reference layout rf[npix][4][10][nlambda] after the PSF and the zeroing of
+ * fixed parameters (milosUtils.cu:243-271).                                */
+gm_status gm_synth_rf(gm_ctx *ctx, const double *models, int64_t npix,
+                      float *spectra, float *rf, void *stream);
+/* Normal equations and chi^2 at the given models (covarmf + fchisqr,
+ * lib.cu:65-155): alpha packed upper triangle [npix][45] over slots 0..8,
+ * beta [npix][9], chisqr [npix] (already divided by nfree).                */
+gm_status gm_normal_equations(gm_ctx *ctx, const double *models,
+                              const float *obs, int64_t npix, float *alpha,
+                              float *beta, double *chisqr, void *stream);
+/* Damped step solve (milosUtils.cu:647-653 + mil_svd, milosUtils.cu:280-325)
+ * for npix systems: delta[npix][9] = solve(alpha with diagonal*(1+flambda),
+ * beta) using the context's precision and solver.                          */
+gm_status gm_step_solve(gm_ctx *ctx, const float *alpha, const float *beta,
+                        const double *flambda, int64_t npix, double *delta,
+                        void *stream);
+/* Classical estimates (estimacionesClasicas, milosUtils.cu:386-563) applied
+ * to the init model with the NaN guards of milosUtils.cu:621-628.          */
+gm_status gm_classical_estimates(gm_ctx *ctx, const float *obs, int64_t npix,
+                                 double *models, void *stream);
+
+/* ---- host-side set-up helpers (no device work) -------------------------- */
+/* Zeeman pattern from quantum numbers (create_cuantic.c:31-215). */
+gm_status gm_zeeman_from_quantum(double sl, double ll, double jl, double su,
+                                 double lu, double ju, double fo,
+                                 gm_zeeman *out);
+/* Parses one LINES record with index `line_index` (readConfig.c:388-492);
+ * returns the central wavelength and the quantum numbers. */
+gm_status gm_parse_lines_file(const char *path, int32_t line_index,
+                              double *lambda0, double qn[6]);
+/* Gaussian PSF of FWHM (mA) on the grid, centred at lambda[nlambda/2]
+ * (fgaussC.c:14-87, gmilos.cu:244-245). */
+gm_status gm_psf_gaussian(double fwhm_mA, const double *lambda,
+                          int32_t nlambda, double *psf);
+/* Tabulated PSF (mA, value) resampled onto the grid offsets (mA) with the
+ * reference's centring offset and unit-sum normalisation (gmilos.cu:247-291,
+ * interpolLineal.c:46-98). */
+gm_status gm_psf_from_table(const double *dl_mA, const double *val,
+                            int32_t ntab, const double *offsets_mA,
+                            int32_t nlambda, double step_mA, double *psf);
+/* Fills a gm_setup with the reference defaults (readConfig.c:704-744). */
+void gm_setup_defaults(gm_setup *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GMILOS_H */

@@ -1,0 +1,259 @@
+"""GPU parity: every piece of the HIP path and the whole LM inversion against
+the FP64 oracle on the same seeded inputs.
+
+Tolerances (the HIP path computes synthesis, response functions, PSF and
+normal equations in FP32):
+  synthesis          |d| <= 5e-6 * Ic + 2e-5 * |x|
+  response functions |d| <= 5e-4 * max|RF vector| (per Stokes and parameter)
+  normal equations   |d alpha| <= 1e-3 * max|alpha|, |d beta| <= 2e-3 * max|beta|
+  step solve         FP64: rel 1e-6; FP32: rel 2e-2 (of |x|)
+  estimates (FP64)   rel 1e-9
+  inversion          medians of |d| vs the oracle's inversion: B <= 1 G,
+                     gamma, phi <= 0.1 deg, vlos <= 1 m/s
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, crisp_table, model_row, read_mod
+from workload import observations, param_errors
+
+pytestmark = pytest.mark.gpu
+
+FREE9 = ("eta0", "B", "vlos", "dopp", "aa", "gm", "az", "S0", "S1")
+
+
+def pair(gm, orc, kind="single", psf=None, **kw):
+    """Product and oracle set-ups from one description."""
+    okw = dict(kw)
+    pkw = dict(kw)
+    for k in ("precision", "solver", "psf_tail_tol", "init_mode"):
+        okw.pop(k, None)
+    if pkw.get("init_mode", gm.GM_INIT_CE) != gm.GM_INIT_CE:
+        okw["use_ce"] = 0
+    if kind == "single":
+        lines, grid = [gm.FE6173], gm.GRID_6173
+    else:
+        lines, grid = [gm.FE6301, gm.FE6302_PAIR], gm.GRID_6302
+        pkw.setdefault("ce_line", 1)
+        okw.setdefault("ce_line", 1)
+    okw.setdefault("init_defaults", gm.BENCH_INIT)
+    okw.setdefault("free", FREE9)
+    p = gm.make_setup(lines, grid, psf=psf, **pkw)
+    o = orc.make_setup(lines, grid, psf=psf, **okw)
+    return p, o
+
+
+def to_dev(x, cuda):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x)).to(cuda)
+
+
+PSFS = {"nopsf": None, "gauss60": ("gauss", 60.0), "crisp": "crisp"}
+
+
+def _psf(name):
+    if name == "crisp":
+        dl, val = crisp_table()
+        return ("table", dl, val)
+    return PSFS[name]
+
+
+@pytest.mark.parametrize("kind,psfname", [("single", "nopsf"), ("single", "gauss60"),
+                                          ("single", "crisp"), ("two", "gauss60")])
+def test_synthesis_matches_oracle(gm, orc, cuda, kind, psfname):
+    p, o = pair(gm, orc, kind, _psf(psfname))
+    models = gm.random_models(256, seed=11)
+    ctx = gm.Context(p)
+    got = ctx.synth(to_dev(models, cuda)).cpu().numpy()
+    ref = orc.synth(o, models)
+    ic = (models[:, 7] + models[:, 8])[:, None, None]
+    tol = 5e-6 * ic + 2e-5 * np.abs(ref)
+    assert (np.abs(got - ref) <= tol).all(), np.abs(got - ref).max()
+
+
+@pytest.mark.parametrize("rules", [1, 0])
+@pytest.mark.parametrize("kind,psfname", [("single", "nopsf"), ("single", "gauss60"),
+                                          ("single", "crisp"), ("two", "gauss60")])
+def test_response_functions_match_oracle(gm, orc, cuda, kind, psfname, rules):
+    p, o = pair(gm, orc, kind, _psf(psfname), rf_reference_rules=rules)
+    models = gm.random_models(128, seed=12)
+    ctx = gm.Context(p)
+    sp, rf = ctx.synth_rf(to_dev(models, cuda))
+    rf = rf.cpu().numpy()
+    sp_ref, rf_ref = orc.synth_rf(o, models)
+    np.testing.assert_allclose(sp.cpu().numpy(), sp_ref, atol=2e-5)
+    scale = np.abs(rf_ref).max(axis=-1, keepdims=True)
+    bad = np.abs(rf - rf_ref) > 5e-4 * scale + 1e-12
+    assert not bad.any(), np.argwhere(bad)[:5]
+    assert np.all(rf[:, :, 9, :] == 0)
+
+
+def test_normal_equations_match_oracle(gm, orc, cuda):
+    p, o = pair(gm, orc, "single", ("gauss", 60.0))
+    truth = gm.random_models(128, seed=13)
+    obs = observations(orc, o, truth, seed=13)
+    models = gm.random_models(128, seed=14)       # away from the optimum
+    ctx = gm.Context(p)
+    a, b, c = ctx.normal_equations(to_dev(models, cuda), to_dev(obs, cuda))
+    a, b, c = a.cpu().numpy(), b.cpu().numpy(), c.cpu().numpy()
+    sp, rf = orc.synth_rf(o, models)
+    for i in range(len(models)):
+        A, B = orc.covarm(o, obs[i], sp[i], rf[i])
+        A9 = gm.unpack_alpha(a[i])
+        assert np.abs(A9 - A[:9, :9]).max() <= 1e-3 * np.abs(A).max()
+        assert np.abs(b[i] - B[:9]).max() <= 2e-3 * np.abs(B).max()
+        assert c[i] == pytest.approx(orc.chisqr(o, obs[i], sp[i]), rel=2e-5)
+
+
+def _random_systems(n, seed, degenerate=False):
+    rng = np.random.default_rng(seed)
+    A = np.zeros((n, 9, 9))
+    for i in range(n):
+        J = rng.normal(size=(120, 9)) * rng.uniform(0.01, 10, 9)
+        A[i] = J.T @ J
+        if degenerate and i % 3 == 0:
+            k = i % 9
+            A[i, k, :] = A[i, :, k] = 0.0
+    b = rng.normal(size=(n, 9)) * 10
+    lam = 10.0 ** rng.uniform(-4, 2, n)
+    return A, b, lam
+
+
+@pytest.mark.parametrize("precision,solver", [(1, 0), (1, 1), (0, 0), (0, 1)])
+def test_step_solve_matches_oracle(gm, orc, cuda, precision, solver):
+    p, _ = pair(gm, orc, "single", None, precision=precision, solver=solver)
+    ctx = gm.Context(p)
+    A, b, lam = _random_systems(256, 5, degenerate=True)
+    packed = np.zeros((256, 45), dtype=np.float32)
+    for q in range(9):
+        for r in range(q, 9):
+            packed[:, gm.packed_index(q, r)] = A[:, q, r]
+    b32 = b.astype(np.float32)
+    d = ctx.step_solve(to_dev(packed, cuda), to_dev(b32, cuda), to_dev(lam, cuda))
+    d = d.cpu().numpy()
+    rtol = 1e-6 if precision == 1 else 2e-2
+    for i in range(256):
+        Af = gm.unpack_alpha(packed[i]).astype(np.float64)
+        Af[np.diag_indices(9)] *= (1.0 + lam[i])
+        x, _ = orc.svd_solve(Af, b32[i].astype(np.float64))
+        assert np.abs(d[i] - x).max() <= rtol * np.abs(x).max() + 1e-12, (i, d[i], x)
+
+
+def test_classical_estimates_match_oracle(gm, orc, cuda):
+    p, o = pair(gm, orc, "single", ("gauss", 60.0))
+    truth = gm.random_models(512, seed=21)
+    obs = observations(orc, o, truth, seed=21)
+    obs[0] = 0.0                            # NaN guards
+    obs[1, 0, :3] = -1.0                    # ghost samples
+    ctx = gm.Context(p)
+    got = ctx.classical_estimates(to_dev(obs, cuda)).cpu().numpy()
+    for i in range(len(obs)):
+        ref = orc.classical_estimates(o, obs[i])
+        for f in (1, 2, 5, 6):
+            if np.isnan(ref[f]):
+                ref[f] = {1: 1.0, 2: 1e-3, 5: 1.0, 6: 1.0}[f]
+        np.testing.assert_allclose(got[i], ref, rtol=1e-9, atol=1e-9)
+
+
+def _inversion_errors(gm, orc, cuda, kind, psf, npix, seed, **kw):
+    p, o = pair(gm, orc, kind, psf, **kw)
+    truth = gm.random_models(npix, seed=seed, weak_fraction=0.1 if kind == "two" else 0.0)
+    obs = observations(orc, o, truth, seed=seed, high_fraction=0.05 if kind == "two" else 0.0)
+    ctx = gm.Context(p)
+    out = ctx.invert(to_dev(obs, cuda))
+    got = {k: v.cpu().numpy() for k, v in out.items()}
+    ref = orc.invert(o, obs)
+    return got, ref, truth, obs, ctx
+
+
+def _assert_medians(d, label=""):
+    med = np.median(d, axis=0)
+    print(label, "median |d|: B %.3g G, vlos %.3g m/s, gamma %.3g deg, phi %.3g deg" %
+          (med[1], med[2] * 1e3, med[5], med[6]))
+    assert med[1] <= 1.0 and med[2] <= 1e-3 and med[5] <= 0.1 and med[6] <= 0.1, med
+
+
+@pytest.mark.parametrize("precision,solver", [(1, 0), (0, 0), (1, 1)])
+def test_inversion_config0_matches_oracle(gm, orc, cuda, precision, solver):
+    """64 x 64 pixels, Fe I 6173, 30 wavelengths, no PSF, 9 free, 50 it."""
+    got, ref, truth, obs, ctx = _inversion_errors(gm, orc, cuda, "single", None, 4096, 100,
+                                                  precision=precision, solver=solver)
+    d = param_errors(got["model"], ref["model"])
+    _assert_medians(d, f"prec {precision} solver {solver}")
+    assert np.all(np.isfinite(got["model"])) and np.all(got["iters"] >= 1)
+    assert np.all(got["iters"] <= 50)
+    rel_c2 = np.abs(got["chisqr"] - ref["chisqr"]) / ref["chisqr"]
+    assert np.median(rel_c2) < 1e-3
+    st = ctx.stats()
+    assert st["npix"] == 4096 and st["lm_rounds"] >= 1
+
+
+def test_inversion_config1_psf_matches_oracle(gm, orc, cuda):
+    """Config-1 line, grid and PSF (Gaussian FWHM 60 mA) on 2048 pixels."""
+    got, ref, truth, obs, ctx = _inversion_errors(gm, orc, cuda, "single", ("gauss", 60.0),
+                                                  2048, 200)
+    _assert_medians(param_errors(got["model"], ref["model"]), "cfg1")
+    # synthetic output is the last evaluated model's spectrum
+    assert np.all(np.isfinite(got["synth"]))
+
+
+def test_inversion_two_lines_matches_oracle(gm, orc, cuda):
+    """Config-2 shape: 6301.5 + 6302.5, 112 wavelengths, PSF, weak-field and
+    high-noise pixels, on 512 pixels."""
+    got, ref, truth, obs, ctx = _inversion_errors(gm, orc, cuda, "two", ("gauss", 60.0), 512,
+                                                  300)
+    _assert_medians(param_errors(got["model"], ref["model"]), "cfg2")
+
+
+def test_inversion_recovers_reference_example(gm, orc, cuda):
+    per = np.loadtxt(os.path.join(GOLDEN, "synthesis_gpu.per"))
+    obs = np.ascontiguousarray(per[:, 2:6].T[None].astype(np.float32))
+    guess = read_mod(os.path.join(GOLDEN, "guess.mod"))
+    truth = model_row(read_mod(os.path.join(GOLDEN, "init.mod")))
+    dl, val = crisp_table()
+    p = gm.make_setup([gm.FE6173], gm.GRID_6173, psf=("table", dl, val), init=guess,
+                      miter=100)
+    out = gm.Context(p).invert(to_dev(obs, cuda))
+    np.testing.assert_allclose(out["model"].cpu().numpy()[0, :9], truth[:9], rtol=5e-3, atol=2e-3)
+
+
+def test_inversion_edge_cases(gm, orc, cuda):
+    import torch
+    p, o = pair(gm, orc, "single", ("gauss", 60.0), miter=20)
+    ctx = gm.Context(p)
+    # empty batch
+    e = torch.empty((0, 4, 30), dtype=torch.float32, device=cuda)
+    out = ctx.invert(e)
+    assert out["model"].shape == (0, 11)
+    # odd sizes, one pixel, degenerate pixels: finite outputs, no fault
+    truth = gm.random_models(7, seed=3)
+    obs = observations(orc, o, truth, seed=3)
+    obs[0] = 0.0
+    obs[1] = np.nan
+    obs[2] = 1e30
+    obs[3, :, ::2] = -1.0
+    for n in (1, 3, 7):
+        out = ctx.invert(to_dev(obs[:n], cuda))
+        torch.cuda.synchronize()
+        it = out["iters"].cpu().numpy()
+        assert np.all(it >= 1) and np.all(it <= 20)
+    good = ctx.invert(to_dev(obs[4:], cuda))
+    assert np.all(np.isfinite(good["model"].cpu().numpy()))
+
+
+def test_fixed_parameters_stay_fixed(gm, orc, cuda):
+    """Only B, gamma, phi free: the rest keep the initial model's values."""
+    p, o = pair(gm, orc, "single", ("gauss", 60.0), free=("B", "gm", "az"))
+    truth = gm.random_models(256, seed=31)
+    obs = observations(orc, o, truth, seed=31)
+    out = gm.Context(p).invert(to_dev(obs, cuda))
+    m = out["model"].cpu().numpy()
+    for f in (0, 2, 3, 4, 7, 8):
+        if f == 2:
+            continue    # vlos is set by the classical estimates
+        np.testing.assert_array_equal(m[:, f], p.init_model[f])
+    ref = orc.invert(o, obs)
+    _assert_medians(param_errors(m, ref["model"]), "3 free")
