@@ -4,7 +4,8 @@ the FP64 oracle on the same seeded inputs.
 Tolerances (the HIP path computes synthesis, response functions, PSF and
 normal equations in FP32):
   synthesis          |d| <= 5e-6 * Ic + 2e-5 * |x|
-  response functions |d| <= 5e-4 * max|RF vector| (per Stokes and parameter)
+  response functions |d| <= 5e-4 * max|RF vector| + 2e-7 Ic / dp, dp = 1% of the
+                     parameter range (per Stokes and parameter)
   normal equations   |d alpha| <= 1e-3 * max|alpha|, |d beta| <= 2e-3 * max|beta|
   step solve         FP64: rel 1e-6; FP32: rel 2e-2 (of |x|)
   estimates (FP64)   rel 1e-9
@@ -74,6 +75,12 @@ def test_synthesis_matches_oracle(gm, orc, cuda, kind, psfname):
     assert (np.abs(got - ref) <= tol).all(), np.abs(got - ref).max()
 
 
+# 1% of each free parameter's benchmark range: an RF error times this step is
+# the error of the predicted Stokes change, compared with a fraction of Ic
+# (weak-field Q/U responses are FP32 differences of nearly equal profiles).
+DP_TYP = (0.2, 30.0, 0.06, 0.0004, 0.005, 1.8, 1.8, 0.003, 0.004)
+
+
 @pytest.mark.parametrize("rules", [1, 0])
 @pytest.mark.parametrize("kind,psfname", [("single", "nopsf"), ("single", "gauss60"),
                                           ("single", "crisp"), ("two", "gauss60")])
@@ -86,7 +93,13 @@ def test_response_functions_match_oracle(gm, orc, cuda, kind, psfname, rules):
     sp_ref, rf_ref = orc.synth_rf(o, models)
     np.testing.assert_allclose(sp.cpu().numpy(), sp_ref, atol=2e-5)
     scale = np.abs(rf_ref).max(axis=-1, keepdims=True)
-    bad = np.abs(rf - rf_ref) > 5e-4 * scale + 1e-12
+    ic = (models[:, 7] + models[:, 8])[:, None, None, None]
+    step = np.array(DP_TYP + (1.0,))[None, None, :, None]
+    tol = 5e-4 * scale + 2e-7 * ic / step
+    bad = np.abs(rf - rf_ref) > tol
+    if bad.any():
+        i = np.argwhere(bad)[0]
+        print("worst case", i, rf[tuple(i)], rf_ref[tuple(i)], tol[tuple(i)])
     assert not bad.any(), np.argwhere(bad)[:5]
     assert np.all(rf[:, :, 9, :] == 0)
 
