@@ -115,7 +115,7 @@ def load_library(path=LIB_PATH):
         "gm_synth_rf": ([vp, vp, i64, vp, vp, vp], i32),
         "gm_normal_equations": ([vp, vp, vp, i64, vp, vp, vp, vp], i32),
         "gm_step_solve": ([vp, vp, vp, vp, i64, vp, vp], i32),
-        "gm_classical_estimates": ([vp, vp, i64, vp, vp], i32),
+        "gm_classical_estimates": ([vp, vp, i64, i32, vp, vp], i32),
         "gm_zeeman_from_quantum": ([ctypes.c_double] * 7 + [P(gm_zeeman)], i32),
         "gm_parse_lines_file": ([ctypes.c_char_p, i32, P(ctypes.c_double),
                                  P(ctypes.c_double)], i32),
@@ -424,11 +424,12 @@ class Context:
                "gm_step_solve")
         return d
 
-    def classical_estimates(self, obs):
+    def classical_estimates(self, obs, for_initial_use=True):
         import torch
         npix = obs.shape[0]
         m = torch.empty((npix, GM_NMODEL), dtype=torch.float64, device=obs.device)
         _check(self.lib.gm_classical_estimates(self.h, ctypes.c_void_p(obs.data_ptr()), npix,
+                                               1 if for_initial_use else 0,
                                                ctypes.c_void_p(m.data_ptr()), self._stream()),
                "gm_classical_estimates")
         return m

@@ -12,8 +12,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "lib", "libgmilos.so")
 SOURCES = ["csrc/gm_kernels.hip", "csrc/gm_capi.cpp", "csrc/gm_setup.cpp"]
-CLI_SOURCES = ["host/gmilos_cli.cpp", "host/gm_io.cpp", "host/gm_fits.cpp"]
+IO_SOURCES = ["host/gm_io.cpp", "host/gm_fits.cpp"]
+IO_LIB = os.path.join(HERE, "lib", "libgmilos_io.so")
+CLI_SOURCES = ["host/gmilos_cli.cpp"]
 CLI = os.path.join(HERE, "bin", "gmilos")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -44,20 +47,32 @@ def build_lib(verbose=False):
     return LIB
 
 
+def build_io(verbose=False):
+    """Host-only file-format library (.mtrol, .mod, .grid, .psf, .per, FITS)."""
+    os.makedirs(os.path.dirname(IO_LIB), exist_ok=True)
+    tmp = IO_LIB + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall",
+          "-I" + os.path.join(ROOT, "include")] + IO_SOURCES + ["-o", tmp])
+    os.replace(tmp, IO_LIB)
+    return IO_LIB
+
+
 def build_cli(verbose=False):
-    """Builds the ``gmilos`` command-line drop-in (host C++ over the C ABI)."""
-    if not all(os.path.exists(os.path.join(HERE, s)) for s in CLI_SOURCES):
-        return None
+    """The ``gmilos <file>.mtrol`` command (host C++ over both libraries)."""
     os.makedirs(os.path.dirname(CLI), exist_ok=True)
-    cmd = [HIPCC, "-O2", "-std=c++17", "-Wno-unused-result",
-           "-I" + os.path.join(ROOT, "include"), "-Ihost"] + CLI_SOURCES + \
-          ["-L" + os.path.dirname(LIB), "-lgmilos", "-Wl,-rpath,$ORIGIN/../lib", "-o", CLI]
-    _run(cmd)
+    tmp = CLI + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__",
+          "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(ROCM, "include")] +
+         CLI_SOURCES + ["-L" + os.path.dirname(LIB), "-lgmilos", "-lgmilos_io",
+                        "-L" + os.path.join(ROCM, "lib"), "-lamdhip64",
+                        "-Wl,-rpath,$ORIGIN/../lib:" + os.path.join(ROCM, "lib"), "-o", tmp])
+    os.replace(tmp, CLI)
     return CLI
 
 
 def build(verbose=False):
     build_lib(verbose)
+    build_io(verbose)
     build_cli(verbose)
     return LIB
 

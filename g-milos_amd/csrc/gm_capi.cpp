@@ -484,10 +484,11 @@ extern "C" gm_status gm_step_solve(gm_ctx *c, const float *alpha, const float *b
 }
 
 extern "C" gm_status gm_classical_estimates(gm_ctx *c, const float *obs, int64_t npix,
-                                            double *models, void *stream)
+                                            int32_t for_initial_use, double *models,
+                                            void *stream)
 {
     if (!c || !obs || !models || npix < 0) return GM_ERR_INVALID;
     if (npix == 0) return GM_OK;
-    gm_launch_ce(c->dS, obs, npix, models, (hipStream_t)stream);
+    gm_launch_ce(c->dS, obs, npix, for_initial_use ? 1 : 0, models, (hipStream_t)stream);
     return hip_ok(hipGetLastError());
 }

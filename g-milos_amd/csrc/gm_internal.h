@@ -103,7 +103,7 @@ void gm_launch_init_state(const gm_dsetup *dS, gm_state st, const float *obs,
 void gm_launch_init_chi(gm_state st, int64_t npix, hipStream_t stream);
 void gm_launch_finalize(gm_state st, int64_t npix, const gm_out &out,
                         hipStream_t stream);
-void gm_launch_ce(const gm_dsetup *dS, const float *obs, int64_t npix,
+void gm_launch_ce(const gm_dsetup *dS, const float *obs, int64_t npix, int initial,
                   double *models, hipStream_t stream);
 void gm_launch_set_count(int32_t *count, int32_t value, hipStream_t stream);
 

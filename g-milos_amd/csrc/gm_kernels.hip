@@ -868,7 +868,7 @@ __global__ __launch_bounds__(64) void k_decide(const gm_dsetup *__restrict__ S, 
 
 /* Classical estimates, FP64 (milosUtils.cu:386-563), pixel-parallel. */
 __device__ void classical_estimates(const gm_dsetup *__restrict__ S, const float *sp,
-                                    double *model)
+                                    double *model, int initial)
 {
     const int n = S->nlambda;
     const int lo = S->ce_lo, hi = S->ce_hi;
@@ -919,8 +919,10 @@ __device__ void classical_estimates(const gm_dsetup *__restrict__ S, const float
     const double tan_gamma = fabs(sqrt(x / y));
     double gamma_rad = atan(tan_gamma);
     double gamma = gamma_rad * (180 / kPI);
-    if (gamma >= 85 && gamma <= 90) gamma_rad = 85 * (kPI / 180);
-    if (gamma > 90 && gamma <= 95) gamma_rad = 95 * (kPI / 180);
+    if (initial) {
+        if (gamma >= 85 && gamma <= 90) gamma_rad = 85 * (kPI / 180);
+        if (gamma > 90 && gamma <= 95) gamma_rad = 95 * (kPI / 180);
+    }
     if (Blos < 0) gamma = (180) - gamma;
     double sum_u = 0.0, sum_q = 0.0;
     for (i = lo; i < hi; i++) {
@@ -944,6 +946,10 @@ __device__ void classical_estimates(const gm_dsetup *__restrict__ S, const float
     model[2] = Vlos;
     model[5] = gamma;
     model[6] = phi;
+    if (!initial) {
+        model[7] = Blos;
+        return;
+    }
     /* NaN guards: milosUtils.cu:621-628 */
     if (isnan(model[1])) model[1] = 1;
     if (isnan(model[2])) model[2] = 1e-3;
@@ -953,7 +959,7 @@ __device__ void classical_estimates(const gm_dsetup *__restrict__ S, const float
 
 __global__ __launch_bounds__(64) void k_ce(const gm_dsetup *__restrict__ S,
                                            const float *__restrict__ obs, int64_t npix,
-                                           double *__restrict__ models)
+                                           int initial, double *__restrict__ models)
 {
     const int n = S->nlambda;
     for (int64_t p = (int64_t)blockIdx.x * 64 + threadIdx.x; p < npix;
@@ -961,7 +967,7 @@ __global__ __launch_bounds__(64) void k_ce(const gm_dsetup *__restrict__ S,
         double m[GM_NMODEL];
 #pragma unroll
         for (int f = 0; f < GM_NMODEL; ++f) m[f] = S->init_model[f];
-        classical_estimates(S, obs + p * 4 * n, m);
+        classical_estimates(S, obs + p * 4 * n, m, initial);
 #pragma unroll
         for (int f = 0; f < GM_NMODEL; ++f) models[p * GM_NMODEL + f] = m[f];
     }
@@ -984,7 +990,7 @@ __global__ __launch_bounds__(64) void k_init_state(const gm_dsetup *__restrict__
         } else {
 #pragma unroll
             for (int f = 0; f < GM_NMODEL; ++f) m[f] = S->init_model[f];
-            if (init_mode == GM_INIT_CE) classical_estimates(S, obs + p * 4 * n, m);
+            if (init_mode == GM_INIT_CE) classical_estimates(S, obs + p * 4 * n, m, 1);
         }
 #pragma unroll
         for (int f = 0; f < GM_NMODEL; ++f) {
@@ -1200,11 +1206,11 @@ void gm_launch_finalize(gm_state st, int64_t npix, const gm_out &out, hipStream_
                        out.iters, out.naccs);
 }
 
-void gm_launch_ce(const gm_dsetup *dS, const float *obs, int64_t npix, double *models,
-                  hipStream_t s)
+void gm_launch_ce(const gm_dsetup *dS, const float *obs, int64_t npix, int initial,
+                  double *models, hipStream_t s)
 {
     const int g = grid_for(npix, 64, kMaxBlocks);
-    hipLaunchKernelGGL(k_ce, dim3(g), dim3(64), 0, s, dS, obs, npix, models);
+    hipLaunchKernelGGL(k_ce, dim3(g), dim3(64), 0, s, dS, obs, npix, initial, models);
 }
 
 void gm_launch_set_count(int32_t *count, int32_t value, hipStream_t s)

@@ -165,9 +165,12 @@ gm_status gm_step_solve(gm_ctx *ctx, const float *alpha, const float *beta,
                         const double *flambda, int64_t npix, double *delta,
                         void *stream);
 /* Classical estimates (estimacionesClasicas, milosUtils.cu:386-563) applied
- * to the init model with the NaN guards of milosUtils.cu:621-628.          */
+ * to the init model.  for_initial_use = 1: the LM start (85/95 degree
+ * inclination guard, NaN guards of milosUtils.cu:621-628); 0: the estimates
+ * mode of gmilos.cu:309-431 (B_los stored in S0, no guards).               */
 gm_status gm_classical_estimates(gm_ctx *ctx, const float *obs, int64_t npix,
-                                 double *models, void *stream);
+                                 int32_t for_initial_use, double *models,
+                                 void *stream);
 
 /* ---- host-side set-up helpers (no device work) -------------------------- */
 /* Zeeman pattern from quantum numbers (create_cuantic.c:31-215). */

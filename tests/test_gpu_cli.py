@@ -1,0 +1,104 @@
+"""`gmilos <file>.mtrol` end to end on the reference's example (README.md:76-89):
+synthesis of run/init.mod reproduces data/synthesis_gpu.per, inverting it
+from run/guess.mod recovers run/init.mod, the estimates mode writes a model,
+and a FITS cube inverts to the same models as the library call."""
+import ctypes
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, model_row, read_mod
+
+pytestmark = pytest.mark.gpu
+CLI = os.path.join(ROOT, "g-milos_amd", "bin", "gmilos")
+
+
+@pytest.fixture()
+def refdir(tmp_path):
+    if not os.path.exists(CLI):
+        pytest.skip("gmilos CLI not built")
+    (tmp_path / "run").mkdir()
+    (tmp_path / "data").mkdir()
+    for f in ("LINES", "init.mod", "guess.mod", "malla.grid", "invert.mtrol",
+              "synthesis.mtrol", "CRISP_6173_1mA.psf"):
+        shutil.copy(os.path.join(GOLDEN, f), tmp_path / "run" / f)
+    shutil.copy(os.path.join(GOLDEN, "synthesis_gpu.per"), tmp_path / "data")
+    return tmp_path
+
+
+def run(cwd, *args):
+    print("gmilos", *args, flush=True)
+    p = subprocess.run([CLI, *args], cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                       text=True, timeout=120)
+    print(p.stdout[-2000:])
+    assert p.returncode == 0, p.stdout
+    return p.stdout
+
+
+def test_cli_synthesis_matches_reference_fixture(refdir):
+    run(refdir, "run/synthesis.mtrol")
+    got = np.loadtxt(refdir / "data" / "synthesis_gpu.per")      # overwritten, as the reference
+    ref = np.loadtxt(os.path.join(GOLDEN, "synthesis_gpu.per"))
+    np.testing.assert_allclose(got[:, :2], ref[:, :2], atol=1e-6)
+    assert np.abs(got[:, 2:] - ref[:, 2:]).max() < 2e-6
+
+
+def test_cli_inverts_reference_example(refdir):
+    run(refdir, "run/invert.mtrol")
+    m = read_mod(refdir / "data" / "synthesis_gpu_model.mod")
+    truth = read_mod(os.path.join(GOLDEN, "init.mod"))
+    np.testing.assert_allclose(model_row(m)[:9], model_row(truth)[:9], rtol=5e-3, atol=2e-3)
+    syn = np.loadtxt(refdir / "data" / "synthesis_gpu_stokes.per")
+    ref = np.loadtxt(os.path.join(GOLDEN, "synthesis_gpu.per"))
+    assert np.abs(syn[:, 2:] - ref[:, 2:]).max() < 1e-3
+    run(refdir, "run/invert.mtrol", "--mixed")
+    m2 = read_mod(refdir / "data" / "synthesis_gpu_model.mod")
+    np.testing.assert_allclose(model_row(m2)[:9], model_row(truth)[:9], rtol=5e-3, atol=2e-3)
+
+
+def test_cli_classical_estimates_mode(refdir):
+    text = open(refdir / "run" / "invert.mtrol").read().splitlines()
+    text[0] = "Number of cycles          (*):-1                         ! estimates"
+    open(refdir / "run" / "ce.mtrol", "w").write("\n".join(text) + "\n")
+    run(refdir, "run/ce.mtrol")
+    m = read_mod(refdir / "data" / "synthesis_gpu_model_ce.mod")
+    assert 500 < m["B"] < 3000 and 0 <= m["az"] <= 180
+    assert m["eta0"] == 0 and m["S1"] == 0
+
+
+def test_cli_fits_cube_matches_library(refdir, gm, cuda):
+    import torch
+    from test_io import _write_cube, IO_LIB
+    io = ctypes.CDLL(IO_LIB)
+    io.gmio_fits_write_image.argtypes = [ctypes.c_char_p, ctypes.c_int32] + [ctypes.c_void_p] * 3
+    io.gmio_fits_read_image.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32),
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
+    nr, nc = 6, 5
+    setup = gm.setup_single_line(psf_fwhm=60.0, precision=gm.GM_PREC_FP32,
+                                 init=read_mod(os.path.join(GOLDEN, "guess.mod")), miter=100)
+    ctx = gm.Context(setup)
+    truth = torch.from_numpy(gm.random_models(nr * nc, seed=5)).to(cuda)
+    obs = ctx.synth(truth)
+    cube = obs.cpu().numpy().reshape(nc, nr, 4, 30).transpose(1, 0, 2, 3)   # [row, col, s, l]
+    path = str(refdir / "data" / "cube.fits").encode()
+    _write_cube(io, path, ("lambda", "stokes", "row", "col"), cube)
+    text = open(refdir / "run" / "invert.mtrol").read().splitlines()
+    text[1] = "Observed profiles          (*):data/cube.fits    !"
+    text[3] = "PSF file                     :60                 ! FWHM"
+    open(refdir / "run" / "cube.mtrol", "w").write("\n".join(text) + "\n")
+    run(refdir, "run/cube.mtrol")
+    naxis = ctypes.c_int32()
+    naxes = np.zeros(4, dtype=np.int64)
+    buf = np.zeros(nr * nc * 13, dtype=np.float32)
+    assert io.gmio_fits_read_image(str(refdir / "data" / "cube_mod.fits").encode(),
+                                   ctypes.byref(naxis), naxes.ctypes.data, buf.ctypes.data,
+                                   buf.size) == 0
+    planes = buf.reshape(13, nr * nc)
+    lib = ctx.invert(obs.contiguous())
+    np.testing.assert_allclose(planes[:11].T, lib["model"].cpu().numpy().astype(np.float32),
+                               rtol=1e-6, atol=1e-5)
+    np.testing.assert_array_equal(planes[11], lib["iters"].cpu().numpy().astype(np.float32))
+    assert os.path.exists(refdir / "data" / "cube_stokes.fits")
