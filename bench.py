@@ -82,7 +82,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--npix", type=int, default=1024 * 1024)
     ap.add_argument("--precision", choices=["mixed", "fp32"], default="mixed")
-    ap.add_argument("--solver", choices=["svd", "ldl"], default="svd")
+    ap.add_argument("--solver", choices=["svd", "ldl", "jacobi"], default="svd")
     ap.add_argument("--miter", type=int, default=50)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -103,7 +103,8 @@ def main():
     torch.cuda.set_device(dev)
 
     prec = gm.GM_PREC_MIXED if args.precision == "mixed" else gm.GM_PREC_FP32
-    solver = gm.GM_SOLVE_SVD if args.solver == "svd" else gm.GM_SOLVE_LDL
+    solver = {"svd": gm.GM_SOLVE_SVD, "ldl": gm.GM_SOLVE_LDL,
+              "jacobi": gm.GM_SOLVE_JACOBI}[args.solver]
     setup = gm.setup_single_line(psf_fwhm=60.0, precision=prec, solver=solver,
                                  miter=args.miter)
     ctx = gm.Context(setup)

@@ -29,7 +29,7 @@ GM_MAX_COMP = 10
 GM_NMODEL = 11
 
 GM_PREC_FP32, GM_PREC_MIXED = 0, 1
-GM_SOLVE_SVD, GM_SOLVE_LDL = 0, 1
+GM_SOLVE_SVD, GM_SOLVE_LDL, GM_SOLVE_JACOBI = 0, 1, 2
 GM_INIT_CE, GM_INIT_MODEL, GM_INIT_PER_PIXEL = 0, 1, 2
 
 # Init_Model order (defines.h:94-106)
@@ -89,7 +89,9 @@ class gm_stats(ctypes.Structure):
 EXPORTED = ("gm_ctx_create", "gm_ctx_destroy", "gm_status_string", "gm_ctx_set_timing",
             "gm_last_stats", "gm_invert", "gm_synth", "gm_synth_rf", "gm_normal_equations",
             "gm_step_solve", "gm_classical_estimates", "gm_zeeman_from_quantum",
-            "gm_parse_lines_file", "gm_psf_gaussian", "gm_psf_from_table", "gm_setup_defaults")
+            "gm_parse_lines_file", "gm_psf_gaussian", "gm_psf_from_table", "gm_setup_defaults",
+            "gm_mlp_create", "gm_mlp_destroy", "gm_mlp_num_params", "gm_mlp_get_params",
+            "gm_mlp_set_params", "gm_mlp_train_step", "gm_mlp_gradients", "gm_mlp_predict")
 
 _lib = None
 
@@ -123,6 +125,15 @@ def load_library(path=LIB_PATH):
         "gm_psf_from_table": ([P(ctypes.c_double), P(ctypes.c_double), i32, P(ctypes.c_double),
                                i32, ctypes.c_double, P(ctypes.c_double)], i32),
         "gm_setup_defaults": ([P(gm_setup)], None),
+        "gm_mlp_create": ([i32, ctypes.c_uint64, P(vp)], i32),
+        "gm_mlp_destroy": ([vp], i32),
+        "gm_mlp_num_params": ([vp], i64),
+        "gm_mlp_get_params": ([vp, vp, i64], i32),
+        "gm_mlp_set_params": ([vp, vp, i64], i32),
+        "gm_mlp_train_step": ([vp, vp, vp, i64, ctypes.c_float, ctypes.c_float,
+                               P(ctypes.c_float), vp], i32),
+        "gm_mlp_gradients": ([vp, vp, vp, i64, vp, P(ctypes.c_float), vp], i32),
+        "gm_mlp_predict": ([vp, vp, i64, vp, vp, vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -433,6 +444,81 @@ class Context:
                                                ctypes.c_void_p(m.data_ptr()), self._stream()),
                "gm_classical_estimates")
         return m
+
+
+class MLP:
+    """Learned initial guess (BF16 MFMA network); device buffers are torch
+    tensors on the current device."""
+
+    def __init__(self, nlambda, seed=0):
+        self.lib = load_library()
+        h = ctypes.c_void_p()
+        _check(self.lib.gm_mlp_create(int(nlambda), int(seed), ctypes.byref(h)), "gm_mlp_create")
+        self.h = h
+        self.nlambda = nlambda
+        self.nparam = self.lib.gm_mlp_num_params(h)
+
+    def close(self):
+        if self.h:
+            self.lib.gm_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        out = np.zeros(self.nparam, dtype=np.float32)
+        _check(self.lib.gm_mlp_get_params(self.h, out.ctypes.data_as(ctypes.c_void_p),
+                                          self.nparam), "gm_mlp_get_params")
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(self.lib.gm_mlp_set_params(self.h, p.ctypes.data_as(ctypes.c_void_p), p.size),
+               "gm_mlp_set_params")
+
+    def train_step(self, obs, models, lr, weight_decay=1e-4):
+        loss = ctypes.c_float()
+        _check(self.lib.gm_mlp_train_step(self.h, ctypes.c_void_p(obs.data_ptr()),
+                                          ctypes.c_void_p(models.data_ptr()), obs.shape[0],
+                                          float(lr), float(weight_decay), ctypes.byref(loss),
+                                          Context._stream()), "gm_mlp_train_step")
+        return loss.value
+
+    def gradients(self, obs, models):
+        g = np.zeros(self.nparam, dtype=np.float32)
+        loss = ctypes.c_float()
+        _check(self.lib.gm_mlp_gradients(self.h, ctypes.c_void_p(obs.data_ptr()),
+                                         ctypes.c_void_p(models.data_ptr()), obs.shape[0],
+                                         g.ctypes.data_as(ctypes.c_void_p), ctypes.byref(loss),
+                                         Context._stream()), "gm_mlp_gradients")
+        return loss.value, g
+
+    def predict(self, obs, base):
+        import torch
+        npix = obs.shape[0]
+        m = torch.empty((npix, GM_NMODEL), dtype=torch.float64, device=obs.device)
+        b = np.ascontiguousarray(base, dtype=np.float64)
+        _check(self.lib.gm_mlp_predict(self.h, ctypes.c_void_p(obs.data_ptr()), npix,
+                                       b.ctypes.data_as(ctypes.c_void_p),
+                                       ctypes.c_void_p(m.data_ptr()), Context._stream()),
+               "gm_mlp_predict")
+        return m
+
+    def layout(self):
+        """[(in, out, w_offset, b_offset)] of the flat parameter vector."""
+        kin = ((4 * self.nlambda + 63) // 64) * 64
+        dims = [kin, 256, 256, 256, 64]
+        out, off = [], 0
+        for l in range(4):
+            wo = off
+            off += dims[l] * dims[l + 1]
+            out.append((dims[l], dims[l + 1], wo, off))
+            off += dims[l + 1]
+        return out
 
 
 def packed_index(p, q):

@@ -45,8 +45,6 @@ struct gm_ctx {
     int64_t scr_cap = 0;
     /* round bookkeeping */
     int32_t *h_counts = nullptr;     /* pinned [kMaxRounds][3]                */
-    hipEvent_t ev_round[kMaxRounds];
-    int ev_round_n = 0;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     int timing = 0;
     std::vector<Events> kev;         /* per-kernel events when timing is on   */
@@ -112,7 +110,8 @@ static gm_status build_dsetup(const gm_setup &s, gm_dsetup &d)
     if (s.init_model[10] < 1.0) return GM_ERR_UNSUPPORTED;   /* filling factor  */
     if (s.ce_line < 0 || s.ce_line >= s.nlines) return GM_ERR_INVALID;
     if (s.precision != GM_PREC_FP32 && s.precision != GM_PREC_MIXED) return GM_ERR_INVALID;
-    if (s.solver != GM_SOLVE_SVD && s.solver != GM_SOLVE_LDL) return GM_ERR_INVALID;
+    if (s.solver != GM_SOLVE_SVD && s.solver != GM_SOLVE_LDL && s.solver != GM_SOLVE_JACOBI)
+        return GM_ERR_INVALID;
     d.nlambda = n;
     d.nlines = s.nlines;
     for (int L = 0; L < s.nlines; ++L) {
@@ -231,7 +230,6 @@ extern "C" gm_status gm_ctx_create(const gm_setup *setup, gm_ctx **out)
         delete c;
         return GM_ERR_NOMEM;
     }
-    for (int i = 0; i < kMaxRounds; ++i) c->ev_round[i] = nullptr;
     hipEventCreate(&c->ev_start);
     hipEventCreate(&c->ev_stop);
     std::memset(&c->st, 0, sizeof c->st);
@@ -248,7 +246,6 @@ extern "C" gm_status gm_ctx_destroy(gm_ctx *c)
     if (c->scr_beta) hipFree(c->scr_beta);
     if (c->dS) hipFree(c->dS);
     if (c->h_counts) hipHostFree(c->h_counts);
-    for (int i = 0; i < c->ev_round_n; ++i) hipEventDestroy(c->ev_round[i]);
     for (auto &e : c->kev) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
     if (c->ev_start) hipEventDestroy(c->ev_start);
     if (c->ev_stop) hipEventDestroy(c->ev_stop);
@@ -276,8 +273,8 @@ static gm_status ensure_state(gm_ctx *c, int64_t npix)
     if (npix <= c->st.cap) return GM_OK;
     int64_t cap = ((npix + 255) / 256) * 256;
     const size_t per = sizeof(double) * (2 * GM_NMODEL + 5) + sizeof(float) * (GM_ALPHA_N + 9) +
-                       sizeof(int32_t) * (2 + 3);
-    const size_t bytes = per * (size_t)cap + 256 * 16 + 64;
+                       sizeof(int32_t) * (2 + 4);
+    const size_t bytes = per * (size_t)cap + 256 * 16 + 64 + sizeof(int32_t) * 3 * (kMaxRounds + 2) + 256;
     if (c->ws) {
         hipDeviceSynchronize();
         hipFree(c->ws);
@@ -305,18 +302,10 @@ static gm_status ensure_state(gm_ctx *c, int64_t npix)
     st.factor = (double *)take(sizeof(double) * cap);
     st.iter = (int32_t *)take(sizeof(int32_t) * cap);
     st.nacc = (int32_t *)take(sizeof(int32_t) * cap);
-    for (int i = 0; i < 3; ++i) st.list[i] = (int32_t *)take(sizeof(int32_t) * cap);
+    for (int i = 0; i < 4; ++i) st.list[i] = (int32_t *)take(sizeof(int32_t) * cap);
     st.counts = (int32_t *)take(sizeof(int32_t) * 4);
+    st.hist = (int32_t *)take(sizeof(int32_t) * 3 * (kMaxRounds + 2));
     return GM_OK;
-}
-
-static hipEvent_t round_event(gm_ctx *c, int r)
-{
-    while (c->ev_round_n <= r) {
-        hipEventCreateWithFlags(&c->ev_round[c->ev_round_n], hipEventDisableTiming);
-        c->ev_round_n++;
-    }
-    return c->ev_round[r];
 }
 
 static Events &kernel_events(gm_ctx *c, size_t i)
@@ -333,8 +322,12 @@ static Events &kernel_events(gm_ctx *c, size_t i)
 extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
                                const double *init, const gm_out *out, void *stream)
 {
-    if (!c || !obs || !out || npix < 0) return GM_ERR_INVALID;
-    if (npix > (int64_t)0x7fffffff) return GM_ERR_INVALID;
+    if (!c || !out || npix < 0) return GM_ERR_INVALID;
+    if (npix == 0) {
+        std::memset(&c->stats, 0, sizeof c->stats);
+        return GM_OK;
+    }
+    if (!obs || npix > (int64_t)0x7fffffff) return GM_ERR_INVALID;
     if (c->setup.init_mode == GM_INIT_PER_PIXEL && !init) return GM_ERR_INVALID;
     std::memset(&c->stats, 0, sizeof c->stats);
     c->stats.npix = npix;
@@ -364,6 +357,7 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
     kbegin(3);
     gm_launch_init_state(c->dS, st, obs, init, c->setup.init_mode, npix, st.list[0], mixed, s);
     gm_launch_set_count(&cnt[0], (int32_t)npix, s);
+    gm_launch_set_count(&cnt[3], 0, s);
     kend();
     kbegin(1);
     gm_launch_rf(lc, c->dS, st.best, st.cap, obs, st.list[0], &cnt[0], 0, npix, st.alpha,
@@ -372,13 +366,17 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
     kbegin(3);
     gm_launch_init_chi(st, npix, s);
     kend();
-    int cur = 0, rounds = 0;
+    int cur = 0;
     const int miter = c->setup.miter > 0 ? c->setup.miter : 1;  /* do-while */
+    /* hist row r: [active in round r, unused, accepted in round r]; the solve
+     * kernel of round r writes hist[3r] and the accepted count of round r-1
+     * into hist[3r - 1]; empty trailing rounds cost four short launches */
+    int32_t *hist = st.hist + 3;
     for (int r = 0; r < miter; ++r) {
         const int nxt = 1 - cur;
         kbegin(2);
         gm_launch_solve(c->dS, st, st.list[cur], &cnt[cur], npix, mixed, solver, &cnt[nxt],
-                        &cnt[2], s);
+                        &cnt[2], hist + 3 * r, st.list[3], &cnt[3], s);
         kend();
         kbegin(0);
         gm_launch_synth(lc, c->dS, st.trial, st.cap, obs, st.list[cur], &cnt[cur], 0, npix,
@@ -386,26 +384,15 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
         kend();
         kbegin(3);
         gm_launch_decide(c->dS, st, st.list[cur], &cnt[cur], npix, st.list[nxt], &cnt[nxt],
-                         st.list[2], &cnt[2], mixed, s);
+                         st.list[2], &cnt[2], &cnt[3], mixed, s);
         kend();
         kbegin(1);
         gm_launch_rf(lc, c->dS, st.best, st.cap, obs, st.list[2], &cnt[2], 0, npix, st.alpha,
                      st.beta, st.cap, nullptr, nullptr, nullptr, mixed, s);
         kend();
-        /* per-round counts: [active this round, active next round, accepted] */
-        int32_t *hc = c->h_counts + 3 * r;
-        GM_HIP(hipMemcpyAsync(hc, &cnt[cur], sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        GM_HIP(hipMemcpyAsync(hc + 1, &cnt[nxt], sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        GM_HIP(hipMemcpyAsync(hc + 2, &cnt[2], sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        GM_HIP(hipEventRecord(round_event(c, r), s));
         cur = nxt;
-        rounds = r + 1;
-        /* stop issuing rounds once a completed round reports no survivors */
-        if (r >= 2) {
-            hipEvent_t e = c->ev_round[r - 2];
-            if (hipEventQuery(e) == hipSuccess && c->h_counts[3 * (r - 2) + 1] == 0) break;
-        }
     }
+    gm_launch_copy_count(&cnt[2], hist + 3 * miter - 1, s);
     kbegin(3);
     gm_launch_finalize(st, npix, *out, s);
     kend();
@@ -415,15 +402,18 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
     float ms = 0;
     hipEventElapsedTime(&ms, c->ev_start, c->ev_stop);
     c->stats.ms_total = ms;
-    c->stats.lm_rounds = rounds;
+    GM_HIP(hipMemcpy(c->h_counts, hist, sizeof(int32_t) * 3 * miter, hipMemcpyDeviceToHost));
     c->stats.synth_evals = npix;
     c->stats.rf_evals = npix;
-    for (int r = 0; r < rounds; ++r) {
+    int rounds = 0;
+    for (int r = 0; r < miter; ++r) {
         const int32_t *hc = c->h_counts + 3 * r;
+        if (hc[0] > 0) rounds = r + 1;
         c->stats.solves += hc[0];
         c->stats.synth_evals += hc[0];
         c->stats.rf_evals += hc[2];
     }
+    c->stats.lm_rounds = rounds;
     if (c->timing) {
         for (size_t i = 0; i < kidx; ++i) {
             float t = 0;
@@ -443,8 +433,9 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
 extern "C" gm_status gm_synth(gm_ctx *c, const double *models, int64_t npix, float *spectra,
                               void *stream)
 {
-    if (!c || !models || !spectra || npix < 0) return GM_ERR_INVALID;
+    if (!c || npix < 0) return GM_ERR_INVALID;
     if (npix == 0) return GM_OK;
+    if (!models || !spectra) return GM_ERR_INVALID;
     gm_launch_synth(c->lc, c->dS, models, 0, nullptr, nullptr, nullptr, npix, npix, nullptr,
                     spectra, 0, (hipStream_t)stream);
     return hip_ok(hipGetLastError());
@@ -453,8 +444,9 @@ extern "C" gm_status gm_synth(gm_ctx *c, const double *models, int64_t npix, flo
 extern "C" gm_status gm_synth_rf(gm_ctx *c, const double *models, int64_t npix, float *spectra,
                                  float *rf, void *stream)
 {
-    if (!c || !models || npix < 0) return GM_ERR_INVALID;
+    if (!c || npix < 0) return GM_ERR_INVALID;
     if (npix == 0) return GM_OK;
+    if (!models) return GM_ERR_INVALID;
     gm_launch_rf(c->lc, c->dS, models, 0, nullptr, nullptr, nullptr, npix, npix, nullptr, nullptr,
                  0, nullptr, spectra, rf, 0, (hipStream_t)stream);
     return hip_ok(hipGetLastError());
@@ -464,8 +456,9 @@ extern "C" gm_status gm_normal_equations(gm_ctx *c, const double *models, const 
                                          int64_t npix, float *alpha, float *beta, double *chisqr,
                                          void *stream)
 {
-    if (!c || !models || !obs || npix < 0) return GM_ERR_INVALID;
+    if (!c || npix < 0) return GM_ERR_INVALID;
     if (npix == 0) return GM_OK;
+    if (!models || !obs) return GM_ERR_INVALID;
     const int mixed = c->setup.precision == GM_PREC_MIXED;
     gm_launch_rf(c->lc, c->dS, models, 0, obs, nullptr, nullptr, npix, npix, alpha, beta, 0,
                  chisqr, nullptr, nullptr, mixed, (hipStream_t)stream);
@@ -476,8 +469,9 @@ extern "C" gm_status gm_step_solve(gm_ctx *c, const float *alpha, const float *b
                                    const double *flambda, int64_t npix, double *delta,
                                    void *stream)
 {
-    if (!c || !alpha || !beta || !flambda || !delta || npix < 0) return GM_ERR_INVALID;
+    if (!c || npix < 0) return GM_ERR_INVALID;
     if (npix == 0) return GM_OK;
+    if (!alpha || !beta || !flambda || !delta) return GM_ERR_INVALID;
     gm_launch_step_solve(alpha, beta, flambda, npix, delta, c->setup.precision == GM_PREC_MIXED,
                          c->setup.solver, (hipStream_t)stream);
     return hip_ok(hipGetLastError());
@@ -487,8 +481,9 @@ extern "C" gm_status gm_classical_estimates(gm_ctx *c, const float *obs, int64_t
                                             int32_t for_initial_use, double *models,
                                             void *stream)
 {
-    if (!c || !obs || !models || npix < 0) return GM_ERR_INVALID;
+    if (!c || npix < 0) return GM_ERR_INVALID;
     if (npix == 0) return GM_OK;
+    if (!obs || !models) return GM_ERR_INVALID;
     gm_launch_ce(c->dS, obs, npix, for_initial_use ? 1 : 0, models, (hipStream_t)stream);
     return hip_ok(hipGetLastError());
 }

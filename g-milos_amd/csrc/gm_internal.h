@@ -61,8 +61,10 @@ struct gm_state {
     double  *factor;      /* [cap] PARBETA_FACTOR (logclambda)              */
     int32_t *iter;        /* [cap]                                           */
     int32_t *nacc;        /* [cap]                                           */
-    int32_t *list[3];     /* two active lists (ping-pong) + accepted list    */
-    int32_t *counts;      /* [4]: list counts                                */
+    int32_t *list[4];     /* two active lists (ping-pong), accepted list and
+                             SVD fallback list                               */
+    int32_t *counts;      /* [4]: counts of the four lists                   */
+    int32_t *hist;        /* [3 * (max rounds + 1)] per-round counts          */
 };
 
 /* Lane mapping of the wavelength-parallel kernels. */
@@ -89,14 +91,15 @@ void gm_launch_synth(const gm_lanecfg &lc, const gm_dsetup *dS, const double *mo
                      double *chi_out, float *synth_out, int mixed, hipStream_t st);
 void gm_launch_solve(const gm_dsetup *dS, gm_state st, const int32_t *list,
                      const int32_t *count, int64_t grid_items, int mixed, int solver,
-                     int32_t *reset0, int32_t *reset1, hipStream_t stream);
+                     int32_t *reset0, int32_t *reset1, int32_t *hist,
+                     int32_t *fb_list, int32_t *fb_count, hipStream_t stream);
 void gm_launch_step_solve(const float *alpha, const float *beta, const double *flambda,
                           int64_t n, double *delta, int mixed, int solver,
                           hipStream_t stream);
 void gm_launch_decide(const gm_dsetup *dS, gm_state st, const int32_t *list,
                       const int32_t *count, int64_t grid_items, int32_t *next_list,
                       int32_t *next_count, int32_t *acc_list, int32_t *acc_count,
-                      int mixed, hipStream_t stream);
+                      int32_t *reset_fb, int mixed, hipStream_t stream);
 void gm_launch_init_state(const gm_dsetup *dS, gm_state st, const float *obs,
                           const double *init, int init_mode, int64_t npix,
                           int32_t *list0, int mixed, hipStream_t stream);
@@ -106,5 +109,6 @@ void gm_launch_finalize(gm_state st, int64_t npix, const gm_out &out,
 void gm_launch_ce(const gm_dsetup *dS, const float *obs, int64_t npix, int initial,
                   double *models, hipStream_t stream);
 void gm_launch_set_count(int32_t *count, int32_t value, hipStream_t stream);
+void gm_launch_copy_count(const int32_t *src, int32_t *dst, hipStream_t stream);
 
 #endif

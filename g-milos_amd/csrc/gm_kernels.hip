@@ -646,20 +646,30 @@ __device__ __forceinline__ void solve_svd(T (&a)[45], const T (&b)[9], T (&x)[9]
     }
 }
 
-/* LDL^T of the damped (symmetric positive semi-definite) matrix; pivots
- * <= 1e-12 drop their component, as the pseudo-inverse drops |w| <= 1e-12. */
+/* LDL^T solve of the damped matrix (symmetric positive semi-definite by
+ * construction).  A pivot of an exactly zero row (a fixed parameter) drops
+ * that component, as the pseudo-inverse does.  Returns whether the result
+ * provably equals the reference's pseudo-inverse solution: every other pivot
+ * is positive and trace(A^-1) < 1e12, so every eigenvalue exceeds 1e-12 and
+ * V diag(1/w) V^T = A^-1 on the remaining block.  The trace comes from the
+ * factors: trace(A^-1) = sum_j (1/d_j) sum_k (L^-1)_jk^2.                   */
 template <typename T>
-__device__ __forceinline__ void solve_ldl(const T (&a)[45], const T (&b)[9], T (&x)[9])
+__device__ __forceinline__ bool solve_ldl(const T (&a)[45], const T (&b)[9], T (&x)[9])
 {
     T L[45], d[9], id[9];
+    bool ok = true;
 #pragma unroll
     for (int j = 0; j < 9; ++j) {
-        T dj = a[ut(j, j)];
+        const T ajj = a[ut(j, j)];
+        T dj = ajj;
 #pragma unroll
         for (int k = 0; k < j; ++k) dj -= L[ut(j, k)] * L[ut(j, k)] * d[k];
-        const bool ok = dj > T(1e-12);
-        d[j] = ok ? dj : T(0);
-        id[j] = ok ? T(1) / dj : T(0);
+        const bool null_row = (ajj == T(0));
+        const bool pos = dj > T(0);
+        ok = ok && (null_row || pos);
+        const bool use = pos && !null_row;
+        d[j] = use ? dj : T(0);
+        id[j] = use ? T(1) / dj : T(0);
 #pragma unroll
         for (int i = j + 1; i < 9; ++i) {
             T lij = a[ut(i, j)];
@@ -685,25 +695,40 @@ __device__ __forceinline__ void solve_ldl(const T (&a)[45], const T (&b)[9], T (
         for (int k = i + 1; k < 9; ++k) xi -= L[ut(k, i)] * x[k];
         x[i] = xi;
     }
+    /* L^-1 in place (unit lower triangular), column by column */
+#pragma unroll
+    for (int j = 0; j < 9; ++j)
+#pragma unroll
+        for (int i = j + 1; i < 9; ++i) {
+            T mij = L[ut(i, j)];
+#pragma unroll
+            for (int k = j + 1; k < i; ++k) mij += L[ut(i, k)] * L[ut(k, j)];
+            L[ut(i, j)] = -mij;
+        }
+    T tr = T(0);
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        T row = T(1);
+#pragma unroll
+        for (int k = 0; k < j; ++k) row += L[ut(j, k)] * L[ut(j, k)];
+        tr += id[j] * row;
+    }
+    return ok && (tr < T(1e12));
 }
 
-template <typename T, int SOLVER>
-__device__ __forceinline__ void damped_solve(const float *__restrict__ alpha,
-                                             const float *__restrict__ beta,
-                                             int64_t es, T flambda, T (&x)[9])
+/* Damped normal matrix alpha_ii (1 + lambda) (milosUtils.cu:648-652). */
+template <typename T>
+__device__ __forceinline__ void load_damped(const float *__restrict__ alpha,
+                                            const float *__restrict__ beta, int64_t es,
+                                            T flambda, T (&a)[45], T (&b)[9])
 {
-    T a[45], b[9];
 #pragma unroll
     for (int e = 0; e < 45; ++e) a[e] = (T)alpha[e * es];
 #pragma unroll
     for (int p = 0; p < 9; ++p) {
         b[p] = (T)beta[p * es];
-        a[ut(p, p)] = a[ut(p, p)] * (T(1) + flambda);   /* milosUtils.cu:648-652 */
+        a[ut(p, p)] = a[ut(p, p)] * (T(1) + flambda);
     }
-    if (SOLVER == GM_SOLVE_SVD)
-        solve_svd<T, (sizeof(T) == 8 ? 12 : 10)>(a, b, x);
-    else
-        solve_ldl<T>(a, b, x);
 }
 
 /* AplicaDelta + check (milosUtils.cu:30-238), double model. */
@@ -752,14 +777,43 @@ __device__ __forceinline__ void apply_delta_check(const gm_dsetup *__restrict__ 
     m[10] = m[10] < 0 ? 0.0 : (m[10] > 1.0 ? 1.0 : m[10]);
 }
 
+template <typename T>
+__device__ __forceinline__ void apply_to_trial(const gm_dsetup *__restrict__ S, const gm_state &st,
+                                               int64_t pix, const T (&x)[9])
+{
+    const int64_t cap = st.cap;
+    double best[GM_NMODEL], m[GM_NMODEL];
+#pragma unroll
+    for (int f = 0; f < GM_NMODEL; ++f) {
+        best[f] = st.best[f * cap + pix];
+        m[f] = st.trial[f * cap + pix];
+    }
+    apply_delta_check<T>(S, best, x, m);
+#pragma unroll
+    for (int f = 0; f < GM_NMODEL; ++f) st.trial[f * cap + pix] = m[f];
+}
+
+/* Step solve for the active list.  SOLVER = GM_SOLVE_SVD: LDL^T where it
+ * provably equals the SVD pseudo-inverse, otherwise the pixel goes to the
+ * fallback list solved by k_solve_svd; GM_SOLVE_LDL: LDL^T only;
+ * GM_SOLVE_JACOBI: always the Jacobi SVD. */
 template <bool MIXED, int SOLVER>
 __global__ __launch_bounds__(64) void k_solve(const gm_dsetup *__restrict__ S,
                                               gm_state st, const int32_t *__restrict__ list,
                                               const int32_t *__restrict__ count,
                                               int32_t *__restrict__ reset0,
-                                              int32_t *__restrict__ reset1)
+                                              int32_t *__restrict__ reset1,
+                                              int32_t *__restrict__ hist,
+                                              int32_t *__restrict__ fb_list,
+                                              int32_t *__restrict__ fb_count)
 {
+    /* hist[0] = active pixels this round, hist[-1] = accepted pixels of the
+     * previous round (read before that list's counter is reset) */
     if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (hist) {
+            hist[0] = *count;
+            if (reset1) hist[-1] = *reset1;
+        }
         if (reset0) *reset0 = 0;
         if (reset1) *reset1 = 0;
     }
@@ -769,17 +823,37 @@ __global__ __launch_bounds__(64) void k_solve(const gm_dsetup *__restrict__ S,
     for (int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x; i < cnt;
          i += (int64_t)gridDim.x * 64) {
         const int64_t pix = list[i];
-        T x[9];
-        damped_solve<T, SOLVER>(st.alpha + pix, st.beta + pix, cap, (T)st.flambda[pix], x);
-        double best[GM_NMODEL], m[GM_NMODEL];
-#pragma unroll
-        for (int f = 0; f < GM_NMODEL; ++f) {
-            best[f] = st.best[f * cap + pix];
-            m[f] = st.trial[f * cap + pix];
+        T a[45], b[9], x[9];
+        load_damped<T>(st.alpha + pix, st.beta + pix, cap, (T)st.flambda[pix], a, b);
+        if (SOLVER == GM_SOLVE_JACOBI) {
+            solve_svd<T, (sizeof(T) == 8 ? 12 : 10)>(a, b, x);
+        } else {
+            const bool proven = solve_ldl<T>(a, b, x);
+            if (SOLVER == GM_SOLVE_SVD && !proven) {
+                fb_list[atomicAdd(fb_count, 1)] = (int32_t)pix;
+                continue;
+            }
         }
-        apply_delta_check<T>(S, best, x, m);
-#pragma unroll
-        for (int f = 0; f < GM_NMODEL; ++f) st.trial[f * cap + pix] = m[f];
+        apply_to_trial<T>(S, st, pix, x);
+    }
+}
+
+/* Jacobi SVD for the pixels the LDL^T test could not certify. */
+template <bool MIXED>
+__global__ __launch_bounds__(64) void k_solve_svd(const gm_dsetup *__restrict__ S, gm_state st,
+                                                  const int32_t *__restrict__ fb_list,
+                                                  const int32_t *__restrict__ fb_count)
+{
+    using T = typename std::conditional<MIXED, double, float>::type;
+    const int64_t cnt = *fb_count;
+    const int64_t cap = st.cap;
+    for (int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x; i < cnt;
+         i += (int64_t)gridDim.x * 64) {
+        const int64_t pix = fb_list[i];
+        T a[45], b[9], x[9];
+        load_damped<T>(st.alpha + pix, st.beta + pix, cap, (T)st.flambda[pix], a, b);
+        solve_svd<T, (sizeof(T) == 8 ? 12 : 10)>(a, b, x);
+        apply_to_trial<T>(S, st, pix, x);
     }
 }
 
@@ -793,8 +867,17 @@ __global__ __launch_bounds__(64) void k_step_solve(const float *__restrict__ alp
     for (int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * 64) {
         /* caller layout: alpha [n][45], beta [n][9] */
-        T x[9];
-        damped_solve<T, SOLVER>(alpha + i * 45, beta + i * 9, 1, (T)flambda[i], x);
+        T a[45], b[9], x[9];
+        load_damped<T>(alpha + i * 45, beta + i * 9, 1, (T)flambda[i], a, b);
+        if (SOLVER == GM_SOLVE_JACOBI) {
+            solve_svd<T, (sizeof(T) == 8 ? 12 : 10)>(a, b, x);
+        } else {
+            T a2[45];
+#pragma unroll
+            for (int e = 0; e < 45; ++e) a2[e] = a[e];
+            const bool proven = solve_ldl<T>(a, b, x);
+            if (SOLVER == GM_SOLVE_SVD && !proven) solve_svd<T, (sizeof(T) == 8 ? 12 : 10)>(a2, b, x);
+        }
 #pragma unroll
         for (int p = 0; p < 9; ++p) delta[i * 9 + p] = (double)x[p];
     }
@@ -802,22 +885,26 @@ __global__ __launch_bounds__(64) void k_step_solve(const float *__restrict__ alp
 
 /* Accept / reject and LM bookkeeping (milosUtils.cu:660-698); builds the
  * next active list and the list of accepted pixels that need new normal
- * equations.  One atomic per wave per list.                                */
+ * equations, with one atomic per 1024-pixel workgroup and list.            */
 template <bool MIXED>
-__global__ __launch_bounds__(64) void k_decide(const gm_dsetup *__restrict__ S, gm_state st,
-                                               const int32_t *__restrict__ list,
-                                               const int32_t *__restrict__ count,
-                                               int32_t *__restrict__ next_list,
-                                               int32_t *__restrict__ next_count,
-                                               int32_t *__restrict__ acc_list,
-                                               int32_t *__restrict__ acc_count)
+__global__ __launch_bounds__(1024) void k_decide(const gm_dsetup *__restrict__ S, gm_state st,
+                                                 const int32_t *__restrict__ list,
+                                                 const int32_t *__restrict__ count,
+                                                 int32_t *__restrict__ next_list,
+                                                 int32_t *__restrict__ next_count,
+                                                 int32_t *__restrict__ acc_list,
+                                                 int32_t *__restrict__ acc_count,
+                                                 int32_t *__restrict__ reset_fb)
 {
     using T = typename std::conditional<MIXED, double, float>::type;
+    __shared__ int wc[16], wa[16];
+    __shared__ int base_c, base_a;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && reset_fb) *reset_fb = 0;
     const int64_t cnt = *count;
     const int64_t cap = st.cap;
-    const int lane = threadIdx.x;
-    for (int64_t base = (int64_t)blockIdx.x * 64; base < cnt; base += (int64_t)gridDim.x * 64) {
-        const int64_t i = base + lane;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int64_t base = (int64_t)blockIdx.x * 1024; base < cnt; base += (int64_t)gridDim.x * 1024) {
+        const int64_t i = base + tid;
         const bool valid = i < cnt;
         int32_t pix = valid ? list[i] : 0;
         bool cont = false, accn = false;
@@ -853,16 +940,28 @@ __global__ __launch_bounds__(64) void k_decide(const gm_dsetup *__restrict__ S, 
         }
         const uint64_t mc = __ballot(cont);
         const uint64_t ma = __ballot(accn);
-        const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-        int bc = 0, ba = 0;
         if (lane == 0) {
-            if (mc) bc = atomicAdd(next_count, (int)__popcll(mc));
-            if (ma) ba = atomicAdd(acc_count, (int)__popcll(ma));
+            wc[wave] = (int)__popcll(mc);
+            wa[wave] = (int)__popcll(ma);
         }
-        bc = __shfl(bc, 0, 64);
-        ba = __shfl(ba, 0, 64);
-        if (cont) next_list[bc + __popcll(mc & below)] = pix;
-        if (accn) acc_list[ba + __popcll(ma & below)] = pix;
+        __syncthreads();
+        if (tid == 0) {
+            int sc = 0, sa = 0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+                const int c = wc[w], a = wa[w];
+                wc[w] = sc;
+                wa[w] = sa;
+                sc += c;
+                sa += a;
+            }
+            base_c = sc ? atomicAdd(next_count, sc) : 0;
+            base_a = sa ? atomicAdd(acc_count, sa) : 0;
+        }
+        __syncthreads();
+        const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        if (cont) next_list[base_c + wc[wave] + __popcll(mc & below)] = pix;
+        if (accn) acc_list[base_a + wa[wave] + __popcll(ma & below)] = pix;
+        __syncthreads();
     }
 }
 
@@ -1028,6 +1127,7 @@ __global__ void k_finalize(gm_state st, int64_t npix, double *__restrict__ model
 }
 
 __global__ void k_set_count(int32_t *c, int32_t v) { *c = v; }
+__global__ void k_copy_count(const int32_t *src, int32_t *dst) { *dst = *src; }
 
 inline int grid_for(int64_t items, int per_block, int cap_blocks)
 {
@@ -1136,19 +1236,29 @@ void gm_launch_synth(const gm_lanecfg &lc, const gm_dsetup *dS, const double *mo
 
 void gm_launch_solve(const gm_dsetup *dS, gm_state st, const int32_t *list,
                      const int32_t *count, int64_t grid_items, int mixed, int solver,
-                     int32_t *reset0, int32_t *reset1, hipStream_t s)
+                     int32_t *reset0, int32_t *reset1, int32_t *hist, int32_t *fb_list,
+                     int32_t *fb_count, hipStream_t s)
 {
     const int g = grid_for(grid_items, 64, kMaxBlocks);
+#define GM_SOLVE_CASE(MX, SV)                                                                    \
+    hipLaunchKernelGGL((k_solve<MX, SV>), dim3(g), dim3(64), 0, s, dS, st, list, count, reset0,  \
+                       reset1, hist, fb_list, fb_count)
     if (mixed) {
-        if (solver == GM_SOLVE_SVD)
-            hipLaunchKernelGGL((k_solve<true, GM_SOLVE_SVD>), dim3(g), dim3(64), 0, s, dS, st, list, count, reset0, reset1);
-        else
-            hipLaunchKernelGGL((k_solve<true, GM_SOLVE_LDL>), dim3(g), dim3(64), 0, s, dS, st, list, count, reset0, reset1);
+        if (solver == GM_SOLVE_SVD) GM_SOLVE_CASE(true, GM_SOLVE_SVD);
+        else if (solver == GM_SOLVE_LDL) GM_SOLVE_CASE(true, GM_SOLVE_LDL);
+        else GM_SOLVE_CASE(true, GM_SOLVE_JACOBI);
     } else {
-        if (solver == GM_SOLVE_SVD)
-            hipLaunchKernelGGL((k_solve<false, GM_SOLVE_SVD>), dim3(g), dim3(64), 0, s, dS, st, list, count, reset0, reset1);
+        if (solver == GM_SOLVE_SVD) GM_SOLVE_CASE(false, GM_SOLVE_SVD);
+        else if (solver == GM_SOLVE_LDL) GM_SOLVE_CASE(false, GM_SOLVE_LDL);
+        else GM_SOLVE_CASE(false, GM_SOLVE_JACOBI);
+    }
+#undef GM_SOLVE_CASE
+    if (solver == GM_SOLVE_SVD) {
+        const int gf = grid_for(grid_items, 64, 1024);
+        if (mixed)
+            hipLaunchKernelGGL((k_solve_svd<true>), dim3(gf), dim3(64), 0, s, dS, st, fb_list, fb_count);
         else
-            hipLaunchKernelGGL((k_solve<false, GM_SOLVE_LDL>), dim3(g), dim3(64), 0, s, dS, st, list, count, reset0, reset1);
+            hipLaunchKernelGGL((k_solve_svd<false>), dim3(gf), dim3(64), 0, s, dS, st, fb_list, fb_count);
     }
 }
 
@@ -1156,31 +1266,32 @@ void gm_launch_step_solve(const float *alpha, const float *beta, const double *f
                           int64_t n, double *delta, int mixed, int solver, hipStream_t s)
 {
     const int g = grid_for(n, 64, kMaxBlocks);
+#define GM_STEP_CASE(MX, SV)                                                                     \
+    hipLaunchKernelGGL((k_step_solve<MX, SV>), dim3(g), dim3(64), 0, s, alpha, beta, flambda, n, delta)
     if (mixed) {
-        if (solver == GM_SOLVE_SVD)
-            hipLaunchKernelGGL((k_step_solve<true, GM_SOLVE_SVD>), dim3(g), dim3(64), 0, s, alpha, beta, flambda, n, delta);
-        else
-            hipLaunchKernelGGL((k_step_solve<true, GM_SOLVE_LDL>), dim3(g), dim3(64), 0, s, alpha, beta, flambda, n, delta);
+        if (solver == GM_SOLVE_SVD) GM_STEP_CASE(true, GM_SOLVE_SVD);
+        else if (solver == GM_SOLVE_LDL) GM_STEP_CASE(true, GM_SOLVE_LDL);
+        else GM_STEP_CASE(true, GM_SOLVE_JACOBI);
     } else {
-        if (solver == GM_SOLVE_SVD)
-            hipLaunchKernelGGL((k_step_solve<false, GM_SOLVE_SVD>), dim3(g), dim3(64), 0, s, alpha, beta, flambda, n, delta);
-        else
-            hipLaunchKernelGGL((k_step_solve<false, GM_SOLVE_LDL>), dim3(g), dim3(64), 0, s, alpha, beta, flambda, n, delta);
+        if (solver == GM_SOLVE_SVD) GM_STEP_CASE(false, GM_SOLVE_SVD);
+        else if (solver == GM_SOLVE_LDL) GM_STEP_CASE(false, GM_SOLVE_LDL);
+        else GM_STEP_CASE(false, GM_SOLVE_JACOBI);
     }
+#undef GM_STEP_CASE
 }
 
 void gm_launch_decide(const gm_dsetup *dS, gm_state st, const int32_t *list,
                       const int32_t *count, int64_t grid_items, int32_t *next_list,
-                      int32_t *next_count, int32_t *acc_list, int32_t *acc_count, int mixed,
-                      hipStream_t s)
+                      int32_t *next_count, int32_t *acc_list, int32_t *acc_count,
+                      int32_t *reset_fb, int mixed, hipStream_t s)
 {
-    const int g = grid_for(grid_items, 64, kMaxBlocks);
+    const int g = grid_for(grid_items, 1024, 512);
     if (mixed)
-        hipLaunchKernelGGL((k_decide<true>), dim3(g), dim3(64), 0, s, dS, st, list, count,
-                           next_list, next_count, acc_list, acc_count);
+        hipLaunchKernelGGL((k_decide<true>), dim3(g), dim3(1024), 0, s, dS, st, list, count,
+                           next_list, next_count, acc_list, acc_count, reset_fb);
     else
-        hipLaunchKernelGGL((k_decide<false>), dim3(g), dim3(64), 0, s, dS, st, list, count,
-                           next_list, next_count, acc_list, acc_count);
+        hipLaunchKernelGGL((k_decide<false>), dim3(g), dim3(1024), 0, s, dS, st, list, count,
+                           next_list, next_count, acc_list, acc_count, reset_fb);
 }
 
 void gm_launch_init_state(const gm_dsetup *dS, gm_state st, const float *obs,
@@ -1216,4 +1327,9 @@ void gm_launch_ce(const gm_dsetup *dS, const float *obs, int64_t npix, int initi
 void gm_launch_set_count(int32_t *count, int32_t value, hipStream_t s)
 {
     hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, s, count, value);
+}
+
+void gm_launch_copy_count(const int32_t *src, int32_t *dst, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(1), 0, s, src, dst);
 }

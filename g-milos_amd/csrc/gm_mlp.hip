@@ -1,0 +1,337 @@
+/*
+ * gm_mlp.hip -- learned initial guess: a 4-layer perceptron trained and run
+ * with hand-written BF16 MFMA kernels for gfx950.
+ *
+ * Network: input 4 x nlambda Stokes samples normalised by the continuum
+ * (padded to a multiple of 64), three hidden layers of 256 with GELU, 64
+ * outputs of which 11 are used: B, cos/sin(gamma), cos/sin(2 phi), vlos,
+ * dopp, a, eta0, S0/Ic, Ic (standardised).  BF16 weights and activations,
+ * FP32 accumulation, FP32 master weights updated by AdamW.
+ *
+ * GEMM tiles: a workgroup of 4 waves computes a 64 x 64 tile of C = A.B with
+ * v_mfma_f32_16x16x16_bf16 (each wave a 32 x 32 quarter = 2 x 2 MFMA
+ * blocks), staging 64 x 32 slices of A and B through LDS.
+ */
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <stdint.h>
+
+#include "gm_mlp.h"
+
+namespace {
+
+typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float x)
+{
+    /* round to nearest even */
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16) | ((u & 0xffffu) ? 0x40 : 0);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+
+/* GELU, tanh form (Hendrycks & Gimpel 2016) and its derivative. */
+__device__ __forceinline__ float gelu(float x)
+{
+    const float k = 0.7978845608028654f;   /* sqrt(2/pi) */
+    const float t = tanhf(k * (x + 0.044715f * x * x * x));
+    return 0.5f * x * (1.f + t);
+}
+__device__ __forceinline__ float gelu_grad(float x)
+{
+    const float k = 0.7978845608028654f;
+    const float u = k * (x + 0.044715f * x * x * x);
+    const float t = tanhf(u);
+    const float du = k * (1.f + 3.f * 0.044715f * x * x);
+    return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * du;
+}
+
+/* C[M x N] (+)= op(A)[M x K] . op(B)[K x N].
+ * A is row-major M x K (lda) or, with TA, stored K x M (A^T row-major).
+ * B is row-major K x N (ldb) or, with TB, stored N x K.
+ * M, N multiples of 64; K multiple of 32.
+ * Epilogues: EPI_BIAS_GELU (store pre-activation f32 and GELU bf16),
+ *            EPI_BIAS (store f32 and bf16), EPI_GELU_GRAD (multiply by
+ *            GELU'(Zprev) and store bf16 + f32), EPI_STORE (f32). */
+constexpr int EPI_BIAS_GELU = GM_EPI_BIAS_GELU, EPI_BIAS = GM_EPI_BIAS,
+              EPI_GELU_GRAD = GM_EPI_GELU_GRAD, EPI_STORE = GM_EPI_STORE;
+
+template <bool TA, bool TB, int EPI>
+__global__ __launch_bounds__(256) void k_gemm(const uint16_t *__restrict__ A, int lda,
+                                              const uint16_t *__restrict__ B, int ldb,
+                                              int M, int N, int K,
+                                              const float *__restrict__ bias,
+                                              const float *__restrict__ zprev, int ldz,
+                                              float *__restrict__ Cf, uint16_t *__restrict__ Cb,
+                                              int ldc)
+{
+    __shared__ uint16_t sA[64][32 + 8];   /* [m][k] */
+    __shared__ uint16_t sB[64][32 + 8];   /* [n][k] */
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    float4_t acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        /* stage 64 x 32 of A and B (2048 elements each, 8 per thread) */
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int idx = tid * 8 + e;
+            const int r = idx >> 5, c = idx & 31;          /* r: m or n, c: k */
+            sA[r][c] = TA ? A[(size_t)(k0 + c) * lda + (m0 + r)] : A[(size_t)(m0 + r) * lda + (k0 + c)];
+            sB[r][c] = TB ? B[(size_t)(n0 + r) * ldb + (k0 + c)] : B[(size_t)(k0 + c) * ldb + (n0 + r)];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 32; kk += 16) {
+            short4_t a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int row = wm + i * 16 + (lane & 15);
+                const int kb = kk + 4 * (lane >> 4);
+                a[i] = short4_t{(short)sA[row][kb], (short)sA[row][kb + 1], (short)sA[row][kb + 2],
+                                (short)sA[row][kb + 3]};
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int col = wn + j * 16 + (lane & 15);
+                const int kb = kk + 4 * (lane >> 4);
+                b[j] = short4_t{(short)sB[col][kb], (short)sB[col][kb + 1], (short)sB[col][kb + 2],
+                                (short)sB[col][kb + 3]};
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    /* D layout: lane holds rows 4*(lane/16)+r, column lane%16 */
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
+                const int n = n0 + wn + j * 16 + (lane & 15);
+                float v = acc[i][j][r];
+                const size_t o = (size_t)m * ldc + n;
+                if (EPI == EPI_BIAS_GELU) {
+                    v += bias[n];
+                    Cf[o] = v;
+                    Cb[o] = f2bf(gelu(v));
+                } else if (EPI == EPI_BIAS) {
+                    v += bias[n];
+                    Cf[o] = v;
+                    if (Cb) Cb[o] = f2bf(v);
+                } else if (EPI == EPI_GELU_GRAD) {
+                    v *= gelu_grad(zprev[(size_t)m * ldz + n]);
+                    if (Cf) Cf[o] = v;
+                    Cb[o] = f2bf(v);
+                } else {
+                    Cf[o] = v;
+                }
+            }
+}
+
+/* Input: obs float [npix][4][nl] -> bf16 [npix][kin], normalised by the
+ * continuum Ic = max(I[0], I[nl-1]); Q, U, V scaled by 10. */
+__global__ void k_prep_input(const float *__restrict__ obs, int64_t npix, int nl, int kin,
+                             uint16_t *__restrict__ x)
+{
+    const int64_t total = npix * kin;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = i / kin;
+        const int f = (int)(i % kin);
+        float v = 0.f;
+        if (f < 4 * nl) {
+            const float *o = obs + p * 4 * nl;
+            const float ic = fmaxf(fmaxf(o[0], o[nl - 1]), 1e-6f);
+            const int s = f / nl;
+            v = o[f] / ic;
+            v = (s == 0) ? (v - 1.f) * 2.f : v * 10.f;
+        }
+        x[i] = f2bf(v);
+    }
+}
+
+/* Targets from true models (standardised; gm_mlp.h documents the map). */
+__global__ void k_prep_target(const double *__restrict__ models, int64_t npix,
+                              float *__restrict__ t)
+{
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npix;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const double *m = models + p * 11;
+        const double g = m[5] * (3.14159265358979323846 / 180.0);
+        const double ph = 2.0 * m[6] * (3.14159265358979323846 / 180.0);
+        const double ic = m[7] + m[8];
+        float *o = t + p * GM_MLP_NOUT;
+        o[0] = (float)((m[1] - 1500.0) / 866.0);
+        o[1] = (float)cos(g);
+        o[2] = (float)(sin(g) * 2.0 - 1.0);
+        o[3] = (float)cos(ph);
+        o[4] = (float)sin(ph);
+        o[5] = (float)(m[2] / 1.732);
+        o[6] = (float)((m[3] - 0.04) / 0.01155);
+        o[7] = (float)((m[4] - 0.255) / 0.1414);
+        o[8] = (float)((m[0] - 10.5) / 5.485);
+        o[9] = (float)((m[7] / ic - 0.25) / 0.1);
+        o[10] = (float)((ic - 1.0) / 0.2);
+        for (int k = 11; k < GM_MLP_NOUT; ++k) o[k] = 0.f;
+    }
+}
+
+/* Output decode into a start model (init fields untouched where the network
+ * has no output), clamped to the LM's physical ranges. */
+__global__ void k_decode(const float *__restrict__ y, int64_t npix, const double *__restrict__ base,
+                         double *__restrict__ models)
+{
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npix;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const float *o = y + p * GM_MLP_NOUT;
+        double *m = models + p * 11;
+        for (int f = 0; f < 11; ++f) m[f] = base[f];
+        const double rad = 180.0 / 3.14159265358979323846;
+        double B = o[0] * 866.0 + 1500.0;
+        const double sg = fmax((o[2] + 1.0) * 0.5, 0.0);
+        double gm = atan2(sg, (double)o[1]) * rad;
+        double az = 0.5 * atan2((double)o[4], (double)o[3]) * rad;
+        if (az < 0) az += 180.0;
+        const double ic = fmax(o[10] * 0.2 + 1.0, 0.05);
+        const double s0f = fmin(fmax(o[9] * 0.1 + 0.25, 0.01), 0.99);
+        m[0] = fmin(fmax(o[8] * 5.485 + 10.5, 1.0), 2500.0);
+        m[1] = fmin(fmax(B, 0.0), 5000.0);
+        m[2] = fmin(fmax(o[5] * 1.732, -20.0), 20.0);
+        m[3] = fmin(fmax(o[6] * 0.01155 + 0.04, 0.005), 0.6);
+        m[4] = fmin(fmax(o[7] * 0.1414 + 0.255, 0.0001), 10.0);
+        m[5] = fmin(fmax(gm, 0.0), 180.0);
+        m[6] = fmin(fmax(az, 0.0), 180.0);
+        m[7] = s0f * ic;
+        m[8] = (1.0 - s0f) * ic;
+    }
+}
+
+/* Mean-squared error over the used outputs; dY = 2 (Y - T) / (B * 11). */
+__global__ void k_loss(const float *__restrict__ y, const float *__restrict__ t, int64_t b,
+                       uint16_t *__restrict__ dy, float *__restrict__ loss_acc)
+{
+    float local = 0.f;
+    const float sc = 2.f / (float)(b * 11);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < b * GM_MLP_NOUT;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int k = (int)(i % GM_MLP_NOUT);
+        float d = (k < 11) ? (y[i] - t[i]) : 0.f;
+        local += d * d;
+        dy[i] = f2bf(d * sc);
+    }
+    for (int o = 32; o >= 1; o >>= 1) local += __shfl_xor(local, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(loss_acc, local / (float)(b * 11));
+}
+
+/* db[n] = sum_b dY[b][n] (dY bf16). */
+__global__ void k_bias_grad(const uint16_t *__restrict__ dy, int64_t b, int n, float *__restrict__ db)
+{
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= n) return;
+    float s = 0.f;
+    for (int64_t r = 0; r < b; ++r) s += bf2f(dy[r * n + col]);
+    db[col] = s;
+}
+
+/* AdamW (Loshchilov & Hutter 2019) on FP32 master weights; refreshes the
+ * BF16 copy. */
+__global__ void k_adamw(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m,
+                        float *__restrict__ v, uint16_t *__restrict__ wb, int64_t n, float lr,
+                        float b1, float b2, float eps, float wd, float bc1, float bc2)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        float wi = w[i] * (1.f - lr * wd);
+        wi -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+        w[i] = wi;
+        if (wb) wb[i] = f2bf(wi);
+    }
+}
+
+__global__ void k_to_bf16(const float *__restrict__ x, uint16_t *__restrict__ y, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = f2bf(x[i]);
+}
+
+} // namespace
+
+/* ---- launchers ------------------------------------------------------------ */
+#define GEMM_CASE(TA, TB, EPI)                                                              \
+    hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), grid, dim3(256), 0, s, A, lda, B, ldb, M, N, K, \
+                       bias, zprev, ldz, Cf, Cb, ldc)
+
+void gm_mlp_gemm(int ta, int tb, int epi, const uint16_t *A, int lda, const uint16_t *B, int ldb,
+                 int M, int N, int K, const float *bias, const float *zprev, int ldz, float *Cf,
+                 uint16_t *Cb, int ldc, hipStream_t s)
+{
+    dim3 grid(N / 64, M / 64);
+    if (!ta && !tb) {
+        if (epi == EPI_BIAS_GELU) GEMM_CASE(false, false, EPI_BIAS_GELU);
+        else if (epi == EPI_BIAS) GEMM_CASE(false, false, EPI_BIAS);
+        else if (epi == EPI_GELU_GRAD) GEMM_CASE(false, false, EPI_GELU_GRAD);
+        else GEMM_CASE(false, false, EPI_STORE);
+    } else if (!ta && tb) {
+        if (epi == EPI_GELU_GRAD) GEMM_CASE(false, true, EPI_GELU_GRAD);
+        else GEMM_CASE(false, true, EPI_STORE);
+    } else if (ta && !tb) {
+        GEMM_CASE(true, false, EPI_STORE);
+    } else {
+        GEMM_CASE(true, true, EPI_STORE);
+    }
+}
+
+static int grid1d(int64_t n, int b) { int64_t g = (n + b - 1) / b; return (int)(g > 4096 ? 4096 : (g < 1 ? 1 : g)); }
+
+void gm_mlp_prep_input(const float *obs, int64_t npix, int nl, int kin, uint16_t *x, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_prep_input, dim3(grid1d(npix * kin, 256)), dim3(256), 0, s, obs, npix, nl, kin, x);
+}
+void gm_mlp_prep_target(const double *models, int64_t npix, float *t, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_prep_target, dim3(grid1d(npix, 256)), dim3(256), 0, s, models, npix, t);
+}
+void gm_mlp_decode(const float *y, int64_t npix, const double *base, double *models, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_decode, dim3(grid1d(npix, 256)), dim3(256), 0, s, y, npix, base, models);
+}
+void gm_mlp_loss(const float *y, const float *t, int64_t b, uint16_t *dy, float *loss, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_loss, dim3(grid1d(b * GM_MLP_NOUT, 256)), dim3(256), 0, s, y, t, b, dy, loss);
+}
+void gm_mlp_bias_grad(const uint16_t *dy, int64_t b, int n, float *db, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bias_grad, dim3((n + 63) / 64), dim3(64), 0, s, dy, b, n, db);
+}
+void gm_mlp_adamw(float *w, const float *g, float *m, float *v, uint16_t *wb, int64_t n, float lr,
+                  float b1, float b2, float eps, float wd, float bc1, float bc2, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_adamw, dim3(grid1d(n, 256)), dim3(256), 0, s, w, g, m, v, wb, n, lr, b1, b2,
+                       eps, wd, bc1, bc2);
+}
+void gm_mlp_to_bf16(const float *x, uint16_t *y, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_to_bf16, dim3(grid1d(n, 256)), dim3(256), 0, s, x, y, n);
+}

@@ -51,9 +51,11 @@ typedef enum {
 } gm_precision;
 
 typedef enum {
-    GM_SOLVE_SVD = 0,        /* symmetric SVD (Jacobi), pseudo-inverse with the
-                                reference's |w| > 1e-12 rule                */
-    GM_SOLVE_LDL = 1         /* pivot-thresholded LDL^T of the damped matrix */
+    GM_SOLVE_SVD = 0,        /* the reference's SVD pseudo-inverse (|w| > 1e-12
+                                rule): LDL^T where trace(A^-1) < 1e12 proves
+                                every eigenvalue > 1e-12, Jacobi SVD otherwise */
+    GM_SOLVE_LDL = 1,        /* LDL^T only (zero rows dropped)               */
+    GM_SOLVE_JACOBI = 2      /* always the Jacobi SVD                        */
 } gm_solver;
 
 typedef enum {
@@ -171,6 +173,31 @@ gm_status gm_step_solve(gm_ctx *ctx, const float *alpha, const float *beta,
 gm_status gm_classical_estimates(gm_ctx *ctx, const float *obs, int64_t npix,
                                  int32_t for_initial_use, double *models,
                                  void *stream);
+
+/* ---- learned initial guess ----------------------------------------------- */
+/* A perceptron (4 x nlambda -> 256 -> 256 -> 256 -> 11, GELU) trained and run
+ * with BF16 MFMA kernels; its prediction replaces the fixed init.mod start
+ * (GM_INIT_PER_PIXEL).  No reference counterpart: the reference always starts
+ * from init.mod + classical estimates (milosUtils.cu:615-628).              */
+typedef struct gm_mlp gm_mlp;
+gm_status gm_mlp_create(int32_t nlambda, uint64_t seed, gm_mlp **out);
+gm_status gm_mlp_destroy(gm_mlp *net);
+int64_t   gm_mlp_num_params(const gm_mlp *net);
+/* FP32 master parameters, layer by layer: W[in][out] row-major then b[out]. */
+gm_status gm_mlp_get_params(const gm_mlp *net, float *host, int64_t n);
+gm_status gm_mlp_set_params(gm_mlp *net, const float *host, int64_t n);
+/* One AdamW step on a batch (b a multiple of 64) of observations
+ * [b][4][nlambda] with true models [b][11]; the mean-squared loss of the
+ * batch (before the update) is written to *loss (host). */
+gm_status gm_mlp_train_step(gm_mlp *net, const float *obs, const double *models, int64_t b,
+                            float lr, float weight_decay, float *loss, void *stream);
+/* Loss and gradients (host [num_params]) without updating: for checks. */
+gm_status gm_mlp_gradients(gm_mlp *net, const float *obs, const double *models, int64_t b,
+                           float *grads, float *loss, void *stream);
+/* Start models [npix][11] for any npix: network outputs decoded onto `base`
+ * (host [11], supplies mac / alfa), clamped to the LM's ranges. */
+gm_status gm_mlp_predict(gm_mlp *net, const float *obs, int64_t npix, const double *base,
+                         double *models, void *stream);
 
 /* ---- host-side set-up helpers (no device work) -------------------------- */
 /* Zeeman pattern from quantum numbers (create_cuantic.c:31-215). */
