@@ -86,12 +86,33 @@ class gm_stats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class gm_server_config(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("pool_pixels", ctypes.c_int64),
+                ("max_tiles", ctypes.c_int32), ("device", ctypes.c_int32)]
+
+
+class gm_tile_result(ctypes.Structure):
+    _fields_ = [("ticket", ctypes.c_uint64), ("setup_id", ctypes.c_int32),
+                ("status", ctypes.c_int32), ("npix", ctypes.c_int64),
+                ("t_submit_ms", ctypes.c_double), ("t_admit_ms", ctypes.c_double),
+                ("t_done_ms", ctypes.c_double)]
+
+
+class gm_server_stats(ctypes.Structure):
+    _fields_ = [("tiles_done", ctypes.c_int64), ("pixels_done", ctypes.c_int64),
+                ("lm_rounds", ctypes.c_int64), ("pending_tiles", ctypes.c_int64),
+                ("inflight_tiles", ctypes.c_int64), ("inflight_bytes", ctypes.c_int64),
+                ("peak_inflight_bytes", ctypes.c_int64)]
+
+
 EXPORTED = ("gm_ctx_create", "gm_ctx_destroy", "gm_status_string", "gm_ctx_set_timing",
             "gm_last_stats", "gm_invert", "gm_synth", "gm_synth_rf", "gm_normal_equations",
             "gm_step_solve", "gm_classical_estimates", "gm_zeeman_from_quantum",
             "gm_parse_lines_file", "gm_psf_gaussian", "gm_psf_from_table", "gm_setup_defaults",
             "gm_mlp_create", "gm_mlp_destroy", "gm_mlp_num_params", "gm_mlp_get_params",
-            "gm_mlp_set_params", "gm_mlp_train_step", "gm_mlp_gradients", "gm_mlp_predict")
+            "gm_mlp_set_params", "gm_mlp_train_step", "gm_mlp_gradients", "gm_mlp_predict",
+            "gm_server_create", "gm_server_add_setup", "gm_server_submit", "gm_server_poll",
+            "gm_server_get_stats", "gm_server_clock_ms", "gm_server_destroy")
 
 _lib = None
 
@@ -134,6 +155,13 @@ def load_library(path=LIB_PATH):
                                P(ctypes.c_float), vp], i32),
         "gm_mlp_gradients": ([vp, vp, vp, i64, vp, P(ctypes.c_float), vp], i32),
         "gm_mlp_predict": ([vp, vp, i64, vp, vp, vp], i32),
+        "gm_server_create": ([P(gm_server_config), P(vp)], i32),
+        "gm_server_add_setup": ([vp, P(gm_setup), P(i32)], i32),
+        "gm_server_submit": ([vp, i32, vp, i64, vp, P(gm_out), P(ctypes.c_uint64)], i32),
+        "gm_server_poll": ([vp, P(gm_tile_result), i32, P(i32), i32], i32),
+        "gm_server_get_stats": ([vp, P(gm_server_stats)], i32),
+        "gm_server_clock_ms": ([vp], ctypes.c_double),
+        "gm_server_destroy": ([vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -519,6 +547,90 @@ class MLP:
             out.append((dims[l], dims[l + 1], wo, off))
             off += dims[l + 1]
         return out
+
+
+class Server:
+    """Serving loop (gm_server): tiles of any size for registered setups are
+    submitted as they arrive and complete independently.  The caller keeps a
+    tile's device buffers alive until its result is polled; this wrapper holds
+    references to the tensors for that purpose."""
+
+    def __init__(self, hbm_budget_bytes=0, pool_pixels=0, max_tiles=0, device=0):
+        self.lib = load_library()
+        cfg = gm_server_config(int(hbm_budget_bytes), int(pool_pixels), int(max_tiles), int(device))
+        h = ctypes.c_void_p()
+        _check(self.lib.gm_server_create(ctypes.byref(cfg), ctypes.byref(h)), "gm_server_create")
+        self.h = h
+        self.nlambda = {}
+        self.keep = {}
+
+    def add_setup(self, setup):
+        sid = ctypes.c_int32()
+        _check(self.lib.gm_server_add_setup(self.h, ctypes.byref(setup), ctypes.byref(sid)),
+               "gm_server_add_setup")
+        self.nlambda[sid.value] = setup.nlambda
+        return sid.value
+
+    def submit(self, setup_id, obs, init=None, want_synth=True):
+        """obs: float32 [npix, 4, nlambda] device tensor, already written (the
+        producer stream synchronised).  Returns (ticket, outputs dict)."""
+        import torch
+        assert obs.dtype == torch.float32 and obs.is_contiguous()
+        assert obs.shape[1] == 4 and obs.shape[2] == self.nlambda[setup_id]
+        npix, dev = obs.shape[0], obs.device
+        out = {"model": torch.empty((npix, GM_NMODEL), dtype=torch.float64, device=dev),
+               "chisqr": torch.empty(npix, dtype=torch.float32, device=dev),
+               "iters": torch.empty(npix, dtype=torch.int32, device=dev),
+               "naccs": torch.empty(npix, dtype=torch.int32, device=dev)}
+        if want_synth:
+            out["synth"] = torch.empty_like(obs)
+        torch.cuda.synchronize(dev)
+        o = gm_out(out["model"].data_ptr(), out["chisqr"].data_ptr(), out["iters"].data_ptr(),
+                   out["synth"].data_ptr() if want_synth else None, out["naccs"].data_ptr())
+        t = ctypes.c_uint64()
+        ip = ctypes.c_void_p(init.data_ptr()) if init is not None else None
+        _check(self.lib.gm_server_submit(self.h, int(setup_id), ctypes.c_void_p(obs.data_ptr()),
+                                         npix, ip, ctypes.byref(o), ctypes.byref(t)),
+               "gm_server_submit")
+        self.keep[t.value] = (obs, init, out)
+        return t.value, out
+
+    def poll(self, max_results=256, wait_ms=0):
+        """Completed tiles as dicts (ticket, setup_id, status, npix, latency_ms,
+        queue_ms, outputs)."""
+        buf = (gm_tile_result * max_results)()
+        n = ctypes.c_int32()
+        _check(self.lib.gm_server_poll(self.h, buf, max_results, ctypes.byref(n), int(wait_ms)),
+               "gm_server_poll")
+        res = []
+        for k in range(n.value):
+            r = buf[k]
+            kept = self.keep.pop(r.ticket, (None, None, None))
+            res.append({"ticket": r.ticket, "setup_id": r.setup_id, "status": r.status,
+                        "npix": r.npix, "t_submit_ms": r.t_submit_ms,
+                        "latency_ms": r.t_done_ms - r.t_submit_ms,
+                        "queue_ms": r.t_admit_ms - r.t_submit_ms, "out": kept[2]})
+        return res
+
+    def clock_ms(self):
+        return self.lib.gm_server_clock_ms(self.h)
+
+    def stats(self):
+        st = gm_server_stats()
+        _check(self.lib.gm_server_get_stats(self.h, ctypes.byref(st)), "gm_server_get_stats")
+        return {f: getattr(st, f) for f, _ in gm_server_stats._fields_}
+
+    def close(self):
+        if self.h:
+            self.lib.gm_server_destroy(self.h)
+            self.h = None
+            self.keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def packed_index(p, q):

@@ -268,6 +268,12 @@ extern "C" gm_status gm_last_stats(const gm_ctx *c, gm_stats *s)
 }
 
 /* Carves the SoA state out of one allocation sized for `cap` pixels. */
+const gm_dsetup *gm_ctx_device_setup(const gm_ctx *c) { return c->dS; }
+const gm_dsetup &gm_ctx_host_setup(const gm_ctx *c) { return c->host_ds; }
+const gm_lanecfg &gm_ctx_lanes(const gm_ctx *c) { return c->lc; }
+const gm_setup &gm_ctx_public_setup(const gm_ctx *c) { return c->setup; }
+int gm_ctx_device(const gm_ctx *c) { return c->device; }
+
 static gm_status ensure_state(gm_ctx *c, int64_t npix)
 {
     if (npix <= c->st.cap) return GM_OK;
@@ -355,7 +361,7 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
 
     GM_HIP(hipEventRecord(c->ev_start, s));
     kbegin(3);
-    gm_launch_init_state(c->dS, st, obs, init, c->setup.init_mode, npix, st.list[0], mixed, s);
+    gm_launch_init_state(c->dS, st, obs, init, c->setup.init_mode, 0, npix, st.list[0], s);
     gm_launch_set_count(&cnt[0], (int32_t)npix, s);
     gm_launch_set_count(&cnt[3], 0, s);
     kend();
@@ -364,7 +370,7 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
                  st.beta, st.cap, st.ochisqr, out->synth, nullptr, mixed, s);
     kend();
     kbegin(3);
-    gm_launch_init_chi(st, npix, s);
+    gm_launch_init_chi(st, 0, npix, s);
     kend();
     int cur = 0;
     const int miter = c->setup.miter > 0 ? c->setup.miter : 1;  /* do-while */
@@ -384,7 +390,7 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
         kend();
         kbegin(3);
         gm_launch_decide(c->dS, st, st.list[cur], &cnt[cur], npix, st.list[nxt], &cnt[nxt],
-                         st.list[2], &cnt[2], &cnt[3], mixed, s);
+                         st.list[2], &cnt[2], &cnt[3], nullptr, nullptr, mixed, s);
         kend();
         kbegin(1);
         gm_launch_rf(lc, c->dS, st.best, st.cap, obs, st.list[2], &cnt[2], 0, npix, st.alpha,

@@ -67,6 +67,18 @@ struct gm_state {
     int32_t *hist;        /* [3 * (max rounds + 1)] per-round counts          */
 };
 
+/* Output slot of one tile of the serving loop (device table). */
+struct gm_tile_dev {
+    double  *model;       /* [npix][11]                                      */
+    float   *chisqr;
+    int32_t *iters;
+    int32_t *naccs;
+    float   *synth;       /* [npix][4][nlambda] or NULL                      */
+    int64_t  slot0;       /* first state slot of the tile                    */
+    int32_t  remaining;   /* pixels still in the LM loop                     */
+    int32_t  pad;
+};
+
 /* Lane mapping of the wavelength-parallel kernels. */
 struct gm_lanecfg {
     int lpp;   /* lanes per pixel (32 or 64)                                 */
@@ -99,16 +111,30 @@ void gm_launch_step_solve(const float *alpha, const float *beta, const double *f
 void gm_launch_decide(const gm_dsetup *dS, gm_state st, const int32_t *list,
                       const int32_t *count, int64_t grid_items, int32_t *next_list,
                       int32_t *next_count, int32_t *acc_list, int32_t *acc_count,
-                      int32_t *reset_fb, int mixed, hipStream_t stream);
+                      int32_t *reset_fb, int32_t *done_list, int32_t *done_count, int mixed,
+                      hipStream_t stream);
 void gm_launch_init_state(const gm_dsetup *dS, gm_state st, const float *obs,
-                          const double *init, int init_mode, int64_t npix,
-                          int32_t *list0, int mixed, hipStream_t stream);
-void gm_launch_init_chi(gm_state st, int64_t npix, hipStream_t stream);
+                          const double *init, int init_mode, int64_t slot0, int64_t npix,
+                          int32_t *list0, hipStream_t stream);
+void gm_launch_init_chi(gm_state st, int64_t slot0, int64_t npix, hipStream_t stream);
+void gm_launch_append_range(int32_t *list, int32_t *count, int64_t slot0, int64_t n,
+                            hipStream_t stream);
+void gm_launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t stream);
+void gm_launch_retire(gm_state st, const int32_t *done, const int32_t *done_count,
+                      int64_t grid_items, const float *synth, int nl4, const int32_t *slot_tile,
+                      gm_tile_dev *tiles, int32_t *completed, hipStream_t stream);
 void gm_launch_finalize(gm_state st, int64_t npix, const gm_out &out,
                         hipStream_t stream);
 void gm_launch_ce(const gm_dsetup *dS, const float *obs, int64_t npix, int initial,
                   double *models, hipStream_t stream);
 void gm_launch_set_count(int32_t *count, int32_t value, hipStream_t stream);
 void gm_launch_copy_count(const int32_t *src, int32_t *dst, hipStream_t stream);
+
+/* Context internals shared with the serving loop (gm_capi.cpp). */
+const gm_dsetup *gm_ctx_device_setup(const gm_ctx *c);
+const gm_dsetup &gm_ctx_host_setup(const gm_ctx *c);
+const gm_lanecfg &gm_ctx_lanes(const gm_ctx *c);
+const gm_setup &gm_ctx_public_setup(const gm_ctx *c);
+int gm_ctx_device(const gm_ctx *c);
 
 #endif

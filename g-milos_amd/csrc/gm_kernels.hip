@@ -25,7 +25,6 @@ namespace {
 
 constexpr float kVLIGHT = 2.99792458e+5f;   /* defines.h:80 */
 constexpr float kCTE = 4.6686411e-13f;      /* defines.h:82 */
-constexpr float kRR = 0.5641895836f;        /* defines.h:78 */
 constexpr double kPI = 3.14159265358979323846264338327950288;
 
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -66,6 +65,65 @@ __device__ __forceinline__ void voigt32(float a, float u, float &H, float &F)
     const float inv = frcp(qrs * qrs + qis * qis) * s;
     H = (pr * qrs + pi * qis) * inv;
     F = 0.5f * (pi * qrs - pr * qis) * inv;
+}
+
+/* Voigt, Faraday-Voigt and their u-derivatives.  The derivatives are the
+ * identities dH/du = 4aF - 2uH, dF/du = 1/sqrt(pi) - aH - 2uF
+ * (me_der.cu) applied to the rational W = P/Q, i.e. 2 Im(Z W) and
+ * 1/sqrt(pi) - Re(Z W).  Since Z P/Q = 1/sqrt(pi) - R/Q with
+ * R(Z) = Q(Z)/sqrt(pi) - Z P(Z), they equal -2 Im(R/Q) and Re(R/Q) exactly;
+ * evaluating R (coefficients formed in double) avoids the cancellation of
+ * the two terms in the line wings, where |u|^2 times the FP32 rounding of H
+ * and F would otherwise dominate.  (The reference's RR constant differs from
+ * 1/sqrt(pi) by 5e-11, below FP32 resolution of every use.)                */
+__device__ __forceinline__ void voigt32_d(float a, float u, float &H, float &F, float &dH,
+                                          float &dF)
+{
+    constexpr double P[7] = {122.607931777104326, 214.382388694706425, 181.928533092181549,
+                             93.155580458138441, 30.180142196210589, 5.912626209773153,
+                             0.564189583562615};
+    constexpr double Q[8] = {122.60793177387535, 352.730625110963558, 457.334478783897737,
+                             348.703917719495792, 170.354001821091472, 53.992906912940207,
+                             10.479857114260399, 1.0};
+    constexpr double RSPI = 0.56418958354775628695;
+    const float zr = a, zi = -u;
+    float pr = (float)P[6], pi = 0.f;
+#pragma unroll
+    for (int k = 5; k >= 0; --k) {
+        const float tr = pr * zr - pi * zi + (float)P[k];
+        const float ti = pr * zi + pi * zr;
+        pr = tr;
+        pi = ti;
+    }
+    float qr = (float)Q[7], qi = 0.f;
+#pragma unroll
+    for (int k = 6; k >= 0; --k) {
+        const float tr = qr * zr - qi * zi + (float)Q[k];
+        const float ti = qr * zi + qi * zr;
+        qr = tr;
+        qi = ti;
+    }
+    float rr = (float)(Q[7] * RSPI - P[6]), ri = 0.f;
+#pragma unroll
+    for (int k = 6; k >= 1; --k) {
+        const float tr = rr * zr - ri * zi + (float)(Q[k] * RSPI - P[k - 1]);
+        const float ti = rr * zi + ri * zr;
+        rr = tr;
+        ri = ti;
+    }
+    {
+        const float tr = rr * zr - ri * zi + (float)(Q[0] * RSPI);
+        const float ti = rr * zi + ri * zr;
+        rr = tr;
+        ri = ti;
+    }
+    const float s = frcp(fmaxf(fabsf(qr), fabsf(qi)));
+    const float qrs = qr * s, qis = qi * s;
+    const float inv = frcp(qrs * qrs + qis * qis) * s;
+    H = (pr * qrs + pi * qis) * inv;
+    F = 0.5f * (pi * qrs - pr * qis) * inv;
+    dF = (rr * qrs + ri * qis) * inv;
+    dH = -2.f * (ri * qrs - rr * qis) * inv;
 }
 
 /* Model-dependent constants shared by all wavelengths of a pixel
@@ -134,13 +192,16 @@ __device__ __forceinline__ void eval_lambda(const gm_dsetup *__restrict__ S,
                 const float nu = parcial * S->comp_nu[L][t][i];
                 const float w = S->comp_w[L][t][i] * fo;
                 const float u = ub - nu * pp.B;
-                float H, F;
-                voigt32(pp.aa, u, H, F);
+                float H, F, dHv = 0.f, dFv = 0.f;
+                if (RF)
+                    voigt32_d(pp.aa, u, H, F, dHv, dFv);
+                else
+                    voigt32(pp.aa, u, H, F);
                 fi[t] += w * H;
                 shi[t] += 2.f * w * F;
                 if (RF) {
-                    const float dH_u = (4.f * pp.aa * F - 2.f * u * H) * w;
-                    const float dF_u = (kRR - pp.aa * H - 2.f * u * F) * w;
+                    const float dH_u = dHv * w;
+                    const float dF_u = dFv * w;
                     const float uu = -u * pp.inv_dopp;
                     dfi[t][0] += dH_u * (-nu);
                     dfi[t][1] += dH_u * auxCte;
@@ -894,11 +955,13 @@ __global__ __launch_bounds__(1024) void k_decide(const gm_dsetup *__restrict__ S
                                                  int32_t *__restrict__ next_count,
                                                  int32_t *__restrict__ acc_list,
                                                  int32_t *__restrict__ acc_count,
-                                                 int32_t *__restrict__ reset_fb)
+                                                 int32_t *__restrict__ reset_fb,
+                                                 int32_t *__restrict__ done_list,
+                                                 int32_t *__restrict__ done_count)
 {
     using T = typename std::conditional<MIXED, double, float>::type;
-    __shared__ int wc[16], wa[16];
-    __shared__ int base_c, base_a;
+    __shared__ int wc[16], wa[16], wd[16];
+    __shared__ int base_c, base_a, base_d;
     if (blockIdx.x == 0 && threadIdx.x == 0 && reset_fb) *reset_fb = 0;
     const int64_t cnt = *count;
     const int64_t cap = st.cap;
@@ -938,29 +1001,36 @@ __global__ __launch_bounds__(1024) void k_decide(const gm_dsetup *__restrict__ S
             accn = accept && cont;
             if (accn) st.nacc[pix] += 1;
         }
+        const bool fin = valid && !cont && done_list;
         const uint64_t mc = __ballot(cont);
         const uint64_t ma = __ballot(accn);
+        const uint64_t md = __ballot(fin);
         if (lane == 0) {
             wc[wave] = (int)__popcll(mc);
             wa[wave] = (int)__popcll(ma);
+            wd[wave] = (int)__popcll(md);
         }
         __syncthreads();
         if (tid == 0) {
-            int sc = 0, sa = 0;
+            int sc = 0, sa = 0, sd = 0;
             for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
-                const int c = wc[w], a = wa[w];
+                const int c = wc[w], a = wa[w], d = wd[w];
                 wc[w] = sc;
                 wa[w] = sa;
+                wd[w] = sd;
                 sc += c;
                 sa += a;
+                sd += d;
             }
             base_c = sc ? atomicAdd(next_count, sc) : 0;
             base_a = sa ? atomicAdd(acc_count, sa) : 0;
+            base_d = sd ? atomicAdd(done_count, sd) : 0;
         }
         __syncthreads();
         const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
         if (cont) next_list[base_c + wc[wave] + __popcll(mc & below)] = pix;
         if (accn) acc_list[base_a + wa[wave] + __popcll(ma & below)] = pix;
+        if (fin) done_list[base_d + wd[wave] + __popcll(md & below)] = pix;
         __syncthreads();
     }
 }
@@ -1072,16 +1142,21 @@ __global__ __launch_bounds__(64) void k_ce(const gm_dsetup *__restrict__ S,
     }
 }
 
+/* LM start state of state slots slot0 .. slot0+npix-1 (milosUtils.cu:
+ * 595-643): start model (fixed, classical estimates or per pixel), lambda,
+ * counters, and the slot list for the first normal-equation evaluation.
+ * obs is indexed by slot, init by p.                                      */
 __global__ __launch_bounds__(64) void k_init_state(const gm_dsetup *__restrict__ S, gm_state st,
                                                    const float *__restrict__ obs,
                                                    const double *__restrict__ init,
-                                                   int init_mode, int64_t npix,
+                                                   int init_mode, int64_t slot0, int64_t npix,
                                                    int32_t *__restrict__ list0)
 {
     const int n = S->nlambda;
     const int64_t cap = st.cap;
     for (int64_t p = (int64_t)blockIdx.x * 64 + threadIdx.x; p < npix;
          p += (int64_t)gridDim.x * 64) {
+        const int64_t q = slot0 + p;
         double m[GM_NMODEL];
         if (init_mode == GM_INIT_PER_PIXEL) {
 #pragma unroll
@@ -1089,26 +1164,78 @@ __global__ __launch_bounds__(64) void k_init_state(const gm_dsetup *__restrict__
         } else {
 #pragma unroll
             for (int f = 0; f < GM_NMODEL; ++f) m[f] = S->init_model[f];
-            if (init_mode == GM_INIT_CE) classical_estimates(S, obs + p * 4 * n, m, 1);
+            if (init_mode == GM_INIT_CE) classical_estimates(S, obs + q * 4 * n, m, 1);
         }
 #pragma unroll
         for (int f = 0; f < GM_NMODEL; ++f) {
-            st.best[f * cap + p] = m[f];
-            st.trial[f * cap + p] = m[f];
+            st.best[f * cap + q] = m[f];
+            st.trial[f * cap + q] = m[f];
         }
-        st.flambda[p] = S->ilambda;
-        st.factor[p] = 1.0;
-        st.iter[p] = 0;
-        st.nacc[p] = 0;
-        list0[p] = (int32_t)p;
+        st.flambda[q] = S->ilambda;
+        st.factor[q] = 1.0;
+        st.iter[q] = 0;
+        st.nacc[q] = 0;
+        list0[p] = (int32_t)q;
     }
 }
 
-__global__ void k_init_chi(gm_state st, int64_t npix)
+__global__ void k_init_chi(gm_state st, int64_t slot0, int64_t npix)
 {
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npix;
          p += (int64_t)gridDim.x * blockDim.x)
-        st.chisqr0[p] = st.ochisqr[p];
+        st.chisqr0[slot0 + p] = st.ochisqr[slot0 + p];
+}
+
+/* list[*count + i] = slot0 + i for i < n; k_add_count then advances *count. */
+__global__ void k_append_range(int32_t *__restrict__ list, const int32_t *__restrict__ count,
+                               int64_t slot0, int64_t n)
+{
+    const int64_t base = *count;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        list[base + i] = (int32_t)(slot0 + i);
+}
+__global__ void k_add_count(int32_t *c, int32_t v) { *c += v; }
+
+__global__ void k_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+/* Retire pixels that left the LM loop: copy the best model, chi^2,
+ * iteration and acceptance counts and the last evaluated spectrum from the
+ * state slot to the owning tile's output, one wave per pixel; the last pixel
+ * of a tile appends the tile to the completed list (completed[0] = count). */
+__global__ __launch_bounds__(256) void k_retire(gm_state st, const int32_t *__restrict__ done,
+                                                const int32_t *__restrict__ done_count,
+                                                const float *__restrict__ synth, int nl4,
+                                                const int32_t *__restrict__ slot_tile,
+                                                gm_tile_dev *__restrict__ tiles,
+                                                int32_t *__restrict__ completed)
+{
+    const int64_t cnt = *done_count;
+    const int64_t cap = st.cap;
+    const int lane = threadIdx.x & 63;
+    for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < cnt;
+         i += ((int64_t)gridDim.x * blockDim.x) >> 6) {
+        const int64_t slot = done[i];
+        const int t = slot_tile[slot];
+        gm_tile_dev &g = tiles[t];
+        const int64_t idx = slot - g.slot0;
+        if (g.synth)
+            for (int k = lane; k < nl4; k += 64) g.synth[idx * nl4 + k] = synth[slot * nl4 + k];
+        if (lane < GM_NMODEL && g.model) g.model[idx * GM_NMODEL + lane] = st.best[lane * cap + slot];
+        if (lane == 0) {
+            if (g.chisqr) g.chisqr[idx] = (float)st.ochisqr[slot];
+            if (g.iters) g.iters[idx] = st.iter[slot];
+            if (g.naccs) g.naccs[idx] = st.nacc[slot];
+        }
+        __threadfence();
+        if (lane == 0 && atomicSub(&g.remaining, 1) == 1)
+            completed[1 + atomicAdd(&completed[0], 1)] = t;
+    }
 }
 
 __global__ void k_finalize(gm_state st, int64_t npix, double *__restrict__ model,
@@ -1283,31 +1410,56 @@ void gm_launch_step_solve(const float *alpha, const float *beta, const double *f
 void gm_launch_decide(const gm_dsetup *dS, gm_state st, const int32_t *list,
                       const int32_t *count, int64_t grid_items, int32_t *next_list,
                       int32_t *next_count, int32_t *acc_list, int32_t *acc_count,
-                      int32_t *reset_fb, int mixed, hipStream_t s)
+                      int32_t *reset_fb, int32_t *done_list, int32_t *done_count, int mixed,
+                      hipStream_t s)
 {
     const int g = grid_for(grid_items, 1024, 512);
     if (mixed)
         hipLaunchKernelGGL((k_decide<true>), dim3(g), dim3(1024), 0, s, dS, st, list, count,
-                           next_list, next_count, acc_list, acc_count, reset_fb);
+                           next_list, next_count, acc_list, acc_count, reset_fb, done_list,
+                           done_count);
     else
         hipLaunchKernelGGL((k_decide<false>), dim3(g), dim3(1024), 0, s, dS, st, list, count,
-                           next_list, next_count, acc_list, acc_count, reset_fb);
+                           next_list, next_count, acc_list, acc_count, reset_fb, done_list,
+                           done_count);
 }
 
 void gm_launch_init_state(const gm_dsetup *dS, gm_state st, const float *obs,
-                          const double *init, int init_mode, int64_t npix, int32_t *list0,
-                          int mixed, hipStream_t s)
+                          const double *init, int init_mode, int64_t slot0, int64_t npix,
+                          int32_t *list0, hipStream_t s)
 {
-    (void)mixed;
     const int g = grid_for(npix, 64, kMaxBlocks);
-    hipLaunchKernelGGL(k_init_state, dim3(g), dim3(64), 0, s, dS, st, obs, init, init_mode, npix,
-                       list0);
+    hipLaunchKernelGGL(k_init_state, dim3(g), dim3(64), 0, s, dS, st, obs, init, init_mode, slot0,
+                       npix, list0);
 }
 
-void gm_launch_init_chi(gm_state st, int64_t npix, hipStream_t s)
+void gm_launch_init_chi(gm_state st, int64_t slot0, int64_t npix, hipStream_t s)
 {
     const int g = grid_for(npix, 256, kMaxBlocks);
-    hipLaunchKernelGGL(k_init_chi, dim3(g), dim3(256), 0, s, st, npix);
+    hipLaunchKernelGGL(k_init_chi, dim3(g), dim3(256), 0, s, st, slot0, npix);
+}
+
+void gm_launch_append_range(int32_t *list, int32_t *count, int64_t slot0, int64_t n,
+                            hipStream_t s)
+{
+    const int g = grid_for(n, 256, kMaxBlocks);
+    hipLaunchKernelGGL(k_append_range, dim3(g), dim3(256), 0, s, list, count, slot0, n);
+    hipLaunchKernelGGL(k_add_count, dim3(1), dim3(1), 0, s, count, (int32_t)n);
+}
+
+void gm_launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t s)
+{
+    const int g = grid_for(n, 256, kMaxBlocks);
+    hipLaunchKernelGGL(k_fill_i32, dim3(g), dim3(256), 0, s, p, n, v);
+}
+
+void gm_launch_retire(gm_state st, const int32_t *done, const int32_t *done_count,
+                      int64_t grid_items, const float *synth, int nl4, const int32_t *slot_tile,
+                      gm_tile_dev *tiles, int32_t *completed, hipStream_t s)
+{
+    const int g = grid_for(grid_items, 4, kMaxBlocks);
+    hipLaunchKernelGGL(k_retire, dim3(g), dim3(256), 0, s, st, done, done_count, synth, nl4,
+                       slot_tile, tiles, completed);
 }
 
 void gm_launch_finalize(gm_state st, int64_t npix, const gm_out &out, hipStream_t s)

@@ -174,6 +174,59 @@ gm_status gm_classical_estimates(gm_ctx *ctx, const float *obs, int64_t npix,
                                  int32_t for_initial_use, double *models,
                                  void *stream);
 
+/* ---- serving loop ---------------------------------------------------------- */
+/* A persistent inversion service.  No reference counterpart: the reference
+ * inverts one cube per process (gmilos.cu:1200-1256).  Tiles of any size are
+ * submitted as they arrive, each for one registered setup; a worker thread
+ * admits them into a device-resident LM state pool of that setup while the
+ * summed footprint of in-flight tiles stays within the HBM budget, appends
+ * their pixels to the running LM loop at the next round (continuous
+ * batching: pixels of several tiles share every launch), retires converged
+ * pixels into the tile's outputs and compacts the survivors.  Setups run on
+ * separate streams and overlap on the device.  Every pixel's result equals
+ * the one gm_invert gives for it: the LM state is per pixel.               */
+typedef struct gm_server gm_server;
+typedef struct {
+    int64_t hbm_budget_bytes;        /* in-flight footprint limit (0: 8 GiB)    */
+    int64_t pool_pixels;             /* state slots per setup (0: budget-sized) */
+    int32_t max_tiles;               /* in-flight tiles per setup (0: 1024)     */
+    int32_t device;                  /* HIP device ordinal                      */
+} gm_server_config;
+typedef struct {
+    uint64_t ticket;
+    int32_t  setup_id;
+    int32_t  status;                 /* gm_status                               */
+    int64_t  npix;
+    double   t_submit_ms;            /* server clock (gm_server_clock_ms)       */
+    double   t_admit_ms;
+    double   t_done_ms;              /* completion observed by the worker       */
+} gm_tile_result;
+typedef struct {
+    int64_t tiles_done, pixels_done;
+    int64_t lm_rounds;               /* rounds launched, all setups              */
+    int64_t pending_tiles, inflight_tiles;
+    int64_t inflight_bytes, peak_inflight_bytes;
+} gm_server_stats;
+
+gm_status gm_server_create(const gm_server_config *cfg, gm_server **out);
+/* Registers a setup (validated as gm_ctx_create); returns its id. */
+gm_status gm_server_add_setup(gm_server *srv, const gm_setup *setup, int32_t *setup_id);
+/* Queues a tile.  obs [npix][4][nlambda] and the outputs in *out are device
+ * memory the caller keeps valid until the tile's result has been polled;
+ * init [npix][11] when the setup's init_mode is GM_INIT_PER_PIXEL.  npix
+ * must not exceed the setup's pool.  Thread-safe.                          */
+gm_status gm_server_submit(gm_server *srv, int32_t setup_id, const float *obs, int64_t npix,
+                           const double *init, const gm_out *out, uint64_t *ticket);
+/* Moves up to max completed tiles into res (host); waits up to wait_ms for
+ * at least one when none is ready (0: no wait, <0: until one completes or
+ * nothing is outstanding). */
+gm_status gm_server_poll(gm_server *srv, gm_tile_result *res, int32_t max, int32_t *n,
+                         int32_t wait_ms);
+gm_status gm_server_get_stats(gm_server *srv, gm_server_stats *stats);
+double    gm_server_clock_ms(const gm_server *srv);
+/* Stops the worker after the outstanding tiles complete, frees the pools. */
+gm_status gm_server_destroy(gm_server *srv);
+
 /* ---- learned initial guess ----------------------------------------------- */
 /* A perceptron (4 x nlambda -> 256 -> 256 -> 256 -> 11, GELU) trained and run
  * with BF16 MFMA kernels; its prediction replaces the fixed init.mod start
