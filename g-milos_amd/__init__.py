@@ -105,6 +105,22 @@ class gm_server_stats(ctypes.Structure):
                 ("peak_inflight_bytes", ctypes.c_int64)]
 
 
+class gm_prior(ctypes.Structure):
+    _fields_ = [("lo", ctypes.c_double * GM_NMODEL), ("hi", ctypes.c_double * GM_NMODEL),
+                ("weak_fraction", ctypes.c_double), ("weak_b_max", ctypes.c_double)]
+
+
+class gm_guided_stats(ctypes.Structure):
+    _fields_ = [("npix", ctypes.c_int64), ("n_fallback", ctypes.c_int64),
+                ("n_replaced", ctypes.c_int64), ("iters_learned", ctypes.c_int64),
+                ("iters_fallback", ctypes.c_int64), ("ms_predict", ctypes.c_float),
+                ("ms_lm", ctypes.c_float), ("ms_fallback", ctypes.c_float),
+                ("ms_total", ctypes.c_float)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 EXPORTED = ("gm_ctx_create", "gm_ctx_destroy", "gm_status_string", "gm_ctx_set_timing",
             "gm_last_stats", "gm_invert", "gm_synth", "gm_synth_rf", "gm_normal_equations",
             "gm_step_solve", "gm_classical_estimates", "gm_zeeman_from_quantum",
@@ -112,7 +128,9 @@ EXPORTED = ("gm_ctx_create", "gm_ctx_destroy", "gm_status_string", "gm_ctx_set_t
             "gm_mlp_create", "gm_mlp_destroy", "gm_mlp_num_params", "gm_mlp_get_params",
             "gm_mlp_set_params", "gm_mlp_train_step", "gm_mlp_gradients", "gm_mlp_predict",
             "gm_server_create", "gm_server_add_setup", "gm_server_submit", "gm_server_poll",
-            "gm_server_get_stats", "gm_server_clock_ms", "gm_server_destroy")
+            "gm_server_get_stats", "gm_server_clock_ms", "gm_server_destroy",
+            "gm_prior_default", "gm_draw_models", "gm_add_noise", "gm_mlp_fit",
+            "gm_invert_guided")
 
 _lib = None
 
@@ -162,6 +180,14 @@ def load_library(path=LIB_PATH):
         "gm_server_get_stats": ([vp, P(gm_server_stats)], i32),
         "gm_server_clock_ms": ([vp], ctypes.c_double),
         "gm_server_destroy": ([vp], i32),
+        "gm_prior_default": ([P(gm_prior)], None),
+        "gm_draw_models": ([P(gm_prior), ctypes.c_uint64, i64, i64, vp, vp], i32),
+        "gm_add_noise": ([vp, vp, i64, i32, ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
+                          vp], i32),
+        "gm_mlp_fit": ([vp, vp, P(gm_prior), i64, i64, i64, ctypes.c_float, ctypes.c_float,
+                        ctypes.c_double, ctypes.c_uint64, vp, vp], i32),
+        "gm_invert_guided": ([vp, vp, vp, vp, i64, ctypes.c_double, P(gm_out),
+                              P(gm_guided_stats), vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -474,6 +500,54 @@ class Context:
         return m
 
 
+def default_prior(weak_fraction=0.0):
+    pr = gm_prior()
+    load_library().gm_prior_default(ctypes.byref(pr))
+    pr.weak_fraction = float(weak_fraction)
+    return pr
+
+
+def draw_models(npix, seed, prior=None, offset=0, device="cuda"):
+    """Device-drawn true models [npix, 11] (float64 tensor)."""
+    import torch
+    pr = prior if prior is not None else default_prior()
+    m = torch.empty((npix, GM_NMODEL), dtype=torch.float64, device=device)
+    _check(load_library().gm_draw_models(ctypes.byref(pr), int(seed), int(offset), int(npix),
+                                         ctypes.c_void_p(m.data_ptr()), Context._stream()),
+           "gm_draw_models")
+    return m
+
+
+def add_noise(models, spectra, sigma, seed, mu=1.0):
+    """In place: spectra += N(0, (sigma Ic)^2), Ic = S0 + S1 mu."""
+    _check(load_library().gm_add_noise(ctypes.c_void_p(models.data_ptr()),
+                                       ctypes.c_void_p(spectra.data_ptr()), spectra.shape[0],
+                                       spectra.shape[2], float(sigma), float(mu), int(seed),
+                                       Context._stream()), "gm_add_noise")
+    return spectra
+
+
+def invert_guided(learned_ctx, plain_ctx, net, obs, fallback_chisqr, want_synth=True):
+    """Learned start + LM with the fixed-guess fallback; returns (out, stats)."""
+    import torch
+    npix, dev = obs.shape[0], obs.device
+    out = {"model": torch.empty((npix, GM_NMODEL), dtype=torch.float64, device=dev),
+           "chisqr": torch.empty(npix, dtype=torch.float32, device=dev),
+           "iters": torch.empty(npix, dtype=torch.int32, device=dev),
+           "naccs": torch.empty(npix, dtype=torch.int32, device=dev)}
+    if want_synth:
+        out["synth"] = torch.empty_like(obs)
+    o = gm_out(out["model"].data_ptr(), out["chisqr"].data_ptr(), out["iters"].data_ptr(),
+               out["synth"].data_ptr() if want_synth else None, out["naccs"].data_ptr())
+    st = gm_guided_stats()
+    _check(load_library().gm_invert_guided(learned_ctx.h, plain_ctx.h, net.h,
+                                           ctypes.c_void_p(obs.data_ptr()), npix,
+                                           float(fallback_chisqr), ctypes.byref(o),
+                                           ctypes.byref(st), Context._stream()),
+           "gm_invert_guided")
+    return out, st.as_dict()
+
+
 class MLP:
     """Learned initial guess (BF16 MFMA network); device buffers are torch
     tensors on the current device."""
@@ -524,6 +598,19 @@ class MLP:
                                          g.ctypes.data_as(ctypes.c_void_p), ctypes.byref(loss),
                                          Context._stream()), "gm_mlp_gradients")
         return loss.value, g
+
+    def fit(self, ctx, n_train, batch=4096, steps=2000, lr=2e-3, weight_decay=1e-4,
+            sigma=1e-3, seed=1, prior=None, curve=True):
+        """Synthesise n_train profiles with ctx and train; returns the loss
+        curve (numpy) when curve is set."""
+        pr = prior if prior is not None else default_prior()
+        lc = np.zeros(steps, dtype=np.float32) if curve else None
+        _check(self.lib.gm_mlp_fit(self.h, ctx.h, ctypes.byref(pr), int(n_train), int(batch),
+                                   int(steps), float(lr), float(weight_decay), float(sigma),
+                                   int(seed),
+                                   lc.ctypes.data_as(ctypes.c_void_p) if curve else None,
+                                   Context._stream()), "gm_mlp_fit")
+        return lc
 
     def predict(self, obs, base):
         import torch

@@ -57,11 +57,6 @@ static gm_status hip_ok(hipError_t e)
     return (e == hipErrorOutOfMemory) ? GM_ERR_NOMEM : GM_ERR_HIP;
 }
 
-#define GM_TRY(x)                                                              \
-    do {                                                                       \
-        gm_status _s = (x);                                                    \
-        if (_s != GM_OK) return _s;                                            \
-    } while (0)
 #define GM_HIP(x) GM_TRY(hip_ok(x))
 
 extern "C" const char *gm_status_string(gm_status st)

@@ -130,6 +130,12 @@ void gm_launch_ce(const gm_dsetup *dS, const float *obs, int64_t npix, int initi
 void gm_launch_set_count(int32_t *count, int32_t value, hipStream_t stream);
 void gm_launch_copy_count(const int32_t *src, int32_t *dst, hipStream_t stream);
 
+#define GM_TRY(x)                                                              \
+    do {                                                                       \
+        gm_status _s = (x);                                                    \
+        if (_s != GM_OK) return _s;                                            \
+    } while (0)
+
 /* Context internals shared with the serving loop (gm_capi.cpp). */
 const gm_dsetup *gm_ctx_device_setup(const gm_ctx *c);
 const gm_dsetup &gm_ctx_host_setup(const gm_ctx *c);

@@ -252,6 +252,47 @@ gm_status gm_mlp_gradients(gm_mlp *net, const float *obs, const double *models, 
 gm_status gm_mlp_predict(gm_mlp *net, const float *obs, int64_t npix, const double *base,
                          double *models, void *stream);
 
+/* Training data and the guided inversion.  gm_prior: uniform ranges per
+ * model field (lo == hi fixes a field), plus a fraction of pixels whose B is
+ * redrawn below weak_b_max.  gm_prior_default gives eta0 1-20, B 0-3000 G,
+ * vlos -3..3 km/s, dopp 20-60 mA, a 0.01-0.5, gamma and phi 0-180 deg,
+ * S0 0.1-0.4, S1 0.5-0.9, mac 0, alpha 1.                                   */
+typedef struct {
+    double lo[GM_NMODEL], hi[GM_NMODEL];
+    double weak_fraction, weak_b_max;
+} gm_prior;
+void      gm_prior_default(gm_prior *prior);
+/* models [npix][11] (device) for pixel indices offset .. offset+npix-1; a
+ * pixel's draw depends only on (seed, index). */
+gm_status gm_draw_models(const gm_prior *prior, uint64_t seed, int64_t offset, int64_t npix,
+                         double *models, void *stream);
+/* Adds N(0, (sigma Ic)^2), Ic = S0 + S1 mu, to every sample of spectra. */
+gm_status gm_add_noise(const double *models, float *spectra, int64_t npix, int32_t nlambda,
+                       double sigma, double mu, uint64_t seed, void *stream);
+/* Draws n_train models from the prior, synthesises them with synth_ctx
+ * (its line, grid and PSF), adds noise sigma, and runs `steps` AdamW steps
+ * of `batch` rows (linear warm-up, cosine decay from lr to lr/20).
+ * loss_curve (host, [steps]) receives each step's loss when not NULL.      */
+gm_status gm_mlp_fit(gm_mlp *net, gm_ctx *synth_ctx, const gm_prior *prior, int64_t n_train,
+                     int64_t batch, int64_t steps, float lr, float weight_decay, double sigma,
+                     uint64_t seed, float *loss_curve, void *stream);
+typedef struct {
+    int64_t npix;
+    int64_t n_fallback;              /* pixels re-inverted from the fixed guess */
+    int64_t n_replaced;              /* ... whose fallback chi^2 was lower      */
+    int64_t iters_learned;           /* LM iterations, learned start (sum)      */
+    int64_t iters_fallback;          /* LM iterations of the fallback (sum)     */
+    float   ms_predict, ms_lm, ms_fallback, ms_total;
+} gm_guided_stats;
+/* Learned start + LM (learned: init_mode GM_INIT_PER_PIXEL), then every
+ * pixel with chi^2 > fallback_chisqr is inverted again by `plain` (its
+ * fixed guess, classical estimates as configured) and takes that result
+ * when its chi^2 is lower.  The network's decode uses plain's init model
+ * for the fields it does not predict (mac, alpha).                          */
+gm_status gm_invert_guided(gm_ctx *learned, gm_ctx *plain, gm_mlp *net, const float *obs,
+                           int64_t npix, double fallback_chisqr, const gm_out *out,
+                           gm_guided_stats *stats, void *stream);
+
 /* ---- host-side set-up helpers (no device work) -------------------------- */
 /* Zeeman pattern from quantum numbers (create_cuantic.c:31-215). */
 gm_status gm_zeeman_from_quantum(double sl, double ll, double jl, double su,
