@@ -75,6 +75,151 @@ def cpu_baseline(gm, setup_desc, obs_host, budget_s):
                       f"({dt:.1f} s); throughput is linear in pixel count"}
 
 
+def serving_leg(gm, torch, dev, seconds, rate, seed):
+    """Poisson stream of tiles (sizes log-uniform 256-262144 px), each for the
+    1-line 30-lambda set-up or the 2-line 112-lambda set-up, half of them with
+    the PSF; observations are slices of per-set-up cubes synthesised up front
+    (already in HBM).  Reports tile latency percentiles (submit -> complete)
+    and sustained pixels/s over the window."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    setups = [gm.setup_single_line(psf_fwhm=60.0), gm.setup_single_line(),
+              gm.setup_two_lines(psf_fwhm=60.0), gm.setup_two_lines()]
+    names = ["6173/30l/psf", "6173/30l", "6301+6302/112l/psf", "6301+6302/112l"]
+    maxpix = 262144
+    cubes = []
+    for k, su in enumerate(setups):
+        c = gm.Context(su)
+        truth = gm.draw_models(maxpix, seed=seed + k)
+        obs = c.synth(truth)
+        gm.add_noise(truth, obs, 1e-3, seed=seed + 17 * k)
+        cubes.append(obs)
+        del c
+    torch.cuda.synchronize()
+    srv = gm.Server(hbm_budget_bytes=24 << 30, pool_pixels=1 << 21)
+    ids = [srv.add_setup(su) for su in setups]
+    # arrival schedule
+    t_arr, tiles, t = [], [], 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= seconds:
+            break
+        k = int(rng.integers(0, 4))
+        n = int(np.exp(rng.uniform(np.log(256), np.log(maxpix))))
+        o = int(rng.integers(0, maxpix - n + 1))
+        t_arr.append(t)
+        tiles.append((k, n, o))
+    lat, per_setup, done_px_window = [], {i: [] for i in range(4)}, 0
+    t_start = time.perf_counter()
+    clock0 = srv.clock_ms()
+    submitted = 0
+    results = []
+    while submitted < len(tiles):
+        now = time.perf_counter() - t_start
+        if t_arr[submitted] <= now:
+            k, n, o = tiles[submitted]
+            srv.submit(ids[k], cubes[k][o:o + n], want_synth=False)
+            submitted += 1
+            continue
+        results += srv.poll(wait_ms=0)
+        time.sleep(min(0.0005, max(0.0, t_arr[submitted] - now)))
+    window_end = clock0 + seconds * 1000.0
+    while len(results) < len(tiles):
+        results += srv.poll(wait_ms=100)
+    st = srv.stats()
+    srv.close()
+    bad = [r for r in results if r["status"] != 0]
+    for r in results:
+        lat.append(r["latency_ms"])
+        per_setup[r["setup_id"]].append(r["latency_ms"])
+        if r["t_submit_ms"] + r["latency_ms"] <= window_end:
+            done_px_window += r["npix"]
+    total_px = sum(n for _, n, _ in tiles)
+    lat = np.array(lat)
+    return {
+        "workload": f"Poisson {rate:g} tiles/s for {seconds:g} s, tile sizes log-uniform "
+                    "256-262144 px, set-ups 50/50 1-line 30-lambda / 2-line 112-lambda, "
+                    "50% with PSF, 9 free par, 50 LM it",
+        "tiles": len(tiles), "failed": len(bad), "offered_px_per_s": total_px / seconds,
+        "sustained_px_per_s": done_px_window / seconds,
+        "drain_px_per_s": total_px / (max(r["t_submit_ms"] + r["latency_ms"] for r in results)
+                                      - clock0) * 1000.0,
+        "p50_tile_latency_ms": float(np.percentile(lat, 50)),
+        "p99_tile_latency_ms": float(np.percentile(lat, 99)),
+        "per_setup_p99_ms": {names[i]: (float(np.percentile(v, 99)) if v else None)
+                             for i, v in per_setup.items()},
+        "lm_rounds": st["lm_rounds"], "peak_inflight_bytes": st["peak_inflight_bytes"],
+    }
+
+
+def guided_leg(gm, torch, dev, npix, n_train, steps, seed, oracle_sample):
+    """Learned initial guess + LM (mixed precision) against the plain FP32 LM
+    from the fixed guess on the same cube; parameter error of both against the
+    FP64 oracle on a sample of pixels."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    plain_su = gm.setup_single_line(psf_fwhm=60.0, precision=gm.GM_PREC_FP32)
+    learned_su = gm.setup_single_line(psf_fwhm=60.0, precision=gm.GM_PREC_MIXED,
+                                      init_mode=gm.GM_INIT_PER_PIXEL)
+    plain, learned = gm.Context(plain_su), gm.Context(learned_su)
+    net = gm.MLP(30, seed=seed)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    curve = net.fit(plain, n_train=n_train, batch=4096, steps=steps, lr=2e-3, seed=seed + 1)
+    train_s = time.perf_counter() - t0
+    truth = gm.draw_models(npix, seed=seed + 2)
+    obs = plain.synth(truth)
+    gm.add_noise(truth, obs, 1e-3, seed=seed + 3)
+    torch.cuda.synchronize()
+    # plain FP32 path (warm-up on a slice, then timed)
+    plain.invert(obs[:65536].contiguous())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ref = plain.invert(obs)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    thr = float(np.percentile(ref["chisqr"].cpu().numpy(), 99))
+    gm.invert_guided(learned, plain, net, obs[:65536].contiguous(), thr)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out, st = gm.invert_guided(learned, plain, net, obs, thr)
+    torch.cuda.synchronize()
+    t_guided = time.perf_counter() - t0
+    res = {
+        "workload": f"{npix} px, Fe I 6173.34 A, 30 lambda, PSF FWHM 60 mA, 9 free par, "
+                    f"LM <= 50 it; network trained on {n_train} engine-synthesised "
+                    f"noisy profiles ({steps} AdamW steps of 4096)",
+        "train_s": train_s, "train_loss_first": float(curve[:20].mean()),
+        "train_loss_last": float(curve[-20:].mean()),
+        "plain_fp32_px_per_s": npix / t_plain, "guided_px_per_s": npix / t_guided,
+        "speedup": t_plain / t_guided,
+        "plain_iters_per_px": float(ref["iters"].double().mean()),
+        "guided_iters_per_px": st["iters_learned"] / npix,
+        "fallback_iters_per_px": st["iters_fallback"] / npix,
+        "fallback_threshold_chisqr": thr, "n_fallback": st["n_fallback"],
+        "n_replaced": st["n_replaced"],
+        "guided_ms": {k: st[k] for k in ("ms_predict", "ms_lm", "ms_fallback", "ms_total")},
+    }
+    if oracle_sample > 0:
+        import oracle
+        oracle.lib()
+        o = oracle.make_setup([gm.FE6173], gm.GRID_6173, psf=("gauss", 60.0),
+                              init_defaults=gm.BENCH_INIT, free=gm.FREE9, miter=50, noise=1e-6)
+        k = oracle_sample
+        ref_o = oracle.invert(o, obs[:k].cpu().numpy(), threads=min(os.cpu_count() or 1, 16),
+                              want_synth=False)["model"]
+
+        def med(m):
+            d = np.abs(m - ref_o)
+            d[:, 6] = np.minimum(d[:, 6], 180.0 - d[:, 6])
+            d = np.median(d, axis=0)
+            return {"B_G": d[1], "vlos_m_s": d[2] * 1e3, "gamma_deg": d[5], "phi_deg": d[6]}
+        res["median_err_vs_fp64_oracle"] = {
+            "sample_px": k, "plain_fp32": med(ref["model"][:k].cpu().numpy()),
+            "guided": med(out["model"][:k].cpu().numpy())}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +232,14 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--serve-seconds", type=float, default=20.0,
+                    help="length of the serving-loop leg (0 skips it)")
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--guided-npix", type=int, default=2048 * 2048,
+                    help="cube of the learned-guess leg (0 skips it)")
+    ap.add_argument("--train-profiles", type=int, default=1 << 22)
+    ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--oracle-sample", type=int, default=1024)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -189,6 +342,14 @@ def main():
         res["cpu_baseline"] = cpu_baseline(
             gm, ([gm.FE6173], gm.GRID_6173, ("gauss", 60.0), {"miter": args.miter}),
             obs_host, args.cpu_budget)
+    if args.serve_seconds > 0:
+        sv = serving_leg(gm, torch, dev, args.serve_seconds, args.serve_rate, args.seed)
+        res["serving"] = sv
+        res["p50_tile_latency_ms"] = sv["p50_tile_latency_ms"]
+        res["p99_tile_latency_ms"] = sv["p99_tile_latency_ms"]
+    if args.guided_npix > 0:
+        res["learned_guess"] = guided_leg(gm, torch, dev, args.guided_npix, args.train_profiles,
+                                          args.train_steps, args.seed + 5, args.oracle_sample)
     print(json.dumps(res), flush=True)
 
 

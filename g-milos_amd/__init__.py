@@ -671,7 +671,7 @@ class Server:
                "naccs": torch.empty(npix, dtype=torch.int32, device=dev)}
         if want_synth:
             out["synth"] = torch.empty_like(obs)
-        torch.cuda.synchronize(dev)
+        torch.cuda.current_stream(dev).synchronize()      # obs written, outputs allocated
         o = gm_out(out["model"].data_ptr(), out["chisqr"].data_ptr(), out["iters"].data_ptr(),
                    out["synth"].data_ptr() if want_synth else None, out["naccs"].data_ptr())
         t = ctypes.c_uint64()
