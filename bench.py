@@ -75,6 +75,32 @@ def cpu_baseline(gm, setup_desc, obs_host, budget_s):
                       f"({dt:.1f} s); throughput is linear in pixel count"}
 
 
+def pmc_probe(gm, torch, dev, npix, seed):
+    """Counter calibration: a device-to-device copy of 256 MiB (its
+    FETCH_SIZE / WRITE_SIZE scale the counters to bytes), then one launch of
+    k_rf (normal equations at given models, all pixels) and one of k_synth
+    over npix pixels of the 30-lambda PSF set-up."""
+    setup = gm.setup_single_line(psf_fwhm=60.0)
+    ctx = gm.Context(setup)
+    n_copy = 64 << 20
+    a = torch.ones(n_copy, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    truth = gm.draw_models(npix, seed=seed)
+    obs = ctx.synth(truth)
+    gm.add_noise(truth, obs, 1e-3, seed=seed + 1)
+    models = gm.draw_models(npix, seed=seed + 2)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    ctx.normal_equations(models, obs)
+    torch.cuda.synchronize()
+    ctx.synth(models)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": 4 * n_copy, "npix": npix,
+                      "k_rf_algorithmic_bytes": npix * (11 * 8 + 4 * 30 * 4 + 54 * 4 + 8),
+                      "k_synth_algorithmic_bytes": npix * (11 * 8 + 4 * 30 * 4)}), flush=True)
+
+
 def serving_leg(gm, torch, dev, seconds, rate, seed):
     """Poisson stream of tiles (sizes log-uniform 256-262144 px), each for the
     1-line 30-lambda set-up or the 2-line 112-lambda set-up, half of them with
@@ -240,6 +266,9 @@ def main():
     ap.add_argument("--train-profiles", type=int, default=1 << 22)
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--oracle-sample", type=int, default=1024)
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="only run a plain device copy of known size and one normal-equation "
+                         "(k_rf) and one synthesis (k_synth) launch, for counter runs")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -254,6 +283,10 @@ def main():
     gm.load_library()
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+
+    if args.pmc_probe:
+        pmc_probe(gm, torch, dev, args.npix, args.seed)
+        return
 
     prec = gm.GM_PREC_MIXED if args.precision == "mixed" else gm.GM_PREC_FP32
     solver = {"svd": gm.GM_SOLVE_SVD, "ldl": gm.GM_SOLVE_LDL,
