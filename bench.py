@@ -24,6 +24,9 @@ METRIC = ("inverted pixels/s on 1 MI355X (1024², 30λ, 9 par, 50 it, PSF); "
 PUBLISHED_PX_S = 7400.0           # Tesla V100, 30 lambda, 9 par, <= 50 it, PSF
 PEAK_FP32_TFLOPS = 157.3          # MI355X FP32 (MFMA / packed VALU) peak
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per pixel of one k_rf launch (FETCH_SIZE + WRITE_SIZE, scaled by a
+# copy kernel of known size): profiles/k_rf_k_synth_hbm_counters_262144px.md
+K_RF_HBM_BYTES_PER_PIXEL = 905.0
 
 
 def load_product():
@@ -332,6 +335,8 @@ def main():
     W = work_model(30, 3, taps)
     rf_flops = acc["rf_evals"] * (W["rf"] + W["ne"])
     rf_ms = acc["ms_rf"]
+    rf_launches = args.steps * (1 + max(args.miter, 1))   # the initial one + one per round
+    rf_px_per_launch = acc["rf_evals"] / rf_launches
     rf_tflops = rf_flops / (rf_ms / 1000.0) / 1e12 if rf_ms > 0 else 0.0
     total_flops = (acc["synth_evals"] * (W["synth"] + W["chi"]) +
                    acc["rf_evals"] * (W["rf"] + W["ne"]) + acc["solves"] * W["solve"])
@@ -360,7 +365,9 @@ def main():
         "roofline": {"bound": "mfma", "compute": "fp32 (unpacked VALU)",
                      "kernel": "k_rf (synthesis+RF+PSF+normal equations)",
                      "achieved": rf_tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": rf_tflops / PEAK_FP32_TFLOPS, "traffic": None},
+                     "frac": rf_tflops / PEAK_FP32_TFLOPS,
+                     "traffic": K_RF_HBM_BYTES_PER_PIXEL * rf_px_per_launch,
+                     "traffic_unit": "bytes per launch (mean launch of the timed steps)"},
         "whole_job_tflops": job_tflops,
         "kernel_ms_per_step": {k[3:]: acc[k] / args.steps for k in
                                ("ms_rf", "ms_synth", "ms_solve", "ms_other")},
