@@ -142,6 +142,12 @@ def load_library(path=LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is missing: build the HIP library first")
+    try:
+        # torch ships its own HIP runtime; it has to be the first one loaded
+        # so that this library and torch share one runtime in the process
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(path)
     vp, i64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32
     P = ctypes.POINTER

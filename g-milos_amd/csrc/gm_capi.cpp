@@ -273,9 +273,9 @@ static gm_status ensure_state(gm_ctx *c, int64_t npix)
 {
     if (npix <= c->st.cap) return GM_OK;
     int64_t cap = ((npix + 255) / 256) * 256;
-    const size_t per = sizeof(double) * (2 * GM_NMODEL + 5) + sizeof(float) * (GM_ALPHA_N + 9) +
-                       sizeof(int32_t) * (2 + 4);
-    const size_t bytes = per * (size_t)cap + 256 * 16 + 64 + sizeof(int32_t) * 3 * (kMaxRounds + 2) + 256;
+    const size_t per = sizeof(double) * (2 * GM_NMODEL + 5) +
+                       sizeof(float) * (GM_ALPHA_N + 9 + 2 * GM_NPRE) + sizeof(int32_t) * (2 + 4);
+    const size_t bytes = per * (size_t)cap + 256 * 20 + 64 + sizeof(int32_t) * 3 * (kMaxRounds + 2) + 256;
     if (c->ws) {
         hipDeviceSynchronize();
         hipFree(c->ws);
@@ -296,6 +296,8 @@ static gm_status ensure_state(gm_ctx *c, int64_t npix)
     st.trial = (double *)take(sizeof(double) * GM_NMODEL * cap);
     st.alpha = (float *)take(sizeof(float) * GM_ALPHA_N * cap);
     st.beta = (float *)take(sizeof(float) * 9 * cap);
+    st.pre_best = (float *)take(sizeof(float) * GM_NPRE * cap);
+    st.pre_trial = (float *)take(sizeof(float) * GM_NPRE * cap);
     st.chi_trial = (double *)take(sizeof(double) * cap);
     st.ochisqr = (double *)take(sizeof(double) * cap);
     st.chisqr0 = (double *)take(sizeof(double) * cap);
@@ -361,7 +363,7 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
     gm_launch_set_count(&cnt[3], 0, s);
     kend();
     kbegin(1);
-    gm_launch_rf(lc, c->dS, st.best, st.cap, obs, st.list[0], &cnt[0], 0, npix, st.alpha,
+    gm_launch_rf(lc, c->dS, st.best, st.pre_best, st.cap, obs, st.list[0], &cnt[0], 0, npix, st.alpha,
                  st.beta, st.cap, st.ochisqr, out->synth, nullptr, mixed, s);
     kend();
     kbegin(3);
@@ -380,7 +382,7 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
                         &cnt[2], hist + 3 * r, st.list[3], &cnt[3], s);
         kend();
         kbegin(0);
-        gm_launch_synth(lc, c->dS, st.trial, st.cap, obs, st.list[cur], &cnt[cur], 0, npix,
+        gm_launch_synth(lc, c->dS, st.trial, st.pre_trial, st.cap, obs, st.list[cur], &cnt[cur], 0, npix,
                         st.chi_trial, out->synth, mixed, s);
         kend();
         kbegin(3);
@@ -388,7 +390,7 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
                          st.list[2], &cnt[2], &cnt[3], nullptr, nullptr, mixed, s);
         kend();
         kbegin(1);
-        gm_launch_rf(lc, c->dS, st.best, st.cap, obs, st.list[2], &cnt[2], 0, npix, st.alpha,
+        gm_launch_rf(lc, c->dS, st.best, st.pre_best, st.cap, obs, st.list[2], &cnt[2], 0, npix, st.alpha,
                      st.beta, st.cap, nullptr, nullptr, nullptr, mixed, s);
         kend();
         cur = nxt;
@@ -437,7 +439,7 @@ extern "C" gm_status gm_synth(gm_ctx *c, const double *models, int64_t npix, flo
     if (!c || npix < 0) return GM_ERR_INVALID;
     if (npix == 0) return GM_OK;
     if (!models || !spectra) return GM_ERR_INVALID;
-    gm_launch_synth(c->lc, c->dS, models, 0, nullptr, nullptr, nullptr, npix, npix, nullptr,
+    gm_launch_synth(c->lc, c->dS, models, nullptr, 0, nullptr, nullptr, nullptr, npix, npix, nullptr,
                     spectra, 0, (hipStream_t)stream);
     return hip_ok(hipGetLastError());
 }
@@ -448,7 +450,7 @@ extern "C" gm_status gm_synth_rf(gm_ctx *c, const double *models, int64_t npix, 
     if (!c || npix < 0) return GM_ERR_INVALID;
     if (npix == 0) return GM_OK;
     if (!models) return GM_ERR_INVALID;
-    gm_launch_rf(c->lc, c->dS, models, 0, nullptr, nullptr, nullptr, npix, npix, nullptr, nullptr,
+    gm_launch_rf(c->lc, c->dS, models, nullptr, 0, nullptr, nullptr, nullptr, npix, npix, nullptr, nullptr,
                  0, nullptr, spectra, rf, 0, (hipStream_t)stream);
     return hip_ok(hipGetLastError());
 }
@@ -461,7 +463,7 @@ extern "C" gm_status gm_normal_equations(gm_ctx *c, const double *models, const 
     if (npix == 0) return GM_OK;
     if (!models || !obs) return GM_ERR_INVALID;
     const int mixed = c->setup.precision == GM_PREC_MIXED;
-    gm_launch_rf(c->lc, c->dS, models, 0, obs, nullptr, nullptr, npix, npix, alpha, beta, 0,
+    gm_launch_rf(c->lc, c->dS, models, nullptr, 0, obs, nullptr, nullptr, npix, npix, alpha, beta, 0,
                  chisqr, nullptr, nullptr, mixed, (hipStream_t)stream);
     return hip_ok(hipGetLastError());
 }

@@ -10,6 +10,7 @@
 #include "gmilos.h"
 
 #define GM_ALPHA_N   45          /* packed upper triangle of the 9x9 system */
+#define GM_NPRE      19          /* per-pixel FP32 synthesis constants      */
 #define GM_RED_N     55          /* 45 alpha + 9 beta + 1 chi^2             */
 #define GM_SLOTS     36          /* staged values per wavelength: per Stokes
                                     [synth, eta0, B, vlos, dopp, a, gamma, phi,
@@ -54,6 +55,8 @@ struct gm_state {
     double  *trial;       /* [11][cap] last proposed model                  */
     float   *alpha;       /* [45][cap] normal matrix at best                */
     float   *beta;        /* [9][cap]                                        */
+    float   *pre_best;    /* [GM_NPRE][cap] synthesis constants of best      */
+    float   *pre_trial;   /* [GM_NPRE][cap] ... of trial                     */
     double  *chi_trial;   /* [cap] chi^2 of trial                           */
     double  *ochisqr;     /* [cap] chi^2 of best                            */
     double  *chisqr0;     /* [cap] initial chi^2                            */
@@ -91,14 +94,16 @@ struct gm_lanecfg {
 gm_lanecfg gm_lane_config(const gm_dsetup &h);
 
 /* Launchers (gm_kernels.hip).  `list` may be NULL (identity) with the item
- * count given by *count (device) or by n_direct when count is NULL.          */
+ * count given by *count (device) or by n_direct when count is NULL.  `pre`,
+ * when not NULL, holds the pixels' synthesis constants (SoA, same stride as
+ * the models); otherwise they are derived from the models.                 */
 void gm_launch_rf(const gm_lanecfg &lc, const gm_dsetup *dS, const double *models,
-                  int64_t model_stride, const float *obs, const int32_t *list,
+                  const float *pre, int64_t model_stride, const float *obs, const int32_t *list,
                   const int32_t *count, int64_t n_direct, int64_t grid_items,
                   float *alpha, float *beta, int64_t ab_stride, double *chi_out,
                   float *synth_out, float *rf_out, int mixed, hipStream_t st);
 void gm_launch_synth(const gm_lanecfg &lc, const gm_dsetup *dS, const double *models,
-                     int64_t model_stride, const float *obs, const int32_t *list,
+                     const float *pre, int64_t model_stride, const float *obs, const int32_t *list,
                      const int32_t *count, int64_t n_direct, int64_t grid_items,
                      double *chi_out, float *synth_out, int mixed, hipStream_t st);
 void gm_launch_solve(const gm_dsetup *dS, gm_state st, const int32_t *list,

@@ -27,6 +27,10 @@ constexpr float kVLIGHT = 2.99792458e+5f;   /* defines.h:80 */
 constexpr float kCTE = 4.6686411e-13f;      /* defines.h:82 */
 constexpr double kPI = 3.14159265358979323846264338327950288;
 
+#ifndef GM_RF_WAVES
+#define GM_RF_WAVES 3   /* waves per SIMD the RF kernel is compiled for */
+#endif
+
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 /* ------------------------------------------------------------------------ */
@@ -133,6 +137,26 @@ struct PixPre {
     float sinis, cosi, cosis_2, sina, cosa, sinis_cosa, sinis_sina;
     float sinda, cosda, sindi, cosdi;
 };
+
+static_assert(sizeof(PixPre) == GM_NPRE * sizeof(float), "PixPre layout");
+
+/* PixPre rows of the LM state, SoA [GM_NPRE][stride]. */
+__device__ __forceinline__ void store_pre(float *__restrict__ pre, int64_t stride, int64_t pix,
+                                          const PixPre &p)
+{
+    const float *v = reinterpret_cast<const float *>(&p);
+#pragma unroll
+    for (int f = 0; f < GM_NPRE; ++f) pre[f * stride + pix] = v[f];
+}
+__device__ __forceinline__ PixPre load_pre(const float *__restrict__ pre, int64_t stride,
+                                           int64_t pix)
+{
+    PixPre p;
+    float *v = reinterpret_cast<float *>(&p);
+#pragma unroll
+    for (int f = 0; f < GM_NPRE; ++f) v[f] = pre[f * stride + pix];
+    return p;
+}
 
 __device__ __forceinline__ PixPre make_pre(const double *m)
 {
@@ -353,9 +377,9 @@ __device__ __forceinline__ int staged_slot(int p)
 /* Synthesis + response functions + PSF + normal equations + chi^2 for a list
  * of pixels; wavelength-parallel mapping.                                   */
 template <int LPP, int NLPL, bool MIXED>
-__global__ __launch_bounds__(64) void k_rf(
+__global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
     const gm_dsetup *__restrict__ S, const double *__restrict__ models,
-    int64_t mfs, int64_t mps, const float *__restrict__ obs,
+    const float *__restrict__ pre, int64_t mfs, int64_t mps, const float *__restrict__ obs,
     const int32_t *__restrict__ list, const int32_t *__restrict__ count,
     int64_t n_direct, float *__restrict__ alpha, float *__restrict__ beta,
     int64_t aes, int64_t aps, int64_t bes, int64_t bps, double *__restrict__ chi_out,
@@ -369,7 +393,7 @@ __global__ __launch_bounds__(64) void k_rf(
     const int k0 = lane % LPP;
     const int n = S->nlambda;
     const int64_t cnt = count ? (int64_t)(*count) : n_direct;
-    const int region4 = max(rows * 9, LPP * (RSTR / 4));
+    const int region4 = max(rows * 9, (LPP / 2) * (RSTR / 4));
     float4 *conv = lds4 + h * region4;
     float *red = reinterpret_cast<float *>(conv);
     const int use_psf = S->use_psf;
@@ -382,9 +406,14 @@ __global__ __launch_bounds__(64) void k_rf(
         const int64_t item = base + h;
         const bool valid = item < cnt;
         const int64_t pix = list ? (int64_t)list[valid ? item : base] : (valid ? item : base);
-        double m[GM_NMODEL];
-        load_model(models, mfs, mps, pix, m);
-        const PixPre pp = make_pre(m);
+        PixPre pp;
+        if (pre) {
+            pp = load_pre(pre, mfs, pix);
+        } else {
+            double m[GM_NMODEL];
+            load_model(models, mfs, mps, pix, m);
+            pp = make_pre(m);
+        }
 
         /* zero padding rows (the reduction buffer aliases them) */
         for (int r = k0; r < rows; r += LPP)
@@ -498,18 +527,25 @@ __global__ __launch_bounds__(64) void k_rf(
                 }
             }
         }
-        __syncthreads();
-        /* cross-lane reduction of the 55 partial sums through LDS */
-        float4 *rrow = reinterpret_cast<float4 *>(red + k0 * RSTR);
+        /* cross-lane reduction of the 55 partial sums: fold the upper half of
+         * the pixel's lanes onto the lower half with lane swaps, then sum the
+         * HL remaining rows through LDS (aliasing the staged rows) */
+        constexpr int HL = LPP / 2;
 #pragma unroll
-        for (int j = 0; j < 14; ++j)
-            rrow[j] = make_float4(acc[4 * j], acc[4 * j + 1], acc[4 * j + 2],
-                                  (4 * j + 3 < GM_RED_N) ? acc[4 * j + 3] : 0.f);
+        for (int e = 0; e < GM_RED_N; ++e) acc[e] += __shfl_xor(acc[e], HL, 64);
+        __syncthreads();
+        if (k0 < HL) {
+            float4 *rrow = reinterpret_cast<float4 *>(red + k0 * RSTR);
+#pragma unroll
+            for (int j = 0; j < 14; ++j)
+                rrow[j] = make_float4(acc[4 * j], acc[4 * j + 1], acc[4 * j + 2],
+                                      (4 * j + 3 < GM_RED_N) ? acc[4 * j + 3] : 0.f);
+        }
         __syncthreads();
         for (int e = k0; e < GM_RED_N - 1; e += LPP) {
             float sum = 0.f;
 #pragma unroll 8
-            for (int l = 0; l < LPP; ++l) sum += red[l * RSTR + e];
+            for (int l = 0; l < HL; ++l) sum += red[l * RSTR + e];
             if (valid) {
                 if (e < 45) {
                     if (alpha) alpha[e * aes + pix * aps] = sum;
@@ -535,7 +571,7 @@ __global__ __launch_bounds__(64) void k_rf(
 template <int LPP, int NLPL, bool MIXED>
 __global__ __launch_bounds__(64) void k_synth(
     const gm_dsetup *__restrict__ S, const double *__restrict__ models,
-    int64_t mfs, int64_t mps, const float *__restrict__ obs,
+    const float *__restrict__ pre, int64_t mfs, int64_t mps, const float *__restrict__ obs,
     const int32_t *__restrict__ list, const int32_t *__restrict__ count,
     int64_t n_direct, double *__restrict__ chi_out, float *__restrict__ synth_out,
     int rows, int padlo)
@@ -560,9 +596,14 @@ __global__ __launch_bounds__(64) void k_synth(
         const int64_t item = base + h;
         const bool valid = item < cnt;
         const int64_t pix = list ? (int64_t)list[valid ? item : base] : (valid ? item : base);
-        double m[GM_NMODEL];
-        load_model(models, mfs, mps, pix, m);
-        const PixPre pp = make_pre(m);
+        PixPre pp;
+        if (pre) {
+            pp = load_pre(pre, mfs, pix);
+        } else {
+            double m[GM_NMODEL];
+            load_model(models, mfs, mps, pix, m);
+            pp = make_pre(m);
+        }
 #pragma unroll
         for (int s = 0; s < NLPL; ++s) {
             const int k = k0 + s * LPP;
@@ -852,6 +893,7 @@ __device__ __forceinline__ void apply_to_trial(const gm_dsetup *__restrict__ S, 
     apply_delta_check<T>(S, best, x, m);
 #pragma unroll
     for (int f = 0; f < GM_NMODEL; ++f) st.trial[f * cap + pix] = m[f];
+    store_pre(st.pre_trial, cap, pix, make_pre(m));
 }
 
 /* Step solve for the active list.  SOLVER = GM_SOLVE_SVD: LDL^T where it
@@ -984,6 +1026,9 @@ __global__ __launch_bounds__(1024) void k_decide(const gm_dsetup *__restrict__ S
                 flambda = flambda / (T(5.0) * factor);
 #pragma unroll
                 for (int f = 0; f < GM_NMODEL; ++f) st.best[f * cap + pix] = st.trial[f * cap + pix];
+#pragma unroll
+                for (int f = 0; f < GM_NPRE; ++f)
+                    st.pre_best[f * cap + pix] = st.pre_trial[f * cap + pix];
                 ochisqr = chisqr;
             } else {
                 flambda = flambda * T(10.0) * factor;
@@ -1171,6 +1216,7 @@ __global__ __launch_bounds__(64) void k_init_state(const gm_dsetup *__restrict__
             st.best[f * cap + q] = m[f];
             st.trial[f * cap + q] = m[f];
         }
+        store_pre(st.pre_best, cap, q, make_pre(m));
         st.flambda[q] = S->ilambda;
         st.factor[q] = 1.0;
         st.iter[q] = 0;
@@ -1268,38 +1314,38 @@ constexpr int kMaxBlocks = 8192;
 
 template <int LPP, int NLPL>
 void launch_rf_t(const gm_lanecfg &lc, const gm_dsetup *dS, const double *models,
-                 int64_t mfs, int64_t mps, const float *obs, const int32_t *list,
+                 const float *pre, int64_t mfs, int64_t mps, const float *obs, const int32_t *list,
                  const int32_t *count, int64_t n_direct, int64_t grid_items, float *alpha,
                  float *beta, int64_t aes, int64_t aps, int64_t bes, int64_t bps, double *chi,
                  float *synth, float *rf, int mixed, hipStream_t s)
 {
-    const int region4 = (lc.rows * 9 > LPP * 15) ? lc.rows * 9 : LPP * 15;
+    const int region4 = (lc.rows * 9 > (LPP / 2) * 15) ? lc.rows * 9 : (LPP / 2) * 15;
     const size_t shm = (size_t)lc.ppw * region4 * sizeof(float4);
     const int g = grid_for(grid_items, lc.ppw, kMaxBlocks);
     if (mixed)
-        hipLaunchKernelGGL((k_rf<LPP, NLPL, true>), dim3(g), dim3(64), shm, s, dS, models, mfs,
+        hipLaunchKernelGGL((k_rf<LPP, NLPL, true>), dim3(g), dim3(64), shm, s, dS, models, pre, mfs,
                            mps, obs, list, count, n_direct, alpha, beta, aes, aps, bes, bps, chi, synth, rf,
                            lc.rows, lc.padlo);
     else
-        hipLaunchKernelGGL((k_rf<LPP, NLPL, false>), dim3(g), dim3(64), shm, s, dS, models, mfs,
+        hipLaunchKernelGGL((k_rf<LPP, NLPL, false>), dim3(g), dim3(64), shm, s, dS, models, pre, mfs,
                            mps, obs, list, count, n_direct, alpha, beta, aes, aps, bes, bps, chi, synth, rf,
                            lc.rows, lc.padlo);
 }
 
 template <int LPP, int NLPL>
 void launch_synth_t(const gm_lanecfg &lc, const gm_dsetup *dS, const double *models,
-                    int64_t mfs, int64_t mps, const float *obs, const int32_t *list,
+                    const float *pre, int64_t mfs, int64_t mps, const float *obs, const int32_t *list,
                     const int32_t *count, int64_t n_direct, int64_t grid_items, double *chi,
                     float *synth, int mixed, hipStream_t s)
 {
     const size_t shm = (size_t)lc.ppw * lc.rows * sizeof(float4);
     const int g = grid_for(grid_items, lc.ppw, kMaxBlocks);
     if (mixed)
-        hipLaunchKernelGGL((k_synth<LPP, NLPL, true>), dim3(g), dim3(64), shm, s, dS, models, mfs,
+        hipLaunchKernelGGL((k_synth<LPP, NLPL, true>), dim3(g), dim3(64), shm, s, dS, models, pre, mfs,
                            mps, obs, list, count, n_direct, chi, synth, lc.rows, lc.padlo);
     else
         hipLaunchKernelGGL((k_synth<LPP, NLPL, false>), dim3(g), dim3(64), shm, s, dS, models,
-                           mfs, mps, obs, list, count, n_direct, chi, synth, lc.rows, lc.padlo);
+                           pre, mfs, mps, obs, list, count, n_direct, chi, synth, lc.rows, lc.padlo);
 }
 
 } // namespace
@@ -1313,7 +1359,7 @@ static void split_strides(int64_t model_stride, int64_t &mfs, int64_t &mps)
 }
 
 void gm_launch_rf(const gm_lanecfg &lc, const gm_dsetup *dS, const double *models,
-                  int64_t model_stride, const float *obs, const int32_t *list,
+                  const float *pre, int64_t model_stride, const float *obs, const int32_t *list,
                   const int32_t *count, int64_t n_direct, int64_t grid_items, float *alpha,
                   float *beta, int64_t ab_stride, double *chi_out, float *synth_out,
                   float *rf_out, int mixed, hipStream_t st)
@@ -1327,37 +1373,37 @@ void gm_launch_rf(const gm_lanecfg &lc, const gm_dsetup *dS, const double *model
     const int64_t bes = ab_stride > 0 ? ab_stride : 1;
     const int64_t bps = ab_stride > 0 ? 1 : 9;
     if (lc.lpp == 32 && lc.nlpl == 1)
-        launch_rf_t<32, 1>(lc, dS, models, mfs, mps, obs, list, count, n_direct, grid_items, alpha,
+        launch_rf_t<32, 1>(lc, dS, models, pre, mfs, mps, obs, list, count, n_direct, grid_items, alpha,
                            beta, aes, aps, bes, bps, chi_out, synth_out, rf_out, mixed, st);
     else if (lc.lpp == 64 && lc.nlpl == 1)
-        launch_rf_t<64, 1>(lc, dS, models, mfs, mps, obs, list, count, n_direct, grid_items, alpha,
+        launch_rf_t<64, 1>(lc, dS, models, pre, mfs, mps, obs, list, count, n_direct, grid_items, alpha,
                            beta, aes, aps, bes, bps, chi_out, synth_out, rf_out, mixed, st);
     else if (lc.lpp == 64 && lc.nlpl == 2)
-        launch_rf_t<64, 2>(lc, dS, models, mfs, mps, obs, list, count, n_direct, grid_items, alpha,
+        launch_rf_t<64, 2>(lc, dS, models, pre, mfs, mps, obs, list, count, n_direct, grid_items, alpha,
                            beta, aes, aps, bes, bps, chi_out, synth_out, rf_out, mixed, st);
     else
-        launch_rf_t<64, 4>(lc, dS, models, mfs, mps, obs, list, count, n_direct, grid_items, alpha,
+        launch_rf_t<64, 4>(lc, dS, models, pre, mfs, mps, obs, list, count, n_direct, grid_items, alpha,
                            beta, aes, aps, bes, bps, chi_out, synth_out, rf_out, mixed, st);
 }
 
 void gm_launch_synth(const gm_lanecfg &lc, const gm_dsetup *dS, const double *models,
-                     int64_t model_stride, const float *obs, const int32_t *list,
+                     const float *pre, int64_t model_stride, const float *obs, const int32_t *list,
                      const int32_t *count, int64_t n_direct, int64_t grid_items,
                      double *chi_out, float *synth_out, int mixed, hipStream_t st)
 {
     int64_t mfs, mps;
     split_strides(model_stride, mfs, mps);
     if (lc.lpp == 32 && lc.nlpl == 1)
-        launch_synth_t<32, 1>(lc, dS, models, mfs, mps, obs, list, count, n_direct, grid_items,
+        launch_synth_t<32, 1>(lc, dS, models, pre, mfs, mps, obs, list, count, n_direct, grid_items,
                               chi_out, synth_out, mixed, st);
     else if (lc.lpp == 64 && lc.nlpl == 1)
-        launch_synth_t<64, 1>(lc, dS, models, mfs, mps, obs, list, count, n_direct, grid_items,
+        launch_synth_t<64, 1>(lc, dS, models, pre, mfs, mps, obs, list, count, n_direct, grid_items,
                               chi_out, synth_out, mixed, st);
     else if (lc.lpp == 64 && lc.nlpl == 2)
-        launch_synth_t<64, 2>(lc, dS, models, mfs, mps, obs, list, count, n_direct, grid_items,
+        launch_synth_t<64, 2>(lc, dS, models, pre, mfs, mps, obs, list, count, n_direct, grid_items,
                               chi_out, synth_out, mixed, st);
     else
-        launch_synth_t<64, 4>(lc, dS, models, mfs, mps, obs, list, count, n_direct, grid_items,
+        launch_synth_t<64, 4>(lc, dS, models, pre, mfs, mps, obs, list, count, n_direct, grid_items,
                               chi_out, synth_out, mixed, st);
 }
 

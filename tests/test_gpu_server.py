@@ -97,7 +97,7 @@ def test_server_results_same_alone_and_in_mix(gm, cuda):
 def test_server_budget_admission(gm, cuda):
     setup = _setups(gm)[0]
     ctx = gm.Context(setup)
-    per_px = 8 * 27 + 4 * 54 + 4 * 9 + 2 * 4 * 4 * 30        # bytes per pixel slot
+    per_px = 8 * 27 + 4 * (54 + 38) + 4 * 9 + 2 * 4 * 4 * 30  # bytes per pixel slot
     budget = 6000 * per_px
     srv = gm.Server(hbm_budget_bytes=budget, pool_pixels=1 << 15)
     sid = srv.add_setup(setup)
