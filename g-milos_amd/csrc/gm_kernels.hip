@@ -35,35 +35,53 @@ __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x)
 
 /* ------------------------------------------------------------------------ */
 /* Voigt and Faraday-Voigt functions, FP32: Hui, Armstrong & Wray (1978)
- * rational approximation w(z) ~ P(Z)/Q(Z), Z = a - i u, evaluated by complex
- * Horner and a scaled complex division (no |Q|^2 overflow for large |u|).   */
+ * rational approximation w(z) ~ P(Z)/Q(Z), Z = a - i u.  The real-coefficient
+ * polynomials are evaluated at the complex point with Knuth's second-order
+ * recurrence (TAOCP vol. 2, 4.6.4): with r = 2 Re Z and s = |Z|^2,
+ *   A <- r A + B,  B <- c_j - s A_old,   p(Z) = Z A + B,
+ * two FMAs per coefficient instead of a complex multiply-add; then a scaled
+ * complex division (no |Q|^2 overflow for large |u|).                      */
+constexpr double kHuiP[7] = {122.607931777104326, 214.382388694706425, 181.928533092181549,
+                             93.155580458138441,  30.180142196210589,  5.912626209773153,
+                             0.564189583562615};
+constexpr double kHuiQ[8] = {122.60793177387535, 352.730625110963558, 457.334478783897737,
+                             348.703917719495792, 170.354001821091472, 53.992906912940207,
+                             10.479857114260399, 1.0};
+constexpr double kRSPI = 0.56418958354775628695;   /* 1/sqrt(pi) */
+
+template <int N>
+__device__ __forceinline__ void poly_knuth(const double (&c)[N + 1], float x, float y, float r,
+                                           float s, float &re, float &im)
+{
+    float A = (float)c[N], B = (float)c[N - 1];
+#pragma unroll
+    for (int j = N - 2; j >= 0; --j) {
+        const float t = A;
+        A = fmaf(r, A, B);
+        B = fmaf(-s, t, (float)c[j]);
+    }
+    re = fmaf(x, A, B);
+    im = y * A;
+}
+
+/* R(Z) = Q(Z)/sqrt(pi) - Z P(Z), coefficients formed in double. */
+struct HuiR {
+    double c[8];
+    constexpr HuiR() : c{}
+    {
+        c[0] = kHuiQ[0] * kRSPI;
+        for (int k = 1; k < 8; ++k) c[k] = kHuiQ[k] * kRSPI - kHuiP[k - 1];
+    }
+};
+constexpr HuiR kHuiR{};
+
 __device__ __forceinline__ void voigt32(float a, float u, float &H, float &F)
 {
-    constexpr float P[7] = {122.607931777104326f, 214.382388694706425f,
-                            181.928533092181549f, 93.155580458138441f,
-                            30.180142196210589f, 5.912626209773153f,
-                            0.564189583562615f};
-    constexpr float Q[8] = {122.60793177387535f, 352.730625110963558f,
-                            457.334478783897737f, 348.703917719495792f,
-                            170.354001821091472f, 53.992906912940207f,
-                            10.479857114260399f, 1.0f};
-    const float zr = a, zi = -u;
-    float pr = P[6], pi = 0.f;
-#pragma unroll
-    for (int k = 5; k >= 0; --k) {
-        const float tr = pr * zr - pi * zi + P[k];
-        const float ti = pr * zi + pi * zr;
-        pr = tr;
-        pi = ti;
-    }
-    float qr = Q[7], qi = 0.f;
-#pragma unroll
-    for (int k = 6; k >= 0; --k) {
-        const float tr = qr * zr - qi * zi + Q[k];
-        const float ti = qr * zi + qi * zr;
-        qr = tr;
-        qi = ti;
-    }
+    const float x = a, y = -u;
+    const float r = 2.f * a, s2 = fmaf(a, a, u * u);
+    float pr, pi, qr, qi;
+    poly_knuth<6>(kHuiP, x, y, r, s2, pr, pi);
+    poly_knuth<7>(kHuiQ, x, y, r, s2, qr, qi);
     const float s = frcp(fmaxf(fabsf(qr), fabsf(qi)));
     const float qrs = qr * s, qis = qi * s;
     const float inv = frcp(qrs * qrs + qis * qis) * s;
@@ -76,51 +94,19 @@ __device__ __forceinline__ void voigt32(float a, float u, float &H, float &F)
  * (me_der.cu) applied to the rational W = P/Q, i.e. 2 Im(Z W) and
  * 1/sqrt(pi) - Re(Z W).  Since Z P/Q = 1/sqrt(pi) - R/Q with
  * R(Z) = Q(Z)/sqrt(pi) - Z P(Z), they equal -2 Im(R/Q) and Re(R/Q) exactly;
- * evaluating R (coefficients formed in double) avoids the cancellation of
- * the two terms in the line wings, where |u|^2 times the FP32 rounding of H
- * and F would otherwise dominate.  (The reference's RR constant differs from
- * 1/sqrt(pi) by 5e-11, below FP32 resolution of every use.)                */
+ * evaluating R avoids the cancellation of the two terms in the line wings,
+ * where |u|^2 times the FP32 rounding of H and F would otherwise dominate.
+ * (The reference's RR constant differs from 1/sqrt(pi) by 5e-11, below FP32
+ * resolution of every use.)                                                */
 __device__ __forceinline__ void voigt32_d(float a, float u, float &H, float &F, float &dH,
                                           float &dF)
 {
-    constexpr double P[7] = {122.607931777104326, 214.382388694706425, 181.928533092181549,
-                             93.155580458138441, 30.180142196210589, 5.912626209773153,
-                             0.564189583562615};
-    constexpr double Q[8] = {122.60793177387535, 352.730625110963558, 457.334478783897737,
-                             348.703917719495792, 170.354001821091472, 53.992906912940207,
-                             10.479857114260399, 1.0};
-    constexpr double RSPI = 0.56418958354775628695;
-    const float zr = a, zi = -u;
-    float pr = (float)P[6], pi = 0.f;
-#pragma unroll
-    for (int k = 5; k >= 0; --k) {
-        const float tr = pr * zr - pi * zi + (float)P[k];
-        const float ti = pr * zi + pi * zr;
-        pr = tr;
-        pi = ti;
-    }
-    float qr = (float)Q[7], qi = 0.f;
-#pragma unroll
-    for (int k = 6; k >= 0; --k) {
-        const float tr = qr * zr - qi * zi + (float)Q[k];
-        const float ti = qr * zi + qi * zr;
-        qr = tr;
-        qi = ti;
-    }
-    float rr = (float)(Q[7] * RSPI - P[6]), ri = 0.f;
-#pragma unroll
-    for (int k = 6; k >= 1; --k) {
-        const float tr = rr * zr - ri * zi + (float)(Q[k] * RSPI - P[k - 1]);
-        const float ti = rr * zi + ri * zr;
-        rr = tr;
-        ri = ti;
-    }
-    {
-        const float tr = rr * zr - ri * zi + (float)(Q[0] * RSPI);
-        const float ti = rr * zi + ri * zr;
-        rr = tr;
-        ri = ti;
-    }
+    const float x = a, y = -u;
+    const float r = 2.f * a, s2 = fmaf(a, a, u * u);
+    float pr, pi, qr, qi, rr, ri;
+    poly_knuth<6>(kHuiP, x, y, r, s2, pr, pi);
+    poly_knuth<7>(kHuiQ, x, y, r, s2, qr, qi);
+    poly_knuth<7>(kHuiR.c, x, y, r, s2, rr, ri);
     const float s = frcp(fmaxf(fabsf(qr), fabsf(qi)));
     const float qrs = qr * s, qis = qi * s;
     const float inv = frcp(qrs * qrs + qis * qis) * s;
