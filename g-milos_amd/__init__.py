@@ -77,6 +77,7 @@ class gm_out(ctypes.Structure):
 class gm_stats(ctypes.Structure):
     _fields_ = [("npix", ctypes.c_int64), ("synth_evals", ctypes.c_int64),
                 ("rf_evals", ctypes.c_int64), ("solves", ctypes.c_int64),
+                ("svd_solves", ctypes.c_int64),
                 ("lm_rounds", ctypes.c_int32), ("psf_taps", ctypes.c_int32),
                 ("ms_total", ctypes.c_float), ("ms_synth", ctypes.c_float),
                 ("ms_rf", ctypes.c_float), ("ms_solve", ctypes.c_float),

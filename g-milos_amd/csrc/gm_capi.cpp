@@ -413,6 +413,7 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
         const int32_t *hc = c->h_counts + 3 * r;
         if (hc[0] > 0) rounds = r + 1;
         c->stats.solves += hc[0];
+        c->stats.svd_solves += hc[1];
         c->stats.synth_evals += hc[0];
         c->stats.rf_evals += hc[2];
     }

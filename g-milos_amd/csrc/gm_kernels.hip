@@ -517,6 +517,7 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
          * the pixel's lanes onto the lower half with lane swaps, then sum the
          * HL remaining rows through LDS (aliasing the staged rows) */
         constexpr int HL = LPP / 2;
+        const float chi_part = acc[54];          /* FP32 chi^2 term of this lane */
 #pragma unroll
         for (int e = 0; e < GM_RED_N; ++e) acc[e] += __shfl_xor(acc[e], HL, 64);
         __syncthreads();
@@ -543,8 +544,7 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
         if (MIXED) {
             chi = group_sum<double, LPP>(chi);
         } else {
-            float c = acc[54];
-            c = group_sum<float, LPP>(c);
+            const float c = group_sum<float, LPP>(chi_part);
             chi = (double)(c / (float)nfree);
         }
         if (MIXED) chi = chi / (double)nfree;
@@ -901,6 +901,7 @@ __global__ __launch_bounds__(64) void k_solve(const gm_dsetup *__restrict__ S,
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (hist) {
             hist[0] = *count;
+            hist[1] = 0;                     /* Jacobi fallbacks, set by k_solve_svd */
             if (reset1) hist[-1] = *reset1;
         }
         if (reset0) *reset0 = 0;
@@ -931,10 +932,12 @@ __global__ __launch_bounds__(64) void k_solve(const gm_dsetup *__restrict__ S,
 template <bool MIXED>
 __global__ __launch_bounds__(64) void k_solve_svd(const gm_dsetup *__restrict__ S, gm_state st,
                                                   const int32_t *__restrict__ fb_list,
-                                                  const int32_t *__restrict__ fb_count)
+                                                  const int32_t *__restrict__ fb_count,
+                                                  int32_t *__restrict__ hist)
 {
     using T = typename std::conditional<MIXED, double, float>::type;
     const int64_t cnt = *fb_count;
+    if (hist && blockIdx.x == 0 && threadIdx.x == 0) hist[1] = (int32_t)cnt;
     const int64_t cap = st.cap;
     for (int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x; i < cnt;
          i += (int64_t)gridDim.x * 64) {
@@ -1415,9 +1418,11 @@ void gm_launch_solve(const gm_dsetup *dS, gm_state st, const int32_t *list,
     if (solver == GM_SOLVE_SVD) {
         const int gf = grid_for(grid_items, 64, 1024);
         if (mixed)
-            hipLaunchKernelGGL((k_solve_svd<true>), dim3(gf), dim3(64), 0, s, dS, st, fb_list, fb_count);
+            hipLaunchKernelGGL((k_solve_svd<true>), dim3(gf), dim3(64), 0, s, dS, st, fb_list, fb_count,
+                               hist);
         else
-            hipLaunchKernelGGL((k_solve_svd<false>), dim3(gf), dim3(64), 0, s, dS, st, fb_list, fb_count);
+            hipLaunchKernelGGL((k_solve_svd<false>), dim3(gf), dim3(64), 0, s, dS, st, fb_list,
+                               fb_count, hist);
     }
 }
 

@@ -119,6 +119,7 @@ typedef struct {
     int64_t synth_evals;             /* pixel-evaluations of synth+chi^2        */
     int64_t rf_evals;                /* pixel-evaluations of RF+normal eqs       */
     int64_t solves;                  /* pixel step solves                        */
+    int64_t svd_solves;              /* ... of which by the Jacobi SVD           */
     int32_t lm_rounds;               /* LM rounds executed                       */
     int32_t psf_taps;                /* PSF taps applied per sample              */
     float   ms_total;                /* wall time of the device work (events)    */

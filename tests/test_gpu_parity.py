@@ -105,8 +105,11 @@ def test_response_functions_match_oracle(gm, orc, cuda, kind, psfname, rules):
     assert np.all(rf[:, :, 9, :] == 0)
 
 
-def test_normal_equations_match_oracle(gm, orc, cuda):
-    p, o = pair(gm, orc, "single", ("gauss", 60.0))
+@pytest.mark.parametrize("precision,free", [(1, None), (0, None),
+                                            (1, ("eta0", "vlos", "dopp", "aa", "az", "S1"))])
+def test_normal_equations_match_oracle(gm, orc, cuda, precision, free):
+    kw = {"free": free} if free else {}
+    p, o = pair(gm, orc, "single", ("gauss", 60.0), precision=precision, **kw)
     truth = gm.random_models(128, seed=13)
     obs = observations(orc, o, truth, seed=13)
     models = gm.random_models(128, seed=14)       # away from the optimum
@@ -119,6 +122,10 @@ def test_normal_equations_match_oracle(gm, orc, cuda):
         A9 = gm.unpack_alpha(a[i])
         assert np.abs(A9 - A[:9, :9]).max() <= 1e-3 * np.abs(A).max()
         assert np.abs(b[i] - B[:9]).max() <= 2e-3 * np.abs(B).max()
+        if free:                               # fixed rows / columns exactly zero
+            fixed = [k for k, f in enumerate(gm.FREE9) if f not in free]
+            assert np.all(A9[fixed, :] == 0) and np.all(A9[:, fixed] == 0)
+            assert np.all(b[i][fixed] == 0)
         assert c[i] == pytest.approx(orc.chisqr(o, obs[i], sp[i]), rel=2e-5)
 
 
