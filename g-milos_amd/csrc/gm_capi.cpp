@@ -275,7 +275,7 @@ static gm_status ensure_state(gm_ctx *c, int64_t npix)
     int64_t cap = ((npix + 255) / 256) * 256;
     const size_t per = sizeof(double) * (2 * GM_NMODEL + 5) +
                        sizeof(float) * (GM_ALPHA_N + 9 + 2 * GM_NPRE) + sizeof(int32_t) * (2 + 4);
-    const size_t bytes = per * (size_t)cap + 256 * 20 + 64 + sizeof(int32_t) * 3 * (kMaxRounds + 2) + 256;
+    const size_t bytes = per * (size_t)cap + 256 * 22 + 64 + sizeof(int32_t) * 3 * (kMaxRounds + 2) + 256;
     if (c->ws) {
         hipDeviceSynchronize();
         hipFree(c->ws);
@@ -358,7 +358,8 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
 
     GM_HIP(hipEventRecord(c->ev_start, s));
     kbegin(3);
-    gm_launch_init_state(c->dS, st, obs, init, c->setup.init_mode, 0, npix, st.list[0], s);
+    gm_launch_init_state(c->dS, st, obs, init, c->setup.init_mode, c->setup.nlambda, 0, npix,
+                         st.list[0], s);
     gm_launch_set_count(&cnt[0], (int32_t)npix, s);
     gm_launch_set_count(&cnt[3], 0, s);
     kend();
@@ -382,8 +383,8 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
                         &cnt[2], hist + 3 * r, st.list[3], &cnt[3], s);
         kend();
         kbegin(0);
-        gm_launch_synth(lc, c->dS, st.trial, st.pre_trial, st.cap, obs, st.list[cur], &cnt[cur], 0, npix,
-                        st.chi_trial, out->synth, mixed, s);
+        gm_launch_synth(lc, c->dS, st.trial, st.pre_trial, st.cap, obs, st.list[cur], &cnt[cur], 0,
+                        npix, st.chi_trial, out->synth, mixed, s);
         kend();
         kbegin(3);
         gm_launch_decide(c->dS, st, st.list[cur], &cnt[cur], npix, st.list[nxt], &cnt[nxt],

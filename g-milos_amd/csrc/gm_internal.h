@@ -119,8 +119,8 @@ void gm_launch_decide(const gm_dsetup *dS, gm_state st, const int32_t *list,
                       int32_t *reset_fb, int32_t *done_list, int32_t *done_count, int mixed,
                       hipStream_t stream);
 void gm_launch_init_state(const gm_dsetup *dS, gm_state st, const float *obs,
-                          const double *init, int init_mode, int64_t slot0, int64_t npix,
-                          int32_t *list0, hipStream_t stream);
+                          const double *init, int init_mode, int nlambda, int64_t slot0,
+                          int64_t npix, int32_t *list0, hipStream_t stream);
 void gm_launch_init_chi(gm_state st, int64_t slot0, int64_t npix, hipStream_t stream);
 void gm_launch_append_range(int32_t *list, int32_t *count, int64_t slot0, int64_t n,
                             hipStream_t stream);

@@ -1179,38 +1179,52 @@ __global__ __launch_bounds__(64) void k_ce(const gm_dsetup *__restrict__ S,
 /* LM start state of state slots slot0 .. slot0+npix-1 (milosUtils.cu:
  * 595-643): start model (fixed, classical estimates or per pixel), lambda,
  * counters, and the slot list for the first normal-equation evaluation.
- * obs is indexed by slot, init by p.                                      */
+ * obs is indexed by slot, init by p.  For the classical estimates the
+ * block's ppb consecutive spectra are first copied to LDS with coalesced
+ * loads (row stride 4 nlambda + 1 floats against bank conflicts).          */
 __global__ __launch_bounds__(64) void k_init_state(const gm_dsetup *__restrict__ S, gm_state st,
                                                    const float *__restrict__ obs,
                                                    const double *__restrict__ init,
                                                    int init_mode, int64_t slot0, int64_t npix,
-                                                   int32_t *__restrict__ list0)
+                                                   int32_t *__restrict__ list0, int ppb)
 {
+    extern __shared__ float spec[];
     const int n = S->nlambda;
+    const int n4 = 4 * n, stride = 4 * n + 1;
     const int64_t cap = st.cap;
-    for (int64_t p = (int64_t)blockIdx.x * 64 + threadIdx.x; p < npix;
-         p += (int64_t)gridDim.x * 64) {
-        const int64_t q = slot0 + p;
-        double m[GM_NMODEL];
-        if (init_mode == GM_INIT_PER_PIXEL) {
-#pragma unroll
-            for (int f = 0; f < GM_NMODEL; ++f) m[f] = init[p * GM_NMODEL + f];
-        } else {
-#pragma unroll
-            for (int f = 0; f < GM_NMODEL; ++f) m[f] = S->init_model[f];
-            if (init_mode == GM_INIT_CE) classical_estimates(S, obs + q * 4 * n, m, 1);
+    const int tid = threadIdx.x;
+    for (int64_t base = (int64_t)blockIdx.x * ppb; base < npix; base += (int64_t)gridDim.x * ppb) {
+        const int nb = (int)((npix - base < ppb) ? npix - base : ppb);
+        if (init_mode == GM_INIT_CE) {
+            const float *src = obs + (slot0 + base) * n4;
+            for (int i = tid; i < nb * n4; i += 64) spec[(i / n4) * stride + (i % n4)] = src[i];
+            __syncthreads();
         }
+        if (tid < nb) {
+            const int64_t p = base + tid;
+            const int64_t q = slot0 + p;
+            double m[GM_NMODEL];
+            if (init_mode == GM_INIT_PER_PIXEL) {
 #pragma unroll
-        for (int f = 0; f < GM_NMODEL; ++f) {
-            st.best[f * cap + q] = m[f];
-            st.trial[f * cap + q] = m[f];
+                for (int f = 0; f < GM_NMODEL; ++f) m[f] = init[p * GM_NMODEL + f];
+            } else {
+#pragma unroll
+                for (int f = 0; f < GM_NMODEL; ++f) m[f] = S->init_model[f];
+                if (init_mode == GM_INIT_CE) classical_estimates(S, spec + tid * stride, m, 1);
+            }
+#pragma unroll
+            for (int f = 0; f < GM_NMODEL; ++f) {
+                st.best[f * cap + q] = m[f];
+                st.trial[f * cap + q] = m[f];
+            }
+            store_pre(st.pre_best, cap, q, make_pre(m));
+            st.flambda[q] = S->ilambda;
+            st.factor[q] = 1.0;
+            st.iter[q] = 0;
+            st.nacc[q] = 0;
+            list0[p] = (int32_t)q;
         }
-        store_pre(st.pre_best, cap, q, make_pre(m));
-        st.flambda[q] = S->ilambda;
-        st.factor[q] = 1.0;
-        st.iter[q] = 0;
-        st.nacc[q] = 0;
-        list0[p] = (int32_t)q;
+        if (init_mode == GM_INIT_CE) __syncthreads();
     }
 }
 
@@ -1462,12 +1476,17 @@ void gm_launch_decide(const gm_dsetup *dS, gm_state st, const int32_t *list,
 }
 
 void gm_launch_init_state(const gm_dsetup *dS, gm_state st, const float *obs,
-                          const double *init, int init_mode, int64_t slot0, int64_t npix,
-                          int32_t *list0, hipStream_t s)
+                          const double *init, int init_mode, int nlambda, int64_t slot0,
+                          int64_t npix, int32_t *list0, hipStream_t s)
 {
-    const int g = grid_for(npix, 64, kMaxBlocks);
-    hipLaunchKernelGGL(k_init_state, dim3(g), dim3(64), 0, s, dS, st, obs, init, init_mode, slot0,
-                       npix, list0);
+    /* up to 64 pixels per block, at most 32 KB of staged spectra */
+    const int stride = 4 * nlambda + 1;
+    int ppb = 8192 / stride;
+    ppb = ppb < 1 ? 1 : (ppb > 64 ? 64 : ppb);
+    const size_t shm = (init_mode == GM_INIT_CE) ? (size_t)ppb * stride * sizeof(float) : 0;
+    const int g = grid_for(npix, ppb, kMaxBlocks);
+    hipLaunchKernelGGL(k_init_state, dim3(g), dim3(64), shm, s, dS, st, obs, init, init_mode, slot0,
+                       npix, list0, ppb);
 }
 
 void gm_launch_init_chi(gm_state st, int64_t slot0, int64_t npix, hipStream_t s)

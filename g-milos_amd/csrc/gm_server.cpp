@@ -274,7 +274,8 @@ static void admit(Engine &e, const Pending &pd, int t, int64_t slot0)
     (void)hipMemcpyAsync(e.obs + slot0 * nl4, pd.obs, sizeof(float) * nl4 * pd.npix,
                          hipMemcpyDeviceToDevice, s);
     gm_launch_fill_i32(e.slot_tile + slot0, pd.npix, (int32_t)t, s);
-    gm_launch_init_state(e.dS, e.st, e.obs, pd.init, e.init_mode, slot0, pd.npix, e.newlist, s);
+    gm_launch_init_state(e.dS, e.st, e.obs, pd.init, e.init_mode, e.nl, slot0, pd.npix, e.newlist,
+                         s);
     gm_launch_set_count(&e.cnt[4], (int32_t)pd.npix, s);
     gm_launch_rf(e.lc, e.dS, e.st.best, e.st.pre_best, e.cap, e.obs, e.newlist, &e.cnt[4], 0, pd.npix, e.st.alpha,
                  e.st.beta, e.cap, e.st.ochisqr, e.synth, nullptr, e.mixed, s);
