@@ -273,6 +273,55 @@ def guided_leg(gm, torch, dev, npix, n_train, steps, seed, oracle_sample):
     return res
 
 
+def mixed_precision_leg(gm, torch, dev, npix, seed, oracle_sample):
+    """Two lines (Fe I 6301.5 + 6302.5) on 112 wavelengths with the PSF:
+    mixed precision (FP64 solve, chi^2 and accept test) and plain FP32 on the
+    same cube, each against the FP64 oracle per parameter on a sample.  10 %
+    of the pixels weak-field (B < 100 G), 5 % with noise 3e-3 Ic."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    mixed = gm.Context(gm.setup_two_lines(psf_fwhm=60.0, precision=gm.GM_PREC_MIXED))
+    fp32 = gm.Context(gm.setup_two_lines(psf_fwhm=60.0, precision=gm.GM_PREC_FP32))
+    truth = gm.draw_models(npix, seed=seed, prior=gm.default_prior(weak_fraction=0.1))
+    obs = mixed.synth(truth)
+    gm.add_noise(truth, obs, 1e-3, seed=seed + 1)
+    nh = npix // 20                       # first 5 %: total sigma 3e-3 Ic
+    gm.add_noise(truth[:nh], obs[:nh], (3e-3 ** 2 - 1e-3 ** 2) ** 0.5, seed=seed + 2)
+    torch.cuda.synchronize()
+    res = {"workload": f"{npix} px, Fe I 6301.5 + 6302.5 A sharing one atmosphere, 112 lambda "
+                       "step 21.5 mA, PSF FWHM 60 mA, 9 free par, <= 50 LM it; 10 % weak "
+                       "field, 5 % noise 3e-3 Ic"}
+    outs = {}
+    for name, ctx in (("mixed", mixed), ("fp32", fp32)):
+        ctx.invert(obs[:16384].contiguous())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        outs[name] = ctx.invert(obs, want_synth=False)
+        torch.cuda.synchronize()
+        res[name + "_px_per_s"] = npix / (time.perf_counter() - t0)
+    if oracle_sample > 0:
+        import oracle
+        oracle.lib()
+        o = oracle.make_setup([gm.FE6301, gm.FE6302_PAIR], gm.GRID_6302, psf=("gauss", 60.0),
+                              init_defaults=gm.BENCH_INIT, free=gm.FREE9, miter=50, noise=1e-6,
+                              ce_line=1)
+        k = oracle_sample
+        ref = oracle.invert(o, obs[:k].cpu().numpy(), threads=min(os.cpu_count() or 1, 16),
+                            want_synth=False)
+        for name, out in outs.items():
+            m = out["model"][:k].cpu().numpy()
+            d = np.abs(m - ref["model"])
+            d[:, 6] = np.minimum(d[:, 6], 180.0 - d[:, 6])
+            med = np.median(d, axis=0)
+            c2 = out["chisqr"][:k].cpu().numpy().astype(np.float64)
+            res[name + "_median_err_vs_fp64_oracle"] = {
+                "sample_px": k, "eta0": med[0], "B_G": med[1], "vlos_m_s": med[2] * 1e3,
+                "dopp_mA": med[3] * 1e3, "a": med[4], "gamma_deg": med[5], "phi_deg": med[6],
+                "S0": med[7], "S1": med[8],
+                "chisqr_rel": float(np.median(np.abs(c2 - ref["chisqr"]) / ref["chisqr"]))}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -293,6 +342,8 @@ def main():
     ap.add_argument("--train-profiles", type=int, default=1 << 22)
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--oracle-sample", type=int, default=1024)
+    ap.add_argument("--mixed-npix", type=int, default=1024 * 1024,
+                    help="cube of the two-line mixed-precision leg (0 skips it)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only run a plain device copy of known size and one normal-equation "
                          "(k_rf) and one synthesis (k_synth) launch, for counter runs")
@@ -411,6 +462,9 @@ def main():
         res["serving"] = sv
         res["p50_tile_latency_ms"] = sv["p50_tile_latency_ms"]
         res["p99_tile_latency_ms"] = sv["p99_tile_latency_ms"]
+    if args.mixed_npix > 0:
+        res["mixed_precision"] = mixed_precision_leg(gm, torch, dev, args.mixed_npix,
+                                                     args.seed + 11, min(args.oracle_sample, 512))
     if args.guided_npix > 0:
         res["learned_guess"] = guided_leg(gm, torch, dev, args.guided_npix, args.train_profiles,
                                           args.train_steps, args.seed + 5, args.oracle_sample)

@@ -359,6 +359,29 @@ __device__ __forceinline__ int staged_slot(int p)
     return (p <= 6) ? p + 1 : (p == 7 ? -1 : (p == 8 ? 8 : -2));
 }
 
+/* One accept / reject decision (milosUtils.cu:660-698): stop tests on the
+ * chi^2 change and value, lambda update (/5 on accept, x10 on reject, with
+ * the logclambda factor), lambda bounds and the iteration cap.          */
+template <typename T> struct LmStep {
+    bool accept, cont;
+    T flambda;
+};
+template <typename T>
+__device__ __forceinline__ LmStep<T> lm_step(const gm_dsetup *__restrict__ S, T chisqr, T ochisqr,
+                                             T flambda, T factor, int it)
+{
+    bool clanda = (fabs((ochisqr - chisqr) * T(100) / chisqr) < (T)S->toplim) ||
+                  (chisqr < T(0.0001));
+    const bool accept = chisqr - ochisqr < T(0);
+    flambda = accept ? flambda / (T(5.0) * factor) : flambda * T(10.0) * factor;
+    if ((flambda > T(1e+7)) || (flambda < T(1e-25))) clanda = true;
+    LmStep<T> r;
+    r.accept = accept;
+    r.cont = (it + 1 < S->miter) && !clanda;
+    r.flambda = flambda;
+    return r;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Synthesis + response functions + PSF + normal equations + chi^2 for a list
  * of pixels; wavelength-parallel mapping.                                   */
@@ -1005,34 +1028,26 @@ __global__ __launch_bounds__(1024) void k_decide(const gm_dsetup *__restrict__ S
         if (valid) {
             const T chisqr = (T)st.chi_trial[pix];
             T ochisqr = (T)st.ochisqr[pix];
-            T flambda = (T)st.flambda[pix];
-            const T factor = (T)st.factor[pix];
-            int it = st.iter[pix];
-            bool clanda = (fabs(((ochisqr) - (chisqr)) * T(100) / (chisqr)) < (T)S->toplim) ||
-                          ((chisqr) < T(0.0001));
-            bool accept = (chisqr) - (ochisqr) < T(0);
-            if (accept) {
-                flambda = flambda / (T(5.0) * factor);
+            const int it = st.iter[pix];
+            const LmStep<T> d = lm_step<T>(S, chisqr, ochisqr, (T)st.flambda[pix],
+                                           (T)st.factor[pix], it);
+            if (d.accept) {
 #pragma unroll
                 for (int f = 0; f < GM_NMODEL; ++f) st.best[f * cap + pix] = st.trial[f * cap + pix];
 #pragma unroll
                 for (int f = 0; f < GM_NPRE; ++f)
                     st.pre_best[f * cap + pix] = st.pre_trial[f * cap + pix];
                 ochisqr = chisqr;
-            } else {
-                flambda = flambda * T(10.0) * factor;
             }
-            if ((flambda > T(1e+7)) || (flambda < T(1e-25))) clanda = true;
-            it++;
             if (S->logclambda) {
                 const T c0 = (T)st.chisqr0[pix];
                 st.factor[pix] = (double)(log10(chisqr) / log10(c0));
             }
-            st.flambda[pix] = (double)flambda;
+            st.flambda[pix] = (double)d.flambda;
             st.ochisqr[pix] = (double)ochisqr;
-            st.iter[pix] = it;
-            cont = (it < S->miter) && !clanda;
-            accn = accept && cont;
+            st.iter[pix] = it + 1;
+            cont = d.cont;
+            accn = d.accept && cont;
             if (accn) st.nacc[pix] += 1;
         }
         const bool fin = valid && !cont && done_list;
