@@ -352,6 +352,14 @@ __device__ __forceinline__ T group_sum(T v)
     return v;
 }
 
+/* The 30 of the 36 evaluated values (per Stokes: synth, d/d{eta0, B, vlos,
+ * dopp, a, gamma, phi}, d/dS1) that k_rf convolves: all but d/dphi of I and
+ * V (zero) and the four d/dS1. */
+__constant__ constexpr int kStagedSlot[30] = {0,  1,  2,  3,  4,  5,  6,                 /* I */
+                                              9,  10, 11, 12, 13, 14, 15, 16,          /* Q */
+                                              18, 19, 20, 21, 22, 23, 24, 25,          /* U */
+                                              27, 28, 29, 30, 31, 32, 33};             /* V */
+
 /* RF slot (reference layout) -> staged slot j within a Stokes block; -1 is
  * the unconvolved S0 response, -2 the always-zero slot 9.                   */
 __device__ __forceinline__ int staged_slot(int p)
@@ -423,6 +431,16 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
             load_model(models, mfs, mps, pix, m);
             pp = make_pre(m);
         }
+        /* observed samples of this lane's wavelengths, loaded before the
+         * synthesis so that their latency is hidden behind it */
+        float ob[NLPL][4];
+#pragma unroll
+        for (int s = 0; s < NLPL; ++s) {
+            const int k = k0 + s * LPP;
+#pragma unroll
+            for (int st = 0; st < 4; ++st)
+                ob[s][st] = (obs && k < n) ? obs[pix * 4 * n + st * n + k] : 0.f;
+        }
 
         /* zero padding rows (the reduction buffer aliases them) */
         for (int r = k0; r < rows; r += LPP)
@@ -438,12 +456,20 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
             if (k >= n)
 #pragma unroll
                 for (int j = 0; j < GM_SLOTS; ++j) v[j] = 0.f;
+            /* 30 of the 36 values: d/dphi of I and V is zero and the S1
+             * responses follow from the convolved synthesis (below) */
+            float c[32];
+#pragma unroll
+            for (int i = 0; i < 30; ++i) c[i] = v[kStagedSlot[i]];
+            c[30] = 0.f;
+            c[31] = 0.f;
             float4 *row = conv + (padlo + k) * 9;
 #pragma unroll
-            for (int j = 0; j < 9; ++j)
-                row[j] = make_float4(v[4 * j], v[4 * j + 1], v[4 * j + 2], v[4 * j + 3]);
+            for (int j = 0; j < 8; ++j)
+                row[j] = make_float4(c[4 * j], c[4 * j + 1], c[4 * j + 2], c[4 * j + 3]);
         }
         __syncthreads();
+        const float inv_s1 = 1.f / pp.S1;
 
         float acc[GM_RED_N];
 #pragma unroll
@@ -452,20 +478,20 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
 #pragma unroll
         for (int s = 0; s < NLPL; ++s) {
             const int k = k0 + s * LPP;
-            float y[GM_SLOTS];
+            float z[32];
             if (use_psf) {
 #pragma unroll
-                for (int j = 0; j < GM_SLOTS; ++j) y[j] = 0.f;
+                for (int j = 0; j < 32; ++j) z[j] = 0.f;
                 for (int d = dlo; d <= dhi; ++d) {
                     const float tap = S->psf_tap[d - dlo];
                     const float4 *row = conv + (padlo + k + d) * 9;
 #pragma unroll
-                    for (int j = 0; j < 9; ++j) {
+                    for (int j = 0; j < 8; ++j) {
                         const float4 x = row[j];
-                        y[4 * j] += tap * x.x;
-                        y[4 * j + 1] += tap * x.y;
-                        y[4 * j + 2] += tap * x.z;
-                        y[4 * j + 3] += tap * x.w;
+                        z[4 * j] += tap * x.x;
+                        z[4 * j + 1] += tap * x.y;
+                        z[4 * j + 2] += tap * x.z;
+                        z[4 * j + 3] += tap * x.w;
                     }
                 }
                 /* I-variant (Ic trick) on the Stokes-I block: adds
@@ -473,31 +499,43 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
                 const float edge = S->psf_edge[k < n ? k : 0];
                 const float4 *r0 = conv + padlo * 9;
                 const float4 *rn = conv + (padlo + n - 1) * 9;
-                const float4 a0 = r0[0], a1 = r0[1], a2 = r0[2];
-                const float4 b0 = rn[0], b1 = rn[1], b2 = rn[2];
-                y[0] += end_max(a0.x, b0.x) * edge;
-                y[1] += end_max(a0.y, b0.y) * edge;
-                y[2] += end_max(a0.z, b0.z) * edge;
-                y[3] += end_max(a0.w, b0.w) * edge;
-                y[4] += end_max(a1.x, b1.x) * edge;
-                y[5] += end_max(a1.y, b1.y) * edge;
-                y[6] += end_max(a1.z, b1.z) * edge;
-                y[7] += end_max(a1.w, b1.w) * edge;
-                y[8] += end_max(a2.x, b2.x) * edge;
+                const float4 a0 = r0[0], a1 = r0[1];
+                const float4 b0 = rn[0], b1 = rn[1];
+                z[0] += end_max(a0.x, b0.x) * edge;
+                z[1] += end_max(a0.y, b0.y) * edge;
+                z[2] += end_max(a0.z, b0.z) * edge;
+                z[3] += end_max(a0.w, b0.w) * edge;
+                z[4] += end_max(a1.x, b1.x) * edge;
+                z[5] += end_max(a1.y, b1.y) * edge;
+                z[6] += end_max(a1.z, b1.z) * edge;
             } else {
                 const float4 *row = conv + (padlo + k) * 9;
 #pragma unroll
-                for (int j = 0; j < 9; ++j) {
+                for (int j = 0; j < 8; ++j) {
                     const float4 x = row[j];
-                    y[4 * j] = x.x; y[4 * j + 1] = x.y; y[4 * j + 2] = x.z; y[4 * j + 3] = x.w;
+                    z[4 * j] = x.x; z[4 * j + 1] = x.y; z[4 * j + 2] = x.z; z[4 * j + 3] = x.w;
                 }
             }
+            /* back to the 36 slots.  With I = S0 + S1 f and Q, U, V = S1 g
+             * (mil_sinrf.cu:346-424) the S1 responses are f and g, and by
+             * linearity of the correlation (and of the Ic rule, whose end
+             * maximum of f is that of I minus S0, over S1) their convolved
+             * values are (conv(I) - S0)/S1 and conv(Q, U, V)/S1.           */
+            float y[GM_SLOTS];
+#pragma unroll
+            for (int i = 0; i < 30; ++i) y[kStagedSlot[i]] = z[i];
+            y[7] = 0.f;
+            y[34] = 0.f;
+            y[8] = (y[0] - pp.S0) * inv_s1;
+            y[17] = y[9] * inv_s1;
+            y[26] = y[18] * inv_s1;
+            y[35] = y[27] * inv_s1;
             if (k < n) {
                 float g[4][9];
                 float r[4];
 #pragma unroll
                 for (int st = 0; st < 4; ++st) {
-                    r[st] = obs ? y[st * 9] - obs[pix * 4 * n + st * n + k] : 0.f;
+                    r[st] = obs ? y[st * 9] - ob[s][st] : 0.f;
 #pragma unroll
                     for (int p = 0; p < 9; ++p) {
                         const int js = staged_slot(p);
@@ -613,6 +651,16 @@ __global__ __launch_bounds__(64) void k_synth(
             load_model(models, mfs, mps, pix, m);
             pp = make_pre(m);
         }
+        /* observed samples of this lane's wavelengths, loaded before the
+         * synthesis so that their latency is hidden behind it */
+        float ob[NLPL][4];
+#pragma unroll
+        for (int s = 0; s < NLPL; ++s) {
+            const int k = k0 + s * LPP;
+#pragma unroll
+            for (int st = 0; st < 4; ++st)
+                ob[s][st] = (obs && k < n) ? obs[pix * 4 * n + st * n + k] : 0.f;
+        }
 #pragma unroll
         for (int s = 0; s < NLPL; ++s) {
             const int k = k0 + s * LPP;
@@ -644,8 +692,8 @@ __global__ __launch_bounds__(64) void k_synth(
                 y = conv[padlo + k];
             }
             if (k < n && obs) {
-                const float *ob = obs + pix * 4 * n + k;
-                const float r0 = y.x - ob[0], r1 = y.y - ob[n], r2 = y.z - ob[2 * n], r3 = y.w - ob[3 * n];
+                const float r0 = y.x - ob[s][0], r1 = y.y - ob[s][1], r2 = y.z - ob[s][2],
+                            r3 = y.w - ob[s][3];
                 if (MIXED) {
                     chi += (double)S->wsig[0] * ((double)r0 * r0) + (double)S->wsig[1] * ((double)r1 * r1) +
                            (double)S->wsig[2] * ((double)r2 * r2) + (double)S->wsig[3] * ((double)r3 * r3);
