@@ -734,9 +734,74 @@ template <> __device__ __forceinline__ double rsq<double>(double x)
     return y * (1.5 - 0.5 * x * y * y);      /* one Newton step */
 }
 
-/* SVD of the symmetric damped matrix by cyclic Jacobi rotations; the
- * singular values are |eigenvalues| and the solution is the reference's
- * delta = V diag(1/w, |w| > 1e-12) V^T beta (milosUtils.cu:296-322).        */
+/* SVD of the symmetric damped matrix (singular values = |eigenvalues|) by
+ * Jacobi rotations, then the reference's pseudo-inverse
+ * x = V diag(1/w, |w| > 1e-12) V^T b (milosUtils.cu:296-322).
+ *
+ * Rotations follow the round-robin (parallel) ordering: at step K the nine
+ * indices form the four disjoint pairs ((K+i) mod 9, (K-i) mod 9), i = 1..4,
+ * and index K waits, so a sweep is nine steps of four independent rotations
+ * (all 36 pairs once) whose angle computations and updates interleave
+ * instead of forming one 36-long dependency chain.  A' = J^T A J is applied
+ * block-wise on the packed upper triangle: the pair's own 2x2 block is
+ * diagonalised exactly, the 2x2 blocks between two pairs get G_i^T M G_j,
+ * the idle index's entries get the column rotation, and V <- V J.          */
+template <typename T, int K>
+__device__ __forceinline__ void jacobi_step(T (&a)[45], T (&v)[81])
+{
+    constexpr int P[4] = {(K + 1) % 9, (K + 2) % 9, (K + 3) % 9, (K + 4) % 9};
+    constexpr int Q[4] = {(K + 8) % 9, (K + 7) % 9, (K + 6) % 9, (K + 5) % 9};
+    T c[4], sn[4], t[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const T app = a[ut(P[i], P[i])], aqq = a[ut(Q[i], Q[i])], apq = a[ut(P[i], Q[i])];
+        const T d = aqq - app;
+        const T den = fabs(d) + sqrt(d * d + T(4) * apq * apq);
+        T ti = (den > T(0)) ? (T(2) * apq) / den : T(0);
+        ti = (d < T(0)) ? -ti : ti;
+        t[i] = ti;
+        c[i] = rsq<T>(T(1) + ti * ti);
+        sn[i] = ti * c[i];
+    }
+    /* blocks between two pairs: rows of pair i, columns of pair j */
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = i + 1; j < 4; ++j) {
+            const T m00 = a[ut(P[i], P[j])], m01 = a[ut(P[i], Q[j])];
+            const T m10 = a[ut(Q[i], P[j])], m11 = a[ut(Q[i], Q[j])];
+            const T n00 = c[j] * m00 - sn[j] * m01, n01 = sn[j] * m00 + c[j] * m01;
+            const T n10 = c[j] * m10 - sn[j] * m11, n11 = sn[j] * m10 + c[j] * m11;
+            a[ut(P[i], P[j])] = c[i] * n00 - sn[i] * n10;
+            a[ut(P[i], Q[j])] = c[i] * n01 - sn[i] * n11;
+            a[ut(Q[i], P[j])] = sn[i] * n00 + c[i] * n10;
+            a[ut(Q[i], Q[j])] = sn[i] * n01 + c[i] * n11;
+        }
+    /* the waiting index K against each pair */
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const T akp = a[ut(K, P[i])], akq = a[ut(K, Q[i])];
+        a[ut(K, P[i])] = c[i] * akp - sn[i] * akq;
+        a[ut(K, Q[i])] = sn[i] * akp + c[i] * akq;
+    }
+    /* each pair's own block */
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const T apq = a[ut(P[i], Q[i])];
+        a[ut(P[i], P[i])] -= t[i] * apq;
+        a[ut(Q[i], Q[i])] += t[i] * apq;
+        a[ut(P[i], Q[i])] = T(0);
+    }
+#pragma unroll
+    for (int r = 0; r < 9; ++r)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const T vp = v[r * 9 + P[i]], vq = v[r * 9 + Q[i]];
+            v[r * 9 + P[i]] = c[i] * vp - sn[i] * vq;
+            v[r * 9 + Q[i]] = sn[i] * vp + c[i] * vq;
+        }
+}
+
 template <typename T, int MAXSWEEP>
 __device__ __forceinline__ void solve_svd(T (&a)[45], const T (&b)[9], T (&x)[9])
 {
@@ -755,36 +820,16 @@ __device__ __forceinline__ void solve_svd(T (&a)[45], const T (&b)[9], T (&x)[9]
         }
         done = done || (off <= eps2 * dg);
         if (__all(done)) break;
-#pragma unroll
-        for (int p = 0; p < 8; ++p) {
-#pragma unroll
-            for (int q = p + 1; q < 9; ++q) {
-                const T apq = a[ut(p, q)];
-                const T app = a[ut(p, p)], aqq = a[ut(q, q)];
-                const T d = aqq - app;
-                const T den = fabs(d) + sqrt(d * d + T(4) * apq * apq);
-                T t = (den > T(0)) ? (T(2) * apq) / den : T(0);
-                t = (d < T(0)) ? -t : t;
-                t = done ? T(0) : t;
-                const T c = rsq<T>(T(1) + t * t);
-                const T s = t * c;
-                a[ut(p, p)] = app - t * apq;
-                a[ut(q, q)] = aqq + t * apq;
-                a[ut(p, q)] = T(0);
-#pragma unroll
-                for (int r = 0; r < 9; ++r) {
-                    if (r == p || r == q) continue;
-                    const T arp = a[ut(r, p)], arq = a[ut(r, q)];
-                    a[ut(r, p)] = c * arp - s * arq;
-                    a[ut(r, q)] = s * arp + c * arq;
-                }
-#pragma unroll
-                for (int r = 0; r < 9; ++r) {
-                    const T vrp = v[r * 9 + p], vrq = v[r * 9 + q];
-                    v[r * 9 + p] = c * vrp - s * vrq;
-                    v[r * 9 + q] = s * vrp + c * vrq;
-                }
-            }
+        if (!done) {
+            jacobi_step<T, 0>(a, v);
+            jacobi_step<T, 1>(a, v);
+            jacobi_step<T, 2>(a, v);
+            jacobi_step<T, 3>(a, v);
+            jacobi_step<T, 4>(a, v);
+            jacobi_step<T, 5>(a, v);
+            jacobi_step<T, 6>(a, v);
+            jacobi_step<T, 7>(a, v);
+            jacobi_step<T, 8>(a, v);
         }
     }
     T aux[9];
@@ -803,6 +848,126 @@ __device__ __forceinline__ void solve_svd(T (&a)[45], const T (&b)[9], T (&x)[9]
         for (int k = 0; k < 9; ++k) sum += v[i * 9 + k] * aux[k];
         x[i] = sum;
     }
+}
+
+/* The same Jacobi SVD with V split over two lanes: both lanes rotate the
+ * (identical) matrix, lane `half` keeps rows 5*half .. 5*half+4 of V (row 9
+ * a zero dummy), so a lane holds 45 + 45 values instead of 45 + 81 and the
+ * FP64 version stays in registers.  x is complete in the half-0 lane.      */
+template <typename T, int K>
+__device__ __forceinline__ void jacobi_step_rows(T (&a)[45], T (&vr)[5][9])
+{
+    constexpr int P[4] = {(K + 1) % 9, (K + 2) % 9, (K + 3) % 9, (K + 4) % 9};
+    constexpr int Q[4] = {(K + 8) % 9, (K + 7) % 9, (K + 6) % 9, (K + 5) % 9};
+    T c[4], sn[4], t[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const T app = a[ut(P[i], P[i])], aqq = a[ut(Q[i], Q[i])], apq = a[ut(P[i], Q[i])];
+        const T d = aqq - app;
+        const T den = fabs(d) + sqrt(d * d + T(4) * apq * apq);
+        T ti = (den > T(0)) ? (T(2) * apq) / den : T(0);
+        ti = (d < T(0)) ? -ti : ti;
+        t[i] = ti;
+        c[i] = rsq<T>(T(1) + ti * ti);
+        sn[i] = ti * c[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = i + 1; j < 4; ++j) {
+            const T m00 = a[ut(P[i], P[j])], m01 = a[ut(P[i], Q[j])];
+            const T m10 = a[ut(Q[i], P[j])], m11 = a[ut(Q[i], Q[j])];
+            const T n00 = c[j] * m00 - sn[j] * m01, n01 = sn[j] * m00 + c[j] * m01;
+            const T n10 = c[j] * m10 - sn[j] * m11, n11 = sn[j] * m10 + c[j] * m11;
+            a[ut(P[i], P[j])] = c[i] * n00 - sn[i] * n10;
+            a[ut(P[i], Q[j])] = c[i] * n01 - sn[i] * n11;
+            a[ut(Q[i], P[j])] = sn[i] * n00 + c[i] * n10;
+            a[ut(Q[i], Q[j])] = sn[i] * n01 + c[i] * n11;
+        }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const T akp = a[ut(K, P[i])], akq = a[ut(K, Q[i])];
+        a[ut(K, P[i])] = c[i] * akp - sn[i] * akq;
+        a[ut(K, Q[i])] = sn[i] * akp + c[i] * akq;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const T apq = a[ut(P[i], Q[i])];
+        a[ut(P[i], P[i])] -= t[i] * apq;
+        a[ut(Q[i], Q[i])] += t[i] * apq;
+        a[ut(P[i], Q[i])] = T(0);
+    }
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const T vp = vr[r][P[i]], vq = vr[r][Q[i]];
+            vr[r][P[i]] = c[i] * vp - sn[i] * vq;
+            vr[r][Q[i]] = sn[i] * vp + c[i] * vq;
+        }
+}
+
+template <typename T, int MAXSWEEP>
+__device__ __forceinline__ void solve_svd_pair(T (&a)[45], const T (&b)[9], int half, T (&x)[9])
+{
+    T vr[5][9];
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+#pragma unroll
+        for (int j = 0; j < 9; ++j) vr[r][j] = (half * 5 + r == j) ? T(1) : T(0);
+    const T eps2 = (sizeof(T) == 8) ? T(1e-30) : T(1e-14);
+    bool done = false;
+    for (int sweep = 0; sweep < MAXSWEEP; ++sweep) {
+        T off = 0, dg = 0;
+#pragma unroll
+        for (int p = 0; p < 9; ++p) {
+            dg += a[ut(p, p)] * a[ut(p, p)];
+#pragma unroll
+            for (int q = p + 1; q < 9; ++q) off += a[ut(p, q)] * a[ut(p, q)];
+        }
+        done = done || (off <= eps2 * dg);
+        if (__all(done)) break;
+        if (!done) {
+            jacobi_step_rows<T, 0>(a, vr);
+            jacobi_step_rows<T, 1>(a, vr);
+            jacobi_step_rows<T, 2>(a, vr);
+            jacobi_step_rows<T, 3>(a, vr);
+            jacobi_step_rows<T, 4>(a, vr);
+            jacobi_step_rows<T, 5>(a, vr);
+            jacobi_step_rows<T, 6>(a, vr);
+            jacobi_step_rows<T, 7>(a, vr);
+            jacobi_step_rows<T, 8>(a, vr);
+        }
+    }
+    /* V^T b: this lane's rows, then the partner's */
+    T aux[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        T sum = 0;
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            const int row = half * 5 + r;
+            T br = b[0];
+#pragma unroll
+            for (int k = 1; k < 9; ++k) br = (row == k) ? b[k] : br;
+            sum += (row < 9) ? vr[r][j] * br : T(0);
+        }
+        sum += __shfl_xor(sum, 1, 64);
+        const T w = fabs(a[ut(j, j)]);
+        aux[j] = (w > T(1e-12)) ? sum / w : T(0);
+    }
+    T xr[5];
+#pragma unroll
+    for (int r = 0; r < 5; ++r) {
+        T sum = 0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) sum += vr[r][k] * aux[k];
+        xr[r] = sum;
+    }
+#pragma unroll
+    for (int r = 0; r < 5; ++r) x[r] = xr[r];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) x[5 + r] = __shfl_xor(xr[r], 1, 64);
 }
 
 /* LDL^T solve of the damped matrix (symmetric positive semi-definite by
@@ -987,13 +1152,13 @@ __global__ __launch_bounds__(64) void k_solve(const gm_dsetup *__restrict__ S,
         T a[45], b[9], x[9];
         load_damped<T>(st.alpha + pix, st.beta + pix, cap, (T)st.flambda[pix], a, b);
         if (SOLVER == GM_SOLVE_JACOBI) {
-            solve_svd<T, (sizeof(T) == 8 ? 12 : 10)>(a, b, x);
-        } else {
-            const bool proven = solve_ldl<T>(a, b, x);
-            if (SOLVER == GM_SOLVE_SVD && !proven) {
-                fb_list[atomicAdd(fb_count, 1)] = (int32_t)pix;
-                continue;
-            }
+            fb_list[atomicAdd(fb_count, 1)] = (int32_t)pix;
+            continue;
+        }
+        const bool proven = solve_ldl<T>(a, b, x);
+        if (SOLVER == GM_SOLVE_SVD && !proven) {
+            fb_list[atomicAdd(fb_count, 1)] = (int32_t)pix;
+            continue;
         }
         apply_to_trial<T>(S, st, pix, x);
     }
@@ -1010,13 +1175,14 @@ __global__ __launch_bounds__(64) void k_solve_svd(const gm_dsetup *__restrict__ 
     const int64_t cnt = *fb_count;
     if (hist && blockIdx.x == 0 && threadIdx.x == 0) hist[1] = (int32_t)cnt;
     const int64_t cap = st.cap;
-    for (int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x; i < cnt;
-         i += (int64_t)gridDim.x * 64) {
+    const int half = threadIdx.x & 1;
+    for (int64_t i = ((int64_t)blockIdx.x * 64 + threadIdx.x) >> 1; i < cnt;
+         i += ((int64_t)gridDim.x * 64) >> 1) {
         const int64_t pix = fb_list[i];
         T a[45], b[9], x[9];
         load_damped<T>(st.alpha + pix, st.beta + pix, cap, (T)st.flambda[pix], a, b);
-        solve_svd<T, (sizeof(T) == 8 ? 12 : 10)>(a, b, x);
-        apply_to_trial<T>(S, st, pix, x);
+        solve_svd_pair<T, (sizeof(T) == 8 ? 12 : 10)>(a, b, half, x);
+        if (half == 0) apply_to_trial<T>(S, st, pix, x);
     }
 }
 
@@ -1492,8 +1658,8 @@ void gm_launch_solve(const gm_dsetup *dS, gm_state st, const int32_t *list,
         else GM_SOLVE_CASE(false, GM_SOLVE_JACOBI);
     }
 #undef GM_SOLVE_CASE
-    if (solver == GM_SOLVE_SVD) {
-        const int gf = grid_for(grid_items, 64, 1024);
+    if (solver != GM_SOLVE_LDL) {
+        const int gf = grid_for(grid_items, 32, 1024);
         if (mixed)
             hipLaunchKernelGGL((k_solve_svd<true>), dim3(gf), dim3(64), 0, s, dS, st, fb_list, fb_count,
                                hist);

@@ -237,7 +237,7 @@ def _assert_medians(d, label=""):
     assert med[1] <= 1.0 and med[2] <= 1e-3 and med[5] <= 0.1 and med[6] <= 0.1, med
 
 
-@pytest.mark.parametrize("precision,solver", [(1, 0), (0, 0), (1, 1)])
+@pytest.mark.parametrize("precision,solver", [(1, 0), (0, 0), (1, 1), (1, 2)])
 def test_inversion_config0_matches_oracle(gm, orc, cuda, precision, solver):
     """64 x 64 pixels, Fe I 6173, 30 wavelengths, no PSF, 9 free, 50 it."""
     got, ref, truth, obs, ctx = _inversion_errors(gm, orc, cuda, "single", None, 4096, 100,
