@@ -128,7 +128,7 @@ __global__ __launch_bounds__(256) void k_gemm(const uint16_t *__restrict__ A, in
                 const size_t o = (size_t)m * ldc + n;
                 if (EPI == EPI_BIAS_GELU) {
                     v += bias[n];
-                    Cf[o] = v;
+                    if (Cf) Cf[o] = v;
                     Cb[o] = f2bf(gelu(v));
                 } else if (EPI == EPI_BIAS) {
                     v += bias[n];
@@ -144,25 +144,110 @@ __global__ __launch_bounds__(256) void k_gemm(const uint16_t *__restrict__ A, in
             }
 }
 
+/* C[M x N] = A[M x K] . B[K x N], A and B row-major BF16, for the forward
+ * pass: 128 x TN tiles per 256-thread workgroup, each wave a 64 x TN/2
+ * block of v_mfma_f32_16x16x32_bf16 (gfx950), K staged 32 at a time
+ * through LDS with 16-byte global loads; B is stored transposed in LDS so
+ * that both MFMA operands are 16-byte LDS reads.  Operand layout: lane l
+ * supplies row (A) / column (B) l % 16 and the eight k = 8 (l / 16) .. +7;
+ * the accumulator holds rows 4 (l / 16) .. +3 of column l % 16.
+ * M % 128 == 0, N % TN == 0, K % 32 == 0.                                  */
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+template <int TN, int EPI>
+__global__ __launch_bounds__(256) void k_gemm_nn(const uint16_t *__restrict__ A, int lda,
+                                                 const uint16_t *__restrict__ B, int ldb, int K,
+                                                 const float *__restrict__ bias,
+                                                 float *__restrict__ Cf, uint16_t *__restrict__ Cb,
+                                                 int ldc)
+{
+    constexpr int FM = 4, FN = TN / 32;            /* 16 x 16 fragments per wave */
+    constexpr int LS = 32 + 8;                     /* LDS row stride (bf16)       */
+    __shared__ __attribute__((aligned(16))) uint16_t sA[128 * LS];
+    __shared__ __attribute__((aligned(16))) uint16_t sB[TN * LS];   /* [n][k] */
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int m0 = blockIdx.y * 128, n0 = blockIdx.x * TN;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * (TN / 2);
+    float4_t acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+    const int kq8 = 8 * (lane >> 4), l16 = lane & 15;
+    for (int k0 = 0; k0 < K; k0 += 32) {
+#pragma unroll
+        for (int it = 0; it < 2; ++it) {                /* A: 512 16-byte pieces */
+            const int idx = tid + 256 * it;
+            const int r = idx >> 2, q = idx & 3;
+            *reinterpret_cast<uint4 *>(&sA[r * LS + q * 8]) =
+                *reinterpret_cast<const uint4 *>(&A[(size_t)(m0 + r) * lda + k0 + q * 8]);
+        }
+#pragma unroll
+        for (int it = 0; it < TN / 64; ++it) {          /* B: 4 TN pieces */
+            const int idx = tid + 256 * it;
+            const int kr = idx / (TN / 8), q = idx % (TN / 8);
+            const uint4 v = *reinterpret_cast<const uint4 *>(&B[(size_t)(k0 + kr) * ldb + n0 + q * 8]);
+            const uint16_t *e = reinterpret_cast<const uint16_t *>(&v);
+#pragma unroll
+            for (int c = 0; c < 8; ++c) sB[(q * 8 + c) * LS + kr] = e[c];
+        }
+        __syncthreads();
+        bf16x8_t a[FM], b[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+            a[i] = *reinterpret_cast<const bf16x8_t *>(&sA[(wm + i * 16 + l16) * LS + kq8]);
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+            b[j] = *reinterpret_cast<const bf16x8_t *>(&sB[(wn + j * 16 + l16) * LS + kq8]);
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
+                const int n = n0 + wn + j * 16 + l16;
+                float v = acc[i][j][r] + bias[n];
+                const size_t o = (size_t)m * ldc + n;
+                if (Cf) Cf[o] = v;                 /* pre-activations: training only */
+                if (EPI == EPI_BIAS_GELU) Cb[o] = f2bf(gelu(v));
+                else if (Cb) Cb[o] = f2bf(v);
+            }
+}
+
 /* Input: obs float [npix][4][nl] -> bf16 [npix][kin], normalised by the
- * continuum Ic = max(I[0], I[nl-1]); Q, U, V scaled by 10. */
+ * continuum Ic = max(I[0], I[nl-1]); I -> 2 (I/Ic - 1), Q, U, V -> 10 x/Ic.
+ * One thread per 8 consecutive features (one 16-byte store). */
 __global__ void k_prep_input(const float *__restrict__ obs, int64_t npix, int nl, int kin,
                              uint16_t *__restrict__ x)
 {
-    const int64_t total = npix * kin;
+    const int per = kin / 8;
+    const int64_t total = npix * per;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t p = i / kin;
-        const int f = (int)(i % kin);
-        float v = 0.f;
-        if (f < 4 * nl) {
-            const float *o = obs + p * 4 * nl;
-            const float ic = fmaxf(fmaxf(o[0], o[nl - 1]), 1e-6f);
-            const int s = f / nl;
-            v = o[f] / ic;
-            v = (s == 0) ? (v - 1.f) * 2.f : v * 10.f;
+        const int64_t p = i / per;
+        const int f0 = (int)(i % per) * 8;
+        const float *o = obs + p * 4 * nl;
+        const float ic = fmaxf(fmaxf(o[0], o[nl - 1]), 1e-6f);
+        uint16_t h[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int f = f0 + e;
+            float v = 0.f;
+            if (f < 4 * nl) {
+                v = o[f] / ic;
+                v = (f < nl) ? (v - 1.f) * 2.f : v * 10.f;
+            }
+            h[e] = f2bf(v);
         }
-        x[i] = f2bf(v);
+        *reinterpret_cast<uint4 *>(&x[p * kin + f0]) = *reinterpret_cast<const uint4 *>(h);
     }
 }
 
@@ -193,7 +278,8 @@ __global__ void k_prep_target(const double *__restrict__ models, int64_t npix,
 }
 
 /* Output decode into a start model (init fields untouched where the network
- * has no output), clamped to the LM's physical ranges. */
+ * has no output), clamped to the LM's physical ranges.  FP32 arithmetic:
+ * the result is a starting point for the LM. */
 __global__ void k_decode(const float *__restrict__ y, int64_t npix, const double *__restrict__ base,
                          double *__restrict__ models)
 {
@@ -201,24 +287,25 @@ __global__ void k_decode(const float *__restrict__ y, int64_t npix, const double
          p += (int64_t)gridDim.x * blockDim.x) {
         const float *o = y + p * GM_MLP_NOUT;
         double *m = models + p * 11;
-        for (int f = 0; f < 11; ++f) m[f] = base[f];
-        const double rad = 180.0 / 3.14159265358979323846;
-        double B = o[0] * 866.0 + 1500.0;
-        const double sg = fmax((o[2] + 1.0) * 0.5, 0.0);
-        double gm = atan2(sg, (double)o[1]) * rad;
-        double az = 0.5 * atan2((double)o[4], (double)o[3]) * rad;
-        if (az < 0) az += 180.0;
-        const double ic = fmax(o[10] * 0.2 + 1.0, 0.05);
-        const double s0f = fmin(fmax(o[9] * 0.1 + 0.25, 0.01), 0.99);
-        m[0] = fmin(fmax(o[8] * 5.485 + 10.5, 1.0), 2500.0);
-        m[1] = fmin(fmax(B, 0.0), 5000.0);
-        m[2] = fmin(fmax(o[5] * 1.732, -20.0), 20.0);
-        m[3] = fmin(fmax(o[6] * 0.01155 + 0.04, 0.005), 0.6);
-        m[4] = fmin(fmax(o[7] * 0.1414 + 0.255, 0.0001), 10.0);
-        m[5] = fmin(fmax(gm, 0.0), 180.0);
-        m[6] = fmin(fmax(az, 0.0), 180.0);
+        const float rad = 57.29577951308232f;
+        const float B = o[0] * 866.f + 1500.f;
+        const float sg = fmaxf((o[2] + 1.f) * 0.5f, 0.f);
+        const float gm = atan2f(sg, o[1]) * rad;
+        float az = 0.5f * atan2f(o[4], o[3]) * rad;
+        if (az < 0.f) az += 180.f;
+        const float ic = fmaxf(o[10] * 0.2f + 1.f, 0.05f);
+        const float s0f = fminf(fmaxf(o[9] * 0.1f + 0.25f, 0.01f), 0.99f);
+        m[0] = fminf(fmaxf(o[8] * 5.485f + 10.5f, 1.f), 2500.f);
+        m[1] = fminf(fmaxf(B, 0.f), 5000.f);
+        m[2] = fminf(fmaxf(o[5] * 1.732f, -20.f), 20.f);
+        m[3] = fminf(fmaxf(o[6] * 0.01155f + 0.04f, 0.005f), 0.6f);
+        m[4] = fminf(fmaxf(o[7] * 0.1414f + 0.255f, 0.0001f), 10.f);
+        m[5] = fminf(fmaxf(gm, 0.f), 180.f);
+        m[6] = fminf(fmaxf(az, 0.f), 180.f);
         m[7] = s0f * ic;
-        m[8] = (1.0 - s0f) * ic;
+        m[8] = (1.f - s0f) * ic;
+        m[9] = base[9];
+        m[10] = base[10];
     }
 }
 
@@ -239,14 +326,23 @@ __global__ void k_loss(const float *__restrict__ y, const float *__restrict__ t,
     if ((threadIdx.x & 63) == 0) atomicAdd(loss_acc, local / (float)(b * 11));
 }
 
-/* db[n] = sum_b dY[b][n] (dY bf16). */
-__global__ void k_bias_grad(const uint16_t *__restrict__ dy, int64_t b, int n, float *__restrict__ db)
+/* db[n] = sum_b dY[b][n] (dY bf16): a block sums 64 columns over a slice
+ * of rows (four row groups, coalesced 128-byte row reads), reduces the
+ * groups in LDS and adds its partial sum to db (zeroed beforehand). */
+__global__ __launch_bounds__(256) void k_bias_grad(const uint16_t *__restrict__ dy, int64_t b, int n,
+                                                   int64_t rows_per_block, float *__restrict__ db)
 {
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= n) return;
+    __shared__ float part[4][64];
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int col = blockIdx.x * 64 + c;
+    const int64_t r0 = (int64_t)blockIdx.y * rows_per_block;
+    const int64_t r1 = (r0 + rows_per_block < b) ? r0 + rows_per_block : b;
     float s = 0.f;
-    for (int64_t r = 0; r < b; ++r) s += bf2f(dy[r * n + col]);
-    db[col] = s;
+    if (col < n)
+        for (int64_t r = r0 + g; r < r1; r += 4) s += bf2f(dy[r * n + col]);
+    part[g][c] = s;
+    __syncthreads();
+    if (g == 0 && col < n) atomicAdd(&db[col], part[0][c] + part[1][c] + part[2][c] + part[3][c]);
 }
 
 /* AdamW (Loshchilov & Hutter 2019) on FP32 master weights; refreshes the
@@ -287,6 +383,29 @@ void gm_mlp_gemm(int ta, int tb, int epi, const uint16_t *A, int lda, const uint
                  int M, int N, int K, const float *bias, const float *zprev, int ldz, float *Cf,
                  uint16_t *Cb, int ldc, hipStream_t s)
 {
+    if (!ta && !tb && (epi == EPI_BIAS_GELU || epi == EPI_BIAS) && M % 128 == 0 && K % 32 == 0 &&
+        (lda % 8) == 0 && (ldb % 8) == 0) {
+        if (N % 128 == 0) {
+            dim3 g(N / 128, M / 128);
+            if (epi == EPI_BIAS_GELU)
+                hipLaunchKernelGGL((k_gemm_nn<128, EPI_BIAS_GELU>), g, dim3(256), 0, s, A, lda, B, ldb,
+                                   K, bias, Cf, Cb, ldc);
+            else
+                hipLaunchKernelGGL((k_gemm_nn<128, EPI_BIAS>), g, dim3(256), 0, s, A, lda, B, ldb, K,
+                                   bias, Cf, Cb, ldc);
+            return;
+        }
+        if (N % 64 == 0) {
+            dim3 g(N / 64, M / 128);
+            if (epi == EPI_BIAS_GELU)
+                hipLaunchKernelGGL((k_gemm_nn<64, EPI_BIAS_GELU>), g, dim3(256), 0, s, A, lda, B, ldb,
+                                   K, bias, Cf, Cb, ldc);
+            else
+                hipLaunchKernelGGL((k_gemm_nn<64, EPI_BIAS>), g, dim3(256), 0, s, A, lda, B, ldb, K,
+                                   bias, Cf, Cb, ldc);
+            return;
+        }
+    }
     dim3 grid(N / 64, M / 64);
     if (!ta && !tb) {
         if (epi == EPI_BIAS_GELU) GEMM_CASE(false, false, EPI_BIAS_GELU);
@@ -307,7 +426,8 @@ static int grid1d(int64_t n, int b) { int64_t g = (n + b - 1) / b; return (int)(
 
 void gm_mlp_prep_input(const float *obs, int64_t npix, int nl, int kin, uint16_t *x, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_prep_input, dim3(grid1d(npix * kin, 256)), dim3(256), 0, s, obs, npix, nl, kin, x);
+    hipLaunchKernelGGL(k_prep_input, dim3(grid1d(npix * (kin / 8), 256)), dim3(256), 0, s, obs, npix,
+                       nl, kin, x);
 }
 void gm_mlp_prep_target(const double *models, int64_t npix, float *t, hipStream_t s)
 {
@@ -323,7 +443,10 @@ void gm_mlp_loss(const float *y, const float *t, int64_t b, uint16_t *dy, float 
 }
 void gm_mlp_bias_grad(const uint16_t *dy, int64_t b, int n, float *db, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_bias_grad, dim3((n + 63) / 64), dim3(64), 0, s, dy, b, n, db);
+    const int64_t rpb = 64;
+    (void)hipMemsetAsync(db, 0, sizeof(float) * n, s);
+    hipLaunchKernelGGL(k_bias_grad, dim3((n + 63) / 64, (unsigned)((b + rpb - 1) / rpb)), dim3(256), 0,
+                       s, dy, b, n, rpb, db);
 }
 void gm_mlp_adamw(float *w, const float *g, float *m, float *v, uint16_t *wb, int64_t n, float lr,
                   float b1, float b2, float eps, float wd, float bc1, float bc2, hipStream_t s)

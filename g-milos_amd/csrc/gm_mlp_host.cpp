@@ -65,7 +65,7 @@ static gm_status ensure_batch(gm_mlp *n, int64_t rows)
     if (rows <= n->cap) return GM_OK;
     (void)hipDeviceSynchronize();
     free_batch(n);
-    const int64_t cap = ((rows + 63) / 64) * 64;
+    const int64_t cap = ((rows + 127) / 128) * 128;
     const int H = GM_MLP_HIDDEN;
     n->x = dalloc<uint16_t>(cap * n->kin);
     n->y = dalloc<float>(cap * GM_MLP_NOUT);
@@ -166,15 +166,16 @@ extern "C" gm_status gm_mlp_set_params(gm_mlp *n, const float *host, int64_t cnt
     return hipDeviceSynchronize() == hipSuccess ? GM_OK : GM_ERR_HIP;
 }
 
-/* Forward pass over `rows` (a multiple of 64) already in n->x. */
-static void forward(gm_mlp *n, int64_t rows, hipStream_t s)
+/* Forward pass over `rows` (a multiple of 128) already in n->x; the FP32
+ * pre-activations are kept only for a following backward pass. */
+static void forward(gm_mlp *n, int64_t rows, hipStream_t s, bool keep_z = true)
 {
     const uint16_t *in = n->x;
     int ld = n->kin;
     for (int l = 0; l < 3; ++l) {
         const Layer &L = n->L[l];
         gm_mlp_gemm(0, 0, GM_EPI_BIAS_GELU, in, ld, n->wb + L.woff, L.out, (int)rows, L.out, L.in,
-                    n->w + L.boff, nullptr, 0, n->z[l], n->a[l], L.out, s);
+                    n->w + L.boff, nullptr, 0, keep_z ? n->z[l] : nullptr, n->a[l], L.out, s);
         in = n->a[l];
         ld = L.out;
     }
@@ -264,9 +265,9 @@ extern "C" gm_status gm_mlp_predict(gm_mlp *n, const float *obs, int64_t npix, c
         return GM_ERR_HIP;
     for (int64_t p0 = 0; p0 < npix; p0 += chunk) {
         const int64_t rows = (npix - p0 < chunk) ? npix - p0 : chunk;
-        const int64_t padded = ((rows + 63) / 64) * 64;
+        const int64_t padded = ((rows + 127) / 128) * 128;
         gm_mlp_prep_input(obs + p0 * 4 * n->nl, rows, n->nl, n->kin, n->x, s);
-        forward(n, padded, s);
+        forward(n, padded, s, false);
         gm_mlp_decode(n->y, rows, n->base, models + p0 * GM_NMODEL, s);
     }
     return hipGetLastError() == hipSuccess ? GM_OK : GM_ERR_HIP;
