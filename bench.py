@@ -334,9 +334,12 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=1234)
-    ap.add_argument("--serve-seconds", type=float, default=20.0,
+    ap.add_argument("--serve-seconds", type=float, default=15.0,
                     help="length of the serving-loop leg (0 skips it)")
-    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-rate", type=float, default=200.0,
+                    help="tile arrival rate of the serving leg (tiles/s)")
+    ap.add_argument("--serve-rate-low", type=float, default=100.0,
+                    help="second, below-capacity arrival rate (0 skips it)")
     ap.add_argument("--guided-npix", type=int, default=2048 * 2048,
                     help="cube of the learned-guess leg (0 skips it)")
     ap.add_argument("--train-profiles", type=int, default=1 << 22)
@@ -462,6 +465,9 @@ def main():
         res["serving"] = sv
         res["p50_tile_latency_ms"] = sv["p50_tile_latency_ms"]
         res["p99_tile_latency_ms"] = sv["p99_tile_latency_ms"]
+        if args.serve_rate_low > 0:
+            res["serving_below_capacity"] = serving_leg(gm, torch, dev, args.serve_seconds,
+                                                        args.serve_rate_low, args.seed + 1)
     if args.mixed_npix > 0:
         res["mixed_precision"] = mixed_precision_leg(gm, torch, dev, args.mixed_npix,
                                                      args.seed + 11, min(args.oracle_sample, 512))
