@@ -150,7 +150,64 @@ bool read_floats(FILE *fp, const ImageHeader &h, int64_t n, std::vector<float> &
     return true;
 }
 
+/* Any FITS BITPIX (8, 16, 32, 64, -32, -64) converted to double with
+ * BSCALE/BZERO, as CFITSIO's TDOUBLE reads do. */
+bool read_doubles(FILE *fp, const ImageHeader &h, int64_t n, std::vector<double> &out)
+{
+    const int bytes = std::abs(h.bitpix) / 8;
+    if (bytes != 1 && bytes != 2 && bytes != 4 && bytes != 8) return false;
+    std::vector<unsigned char> raw((size_t)(n * bytes));
+    if (std::fread(raw.data(), 1, raw.size(), fp) != raw.size()) return false;
+    out.resize(n);
+    for (int64_t i = 0; i < n; ++i) {
+        const unsigned char *p = raw.data() + i * bytes;
+        uint64_t u = 0;
+        for (int b = 0; b < bytes; ++b) u = (u << 8) | p[b];
+        double v;
+        switch (h.bitpix) {
+        case 8: v = (double)(uint8_t)u; break;
+        case 16: v = (double)(int16_t)(uint16_t)u; break;
+        case 32: v = (double)(int32_t)(uint32_t)u; break;
+        case 64: v = (double)(int64_t)u; break;
+        case -32: { uint32_t w = (uint32_t)u; float f; std::memcpy(&f, &w, 4); v = f; break; }
+        case -64: std::memcpy(&v, &u, 8); break;
+        default: return false;
+        }
+        out[i] = h.bzero + h.bscale * v;
+    }
+    return true;
+}
+
 } // namespace
+
+/* Wavelength FITS file (readFitsLambdaToArray, utilsFits.c:1446-1529): a
+ * 2-D image of naxes (nlambda, 2); the first row holds the line index (the
+ * reference keeps the row's last value), the second the wavelengths in A.
+ * The reference writes past its nlambda-sized array when the image has more
+ * than two rows and reads into a null array for its 4-D form; both are
+ * rejected here.  Returns nlambda, or a negative code. */
+extern "C" int32_t gmio_fits_read_lambda(const char *path, int32_t *line_index, double *lambda,
+                                         int32_t cap)
+{
+    FILE *fp = std::fopen(path, "rb");
+    if (!fp) return -1;
+    std::vector<std::string> cards;
+    ImageHeader h;
+    std::vector<double> v;
+    int32_t rc = -2;
+    if (read_header(fp, cards) && parse_header(cards, h) && h.naxis == 2 && h.naxes[1] == 2 &&
+        h.naxes[0] >= 1) {
+        const int64_t n = h.naxes[0];
+        if (n > cap) rc = -3;
+        else if (read_doubles(fp, h, 2 * n, v)) {
+            *line_index = (int32_t)v[n - 1];
+            for (int64_t k = 0; k < n; ++k) lambda[k] = v[n + k];
+            rc = (int32_t)n;
+        }
+    }
+    std::fclose(fp);
+    return rc;
+}
 
 extern "C" int32_t gmio_fits_write_image(const char *path, int32_t naxis, const int64_t *naxes,
                                          const char *const *ctypes, const float *data)

@@ -65,21 +65,40 @@ int main(int argc, char **argv)
     if (cfg.initial_guess_model[0] && gmio_read_mod(cfg.initial_guess_model, init) != 0)
         return fail("cannot read initial model ", cfg.initial_guess_model);
 
-    /* wavelength grid (gmilos.cu:171-196) */
-    if (!cfg.use_malla_grid) return fail("wavelength FITS files are not supported: ", cfg.wavelength_file);
-    double first, step, last;
-    const int index = gmio_read_grid(cfg.wavelength_file, &first, &step, &last);
-    if (!index) return fail("cannot read grid ", cfg.wavelength_file);
-    const int n = (int)((last - first) / step) + 1;
-    if (n < 2 || n > GM_MAX_LAMBDA) return fail("grid size out of range");
-    std::vector<double> off(n), lam(n);
-    off[0] = first;
-    for (int k = 1; k < n; ++k) off[k] = off[k - 1] + step;
-    double lambda0, qn[6];
-    if (gm_parse_lines_file(cfg.atomic_parameters_file, index, &lambda0, qn) != GM_OK)
-        return fail("quantum line not found in ", cfg.atomic_parameters_file);
-    lam[0] = lambda0 + first / 1000.0;
-    for (int k = 1; k < n; ++k) lam[k] = lam[k - 1] + step / 1000.0;
+    /* wavelengths: .grid offsets (gmilos.cu:171-196) or a FITS file of
+     * absolute wavelengths (gmilos.cu:197-209) */
+    int index = 0, n = 0;
+    double step = 0, lambda0, qn[6];
+    std::vector<double> off, lam;
+    if (cfg.use_malla_grid) {
+        double first, last;
+        index = gmio_read_grid(cfg.wavelength_file, &first, &step, &last);
+        if (!index) return fail("cannot read grid ", cfg.wavelength_file);
+        n = (int)((last - first) / step) + 1;
+        if (n < 2 || n > GM_MAX_LAMBDA) return fail("grid size out of range");
+        off.resize(n);
+        lam.resize(n);
+        off[0] = first;
+        for (int k = 1; k < n; ++k) off[k] = off[k - 1] + step;
+        if (gm_parse_lines_file(cfg.atomic_parameters_file, index, &lambda0, qn) != GM_OK)
+            return fail("quantum line not found in ", cfg.atomic_parameters_file);
+        lam[0] = lambda0 + first / 1000.0;
+        for (int k = 1; k < n; ++k) lam[k] = lam[k - 1] + step / 1000.0;
+    } else if (!std::strcmp(gmio_file_ext(cfg.wavelength_file), ".fits")) {
+        lam.resize(GM_MAX_LAMBDA);
+        n = gmio_fits_read_lambda(cfg.wavelength_file, &index, lam.data(), GM_MAX_LAMBDA);
+        if (n < 2) return fail("the wavelength file has not been read properly: ", cfg.wavelength_file);
+        lam.resize(n);
+        if (gm_parse_lines_file(cfg.atomic_parameters_file, index, &lambda0, qn) != GM_OK)
+            return fail("quantum line not found in ", cfg.atomic_parameters_file);
+        /* the reference leaves the mA offsets unset on this path; a PSF
+         * table is resampled here at the offsets from the line centre */
+        off.resize(n);
+        for (int k = 0; k < n; ++k) off[k] = (lam[k] - lambda0) * 1000.0;
+        step = off[1] - off[0];
+    } else {
+        return fail("the wavelength file must be .grid or .fits: ", cfg.wavelength_file);
+    }
 
     gm_setup s;
     gm_setup_defaults(&s);

@@ -99,6 +99,11 @@ int32_t gmio_fits_write_image(const char *path, int32_t naxis, const int64_t *na
                               const char *const *ctypes, const float *data);
 int32_t gmio_fits_read_image(const char *path, int32_t *naxis, int64_t naxes[4],
                              float *data, int64_t cap);
+/* Wavelength FITS file (readFitsLambdaToArray, utilsFits.c:1446-1529):
+ * naxes (nlambda, 2), row 1 the line index, row 2 the wavelengths in A; any
+ * BITPIX.  Returns nlambda (<= cap) or a negative code. */
+int32_t gmio_fits_read_lambda(const char *path, int32_t *line_index, double *lambda,
+                              int32_t cap);
 
 /* file_ext / get_basefilename of readConfig.c:2312-2349 */
 const char *gmio_file_ext(const char *path);

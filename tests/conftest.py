@@ -73,3 +73,24 @@ def read_mod(path):
                 if k in name:
                     out[v] = float(val.split("!")[0])
     return out
+
+
+def write_fits(path, data, bitpix, bscale=None, bzero=None):
+    """A primary-HDU FITS image written from the standard's definition
+    (80-column cards, 2880-byte records, big-endian data); numpy axis order
+    is the reverse of NAXISn."""
+    cards = ["SIMPLE  = %20s" % "T", "BITPIX  = %20d" % bitpix, "NAXIS   = %20d" % data.ndim]
+    cards += ["NAXIS%d  = %20d" % (i + 1, n) for i, n in enumerate(reversed(data.shape))]
+    if bscale is not None:
+        cards.append("BSCALE  = %20.12G" % bscale)
+    if bzero is not None:
+        cards.append("BZERO   = %20.12G" % bzero)
+    cards.append("END")
+    head = "".join(c.ljust(80) for c in cards).encode()
+    head += b" " * (-len(head) % 2880)
+    dt = {8: ">u1", 16: ">i2", 32: ">i4", 64: ">i8", -32: ">f4", -64: ">f8"}[bitpix]
+    body = np.ascontiguousarray(data, dtype=dt).tobytes()
+    body += b"\0" * (-len(body) % 2880)
+    with open(path, "wb") as f:
+        f.write(head + body)
+

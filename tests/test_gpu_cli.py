@@ -10,7 +10,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, ROOT, model_row, read_mod
+from conftest import GOLDEN, ROOT, model_row, read_mod, write_fits
 
 pytestmark = pytest.mark.gpu
 CLI = os.path.join(ROOT, "g-milos_amd", "bin", "gmilos")
@@ -41,6 +41,22 @@ def run(cwd, *args):
 def test_cli_synthesis_matches_reference_fixture(refdir):
     run(refdir, "run/synthesis.mtrol")
     got = np.loadtxt(refdir / "data" / "synthesis_gpu.per")      # overwritten, as the reference
+    ref = np.loadtxt(os.path.join(GOLDEN, "synthesis_gpu.per"))
+    np.testing.assert_allclose(got[:, :2], ref[:, :2], atol=1e-6)
+    assert np.abs(got[:, 2:] - ref[:, 2:]).max() < 2e-6
+
+
+def test_cli_wavelength_fits_file(refdir):
+    """The synthesis example with its grid given as a FITS file of absolute
+    wavelengths (gmilos.cu:197-209) reproduces the same fixture."""
+    lam = 6173.3356 + np.arange(-350.0, 666.0, 35.0) / 1000.0
+    write_fits(str(refdir / "run" / "lambda.fits"), np.stack([np.full(30, 1.0), lam]), -64)
+    text = open(refdir / "run" / "synthesis.mtrol").read().splitlines()
+    assert text[4].startswith("Wavelength grid file")
+    text[4] = "Wavelength grid file      (s):run/lambda.fits  ! wavelength file"
+    open(refdir / "run" / "synth_fits.mtrol", "w").write("\n".join(text) + "\n")
+    run(refdir, "run/synth_fits.mtrol")
+    got = np.loadtxt(refdir / "data" / "synthesis_gpu.per")
     ref = np.loadtxt(os.path.join(GOLDEN, "synthesis_gpu.per"))
     np.testing.assert_allclose(got[:, :2], ref[:, :2], atol=1e-6)
     assert np.abs(got[:, 2:] - ref[:, 2:]).max() < 2e-6

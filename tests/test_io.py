@@ -10,7 +10,7 @@ import shutil
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, ROOT
+from conftest import GOLDEN, ROOT, write_fits
 
 IO_LIB = os.path.join(ROOT, "g-milos_amd", "lib", "libgmilos_io.so")
 P = 4096
@@ -59,6 +59,8 @@ def io():
     L.gmio_fits_write_image.argtypes = [ctypes.c_char_p, ctypes.c_int32, vp, vp, vp]
     L.gmio_fits_read_image.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32), vp, vp,
                                        ctypes.c_int64]
+    L.gmio_fits_read_lambda.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32), vp,
+                                        ctypes.c_int32]
     L.gmio_file_ext.argtypes = [ctypes.c_char_p]
     L.gmio_file_ext.restype = ctypes.c_char_p
     L.gmio_base_name.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int32]
@@ -224,3 +226,35 @@ def test_fits_model_planes(io, tmp_path):
     np.testing.assert_array_equal(planes[:11].T, models.astype(np.float32))
     np.testing.assert_array_equal(planes[11], it.astype(np.float32))
     np.testing.assert_array_equal(planes[12], c2)
+
+
+def test_fits_wavelength_file(io, tmp_path):
+    """readFitsLambdaToArray (utilsFits.c:1446-1529): naxes (nlambda, 2), row 1
+    the line index, row 2 the wavelengths."""
+    lam = 6173.3356 + np.arange(-350.0, 666.0, 35.0) / 1000.0
+    img = np.stack([np.full(30, 1.0), lam])
+    idx = ctypes.c_int32()
+    out = np.zeros(64)
+    path = str(tmp_path / "l64.fits")
+    write_fits(path, img, -64)
+    assert io.gmio_fits_read_lambda(path.encode(), ctypes.byref(idx), _p(out), 64) == 30
+    assert idx.value == 1
+    np.testing.assert_array_equal(out[:30], lam)
+    # float32 storage reads back the rounded values, as a TDOUBLE read does
+    write_fits(path, img, -32)
+    assert io.gmio_fits_read_lambda(path.encode(), ctypes.byref(idx), _p(out), 64) == 30
+    np.testing.assert_array_equal(out[:30], lam.astype(np.float32).astype(np.float64))
+    # scaled integers: value = BZERO + BSCALE * stored
+    ints = np.stack([np.full(30, 2), np.arange(30) * 7 - 100]).astype(np.int16)
+    write_fits(path, ints, 16, bscale=0.5, bzero=6000.0)
+    assert io.gmio_fits_read_lambda(path.encode(), ctypes.byref(idx), _p(out), 64) == 30
+    np.testing.assert_array_equal(out[:30], 6000.0 + 0.5 * (np.arange(30) * 7 - 100))
+    assert idx.value == 6001                  # line-index row is scaled too
+    # too many wavelengths for the buffer, a third row, a 1-D image, no file
+    write_fits(path, img, -64)
+    assert io.gmio_fits_read_lambda(path.encode(), ctypes.byref(idx), _p(out), 20) == -3
+    write_fits(path, np.vstack([img, img[1:]]), -64)
+    assert io.gmio_fits_read_lambda(path.encode(), ctypes.byref(idx), _p(out), 64) < 0
+    write_fits(path, lam, -64)
+    assert io.gmio_fits_read_lambda(path.encode(), ctypes.byref(idx), _p(out), 64) < 0
+    assert io.gmio_fits_read_lambda(b"missing.fits", ctypes.byref(idx), _p(out), 64) < 0
