@@ -196,6 +196,58 @@ void gmo_conv_ic(double *x, int n, const double *h, double ic)
     for (int k = 0; k < n; k++) x[k] = ic - tmp[k];
 }
 
+/* Circular convolution centred on n/2 (d_convCircular / convCircular,
+ * convolution.cu:90-118, 251-280): the reference forms
+ * a_i = sum_j h_j x_{(i-j) mod n} and stores a_i at index (i - n/2) mod n,
+ * i.e. y_m = sum_j h_j x_{(m + n/2 - j) mod n}.                            */
+void gmo_conv_circ(double *x, int n, const double *h)
+{
+    double tmp[GMO_MAXL];
+    const int c = n / 2;
+    for (int m = 0; m < n; m++) {
+        double acc = 0;
+        for (int j = 0; j < n; j++) {
+            int k = (m + c - j) % n;
+            if (k < 0) k += n;
+            acc += h[j] * x[k];
+        }
+        tmp[m] = acc;
+    }
+    memcpy(x, tmp, sizeof(double) * n);
+}
+
+/* Macroturbulence kernel (fgauss.cu:16-63): a Gaussian of width
+ * sigma = lambda0 * mac / c (mac in km/s) on the grid, centred on
+ * lambda[n/2] and normalised to unit sum.  Its mac derivative follows
+ * me_der.cu:124-131, G (x^2 - 1) / mac, which drops the derivative of the
+ * normalisation; with rf_reference_rules = 0 it is the exact derivative of
+ * the normalised kernel, G (x^2 - sum_j G_j x_j^2) / mac.  With the PSF on
+ * both are correlated with it (mil_sinrf.cu:104-107, me_der.cu:132-138).    */
+void gmo_mac_kernels(const gmo_setup *s, double mac, double *G, double *Gd)
+{
+    const int n = s->nlambda;
+    const double ild = (s->lambda0[0] * mac) / 2.99792458e5;
+    double x2[GMO_MAXL], sum = 0;
+    for (int i = 0; i < n; i++) {
+        const double x = (s->lambda[i] - s->lambda[n / 2]) / ild;
+        x2[i] = x * x;
+        const double term = x2[i] / 2;
+        G[i] = (term < 1e30) ? exp(-term) : 0;
+        sum += G[i];
+    }
+    for (int i = 0; i < n; i++) G[i] /= sum;
+    if (Gd) {
+        double m2 = 1.0;
+        if (!s->rf_reference_rules) {
+            m2 = 0;
+            for (int i = 0; i < n; i++) m2 += G[i] * x2[i];
+        }
+        for (int i = 0; i < n; i++) Gd[i] = G[i] / mac * (x2[i] - m2);
+        if (s->use_psf) gmo_conv(Gd, n, s->psf);
+    }
+    if (s->use_psf) gmo_conv(G, n, s->psf);
+}
+
 /* The continuum used by the Ic variant: the larger end sample
  * (mil_sinrf.cu:118-124, me_der.cu:170-175).                                */
 static double end_max(const double *x, int n)
@@ -220,7 +272,7 @@ static double end_max(const double *x, int n)
  * derivative scale (me_der.cu:71-74 store w*dF where shi = 2*w*F, i.e. half
  * the derivative of shi); 0 gives the exact derivative.                     */
 static void synth_core(const gmo_setup *s, const double *m, double *spectra,
-                       double *rf)
+                       double *rf, int alpha_rf_mixed)
 {
     const int n = s->nlambda;
     const double CC = GMO_PI / 180.0;
@@ -393,8 +445,30 @@ static void synth_core(const gmo_setup *s, const double *m, double *spectra,
         for (int st = 0; st < 4; st++) rf[st * NT * n + 9 * n + l] = 0;
     }
 
-    /* instrument PSF: mil_sinrf.cu:118-130 and me_der.cu:162-189 */
-    if (s->use_psf) {
+    /* macroturbulence (mac > 1e-4): circular convolution with the Gaussian
+     * (PSF-correlated when the PSF is on) instead of the PSF, no Ic rule;
+     * slot 9 holds d/dmac, the unconvolved spectra convolved with the
+     * kernel's derivative (mil_sinrf.cu:101-116, me_der.cu:119-156)        */
+    const double mac = m[GMO_MAC];
+    if (mac > 0.0001) {
+        double G[GMO_MAXL], Gd[GMO_MAXL];
+        gmo_mac_kernels(s, mac, G, rf ? Gd : NULL);
+        if (rf) {
+            const double *src[4] = {I, Q, U, V};
+            for (int st = 0; st < 4; st++) {
+                double *x = rf + st * NT * n + 9 * n;
+                memcpy(x, src[st], sizeof(double) * n);
+                gmo_conv_circ(x, n, Gd);
+                for (int p = 0; p < 9; p++)
+                    if (p != 7) gmo_conv_circ(rf + st * NT * n + p * n, n, G);
+            }
+        }
+        gmo_conv_circ(I, n, G);
+        gmo_conv_circ(Q, n, G);
+        gmo_conv_circ(U, n, G);
+        gmo_conv_circ(V, n, G);
+    } else if (s->use_psf) {
+        /* instrument PSF: mil_sinrf.cu:118-130 and me_der.cu:162-189 */
         gmo_conv_ic(I, n, s->psf, end_max(I, n));
         gmo_conv(Q, n, s->psf);
         gmo_conv(U, n, s->psf);
@@ -418,6 +492,27 @@ static void synth_core(const gmo_setup *s, const double *m, double *spectra,
         spectra[2 * n + l] = U[l];
         spectra[3 * n + l] = V[l];
     }
+    /* stray light: I = alfa I_mag + (1 - alfa) I_stray for every Stokes
+     * parameter (mil_sinrf.cu:133-140); every response is scaled by alfa and
+     * slot 9 becomes d/dalfa = I_mag - I_stray (me_der.cu:191-211).  The
+     * reference's LM loop passes the mixed spectrum as "I_mag" after the
+     * first evaluation (milosUtils.cu:673), which gives alfa (I_mag - I_stray);
+     * alpha_rf_mixed reproduces that.                                       */
+    if (s->use_slight) {
+        const double al = m[GMO_ALFA];
+        double pure[4 * GMO_MAXL];
+        memcpy(pure, spectra, sizeof(double) * 4 * n);
+        for (int k = 0; k < 4 * n; k++) spectra[k] = pure[k] * al + s->slight[k] * (1.0 - al);
+        if (rf) {
+            const double *base = alpha_rf_mixed ? spectra : pure;
+            for (int st = 0; st < 4; st++)
+                for (int p = 0; p < NT; p++)
+                    for (int l = 0; l < n; l++) {
+                        double *r = rf + st * NT * n + p * n + l;
+                        *r = (p == 9) ? base[st * n + l] - s->slight[st * n + l] : *r * al;
+                    }
+        }
+    }
     /* zero the RFs of fixed parameters: milosUtils.cu:243-271 (NTERMS=10,
      * slot 9 follows the filling-factor flag fix[10]).                      */
     if (rf) {
@@ -432,13 +527,19 @@ static void synth_core(const gmo_setup *s, const double *m, double *spectra,
 
 void gmo_synth(const gmo_setup *s, const double *model, double *spectra)
 {
-    synth_core(s, model, spectra, NULL);
+    synth_core(s, model, spectra, NULL, 0);
 }
 
 void gmo_synth_rf(const gmo_setup *s, const double *model, double *spectra,
                   double *rf)
 {
-    synth_core(s, model, spectra, rf);
+    synth_core(s, model, spectra, rf, 0);
+}
+
+void gmo_synth_rf2(const gmo_setup *s, const double *model, double *spectra,
+                   double *rf, int alpha_rf_mixed)
+{
+    synth_core(s, model, spectra, rf, alpha_rf_mixed);
 }
 
 /* Normal equations.  lib.cu:12-59 / 65-118: alpha = sum_s (w_s/sigma^2)
@@ -763,7 +864,7 @@ int gmo_lm_pixel(const gmo_setup *s, const float *obs, const double *init,
         if (chisqr - ochisqr < 0.) {
             flambda = flambda / (5.0 * factor);
             memcpy(best, model, sizeof best);
-            gmo_synth_rf(s, best, synth, rf);
+            gmo_synth_rf2(s, best, synth, rf, 1);
             gmo_covarm(s, obs, synth, rf, alpha, beta);
             ochisqr = chisqr;
             nacc++;

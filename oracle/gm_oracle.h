@@ -56,6 +56,8 @@ typedef struct {
     int     use_ce;                  /* classical estimates overwrite init  */
     int     ce_line;                 /* line used by classical estimates    */
     int     rf_reference_rules;      /* 1: reproduce reference me_der rules  */
+    int     use_slight;              /* stray-light profile mixed in          */
+    double  slight[4 * GMO_MAXL];    /* stray light, I,Q,U,V blocks of n      */
 } gmo_setup;
 
 /* ---- atomic / PSF set-up ---------------------------------------------- */
@@ -71,11 +73,21 @@ int  gmo_psf_interp(const double *dl, const double *psf, int npsf,
 void gmo_voigt(double a, double u, double *H, double *F);
 void gmo_conv(double *x, int n, const double *h);
 void gmo_conv_ic(double *x, int n, const double *h, double ic);
+/* circular convolution centred on n/2 (macroturbulence) */
+void gmo_conv_circ(double *x, int n, const double *h);
+/* macroturbulence kernel and its mac derivative on the grid, both
+ * correlated with the PSF when it is on (Gd may be NULL) */
+void gmo_mac_kernels(const gmo_setup *s, double mac, double *G, double *Gd);
 /* synthesis: spectra[4*n] (I block, Q, U, V) */
 void gmo_synth(const gmo_setup *s, const double *model, double *spectra);
 /* synthesis + response functions rf[s*GMO_NT*n + p*n + l] */
 void gmo_synth_rf(const gmo_setup *s, const double *model, double *spectra,
                   double *rf);
+/* as gmo_synth_rf; alpha_rf_mixed = 1 takes the filling-factor response
+ * from the mixed spectrum, as the reference's LM loop does after the
+ * first evaluation (milosUtils.cu:673) */
+void gmo_synth_rf2(const gmo_setup *s, const double *model, double *spectra,
+                   double *rf, int alpha_rf_mixed);
 void gmo_covarm(const gmo_setup *s, const float *obs, const double *synth,
                 const double *rf, double *alpha, double *beta);
 double gmo_chisqr(const gmo_setup *s, const float *obs, const double *synth,

@@ -41,7 +41,8 @@ class gmo_setup(ctypes.Structure):
                 ("toplim", ctypes.c_double), ("miter", ctypes.c_int), ("mu", ctypes.c_double),
                 ("logclambda", ctypes.c_int), ("fix", ctypes.c_int * 11),
                 ("init_model", ctypes.c_double * NMODEL), ("use_ce", ctypes.c_int),
-                ("ce_line", ctypes.c_int), ("rf_reference_rules", ctypes.c_int)]
+                ("ce_line", ctypes.c_int), ("rf_reference_rules", ctypes.c_int),
+                ("use_slight", ctypes.c_int), ("slight", ctypes.c_double * (4 * MAXL))]
 
 
 _lib = None
@@ -65,6 +66,9 @@ def lib():
         L.gmo_voigt.argtypes = [d, d, P(d), P(d)]
         L.gmo_conv.argtypes = [vp, i, vp]
         L.gmo_conv_ic.argtypes = [vp, i, vp, d]
+        L.gmo_conv_circ.argtypes = [vp, i, vp]
+        L.gmo_mac_kernels.argtypes = [P(gmo_setup), d, vp, vp]
+        L.gmo_synth_rf2.argtypes = [P(gmo_setup), vp, vp, vp, i]
         L.gmo_synth.argtypes = [P(gmo_setup), vp, vp]
         L.gmo_synth_rf.argtypes = [P(gmo_setup), vp, vp, vp]
         L.gmo_covarm.argtypes = [P(gmo_setup), vp, vp, vp, vp, vp]
@@ -116,6 +120,32 @@ def voigt(a, u):
     H, F = ctypes.c_double(), ctypes.c_double()
     lib().gmo_voigt(float(a), float(u), ctypes.byref(H), ctypes.byref(F))
     return H.value, F.value
+
+
+def conv_circ(x, h):
+    x = np.array(x, dtype=np.float64)
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    lib().gmo_conv_circ(_p(x), len(x), _p(h))
+    return x
+
+
+def mac_kernels(s, mac):
+    G = np.zeros(s.nlambda)
+    Gd = np.zeros(s.nlambda)
+    lib().gmo_mac_kernels(ctypes.byref(s), float(mac), _p(G), _p(Gd))
+    return G, Gd
+
+
+def set_slight(s, profile):
+    """Stray-light profile [4][nlambda] (None switches it off)."""
+    if profile is None:
+        s.use_slight = 0
+        return s
+    prof = np.asarray(profile, dtype=np.float64).reshape(4 * s.nlambda)
+    s.use_slight = 1
+    for k in range(4 * s.nlambda):
+        s.slight[k] = float(prof[k])
+    return s
 
 
 def conv(x, h, ic=None):
@@ -209,13 +239,14 @@ def synth(s, models):
     return out
 
 
-def synth_rf(s, models):
+def synth_rf(s, models, alpha_rf_mixed=0):
     models = np.ascontiguousarray(np.atleast_2d(models), dtype=np.float64)
     n = s.nlambda
     sp = np.zeros((models.shape[0], 4, n))
     rf = np.zeros((models.shape[0], 4, NT, n))
     for p in range(models.shape[0]):
-        lib().gmo_synth_rf(ctypes.byref(s), _p(models[p]), _p(sp[p]), _p(rf[p]))
+        lib().gmo_synth_rf2(ctypes.byref(s), _p(models[p]), _p(sp[p]), _p(rf[p]),
+                            int(alpha_rf_mixed))
     return sp, rf
 
 
