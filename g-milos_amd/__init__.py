@@ -65,7 +65,8 @@ class gm_setup(ctypes.Structure):
                 ("init_model", ctypes.c_double * GM_NMODEL), ("init_mode", ctypes.c_int32),
                 ("ce_line", ctypes.c_int32), ("precision", ctypes.c_int32),
                 ("solver", ctypes.c_int32), ("rf_reference_rules", ctypes.c_int32),
-                ("psf_tail_tol", ctypes.c_double)]
+                ("psf_tail_tol", ctypes.c_double), ("use_slight", ctypes.c_int32),
+                ("slight", ctypes.c_float * (4 * GM_MAX_LAMBDA))]
 
 
 class gm_out(ctypes.Structure):
@@ -260,12 +261,15 @@ FREE9 = ("eta0", "B", "vlos", "dopp", "aa", "gm", "az", "S0", "S1")
 def make_setup(lines, offsets_mA, psf=None, init=None, free=FREE9, miter=50, noise=1e-6,
                weights=(1.0, 1.0, 1.0, 1.0), ilambda=0.1, toplim=1e-12, mu=1.0,
                precision=GM_PREC_MIXED, solver=GM_SOLVE_SVD, init_mode=GM_INIT_CE,
-               ce_line=0, rf_reference_rules=1, psf_tail_tol=1e-9, logclambda=0):
+               ce_line=0, rf_reference_rules=1, psf_tail_tol=1e-9, logclambda=0,
+               slight=None):
     """Builds a gm_setup.
 
     lines: list of (lambda0 [A], (S_lo, L_lo, J_lo, S_up, L_up, J_up), fo).
     offsets_mA: grid offsets from lines[0]'s centre in mA (malla.grid style).
     psf: None, ("gauss", fwhm_mA) or ("table", dl_mA, values).
+    slight: optional stray-light profile [4][nlambda] mixed with the
+    initial model's filling factor alfa (init=dict(alfa=...)).
     """
     lib = load_library()
     s = gm_setup()
@@ -328,6 +332,11 @@ def make_setup(lines, offsets_mA, psf=None, init=None, free=FREE9, miter=50, noi
     s.solver = int(solver)
     s.rf_reference_rules = int(rf_reference_rules)
     s.psf_tail_tol = float(psf_tail_tol)
+    if slight is not None:
+        prof = np.asarray(slight, dtype=np.float32).reshape(4 * n)
+        s.use_slight = 1
+        for k in range(4 * n):
+            s.slight[k] = float(prof[k])
     return s
 
 

@@ -85,6 +85,7 @@ gm_lanecfg gm_lane_config(const gm_dsetup &h)
     const int padhi = h.psf_dhi > 0 ? h.psf_dhi : 0;
     lc.padlo = padlo;
     lc.rows = lc.lpp * lc.nlpl + padlo + padhi;
+    lc.ext = h.ext;
     return lc;
 }
 
@@ -100,9 +101,11 @@ static gm_status build_dsetup(const gm_setup &s, gm_dsetup &d)
     if (s.nlines < 1 || s.nlines > GM_MAX_LINES) return GM_ERR_INVALID;
     if (!(s.noise > 0)) return GM_ERR_INVALID;
     if (s.miter < 0 || s.miter > kMaxRounds) return GM_ERR_INVALID;
-    if (s.fix[9] || s.fix[10]) return GM_ERR_UNSUPPORTED;  /* mac / stray light */
-    if (s.init_model[9] > 0.0001) return GM_ERR_UNSUPPORTED; /* macroturbulence */
-    if (s.init_model[10] < 1.0) return GM_ERR_UNSUPPORTED;   /* filling factor  */
+    /* a free filling factor needs the 10-/11-term solve (lm_mils_11) */
+    if (s.fix[10]) return GM_ERR_UNSUPPORTED;
+    if (s.use_slight)
+        for (int k = 0; k < 4 * n; ++k)
+            if (!std::isfinite(s.slight[k])) return GM_ERR_INVALID;
     if (s.ce_line < 0 || s.ce_line >= s.nlines) return GM_ERR_INVALID;
     if (s.precision != GM_PREC_FP32 && s.precision != GM_PREC_MIXED) return GM_ERR_INVALID;
     if (s.solver != GM_SOLVE_SVD && s.solver != GM_SOLVE_LDL && s.solver != GM_SOLVE_JACOBI)
@@ -180,6 +183,12 @@ static gm_status build_dsetup(const gm_setup &s, gm_dsetup &d)
     d.logclambda = s.logclambda;
     for (int f = 0; f < GM_NMODEL; ++f) d.init_model[f] = s.init_model[f];
     for (int k = 0; k < n; ++k) d.lambda[k] = s.lambda[k];
+    d.use_slight = s.use_slight ? 1 : 0;
+    d.ext = (d.use_slight || s.fix[9] || s.init_model[9] > 0.0001) ? 1 : 0;
+    d.mac_l0 = s.lambda0[0];
+    for (int k = 0; k < n; ++k) d.psf_full[k] = d.use_psf ? (float)s.psf[k] : 0.f;
+    if (d.use_slight)
+        for (int k = 0; k < 4 * n; ++k) d.slight[k] = s.slight[k];
     /* classical-estimate range: the whole grid for one line, else the
      * samples nearest to the chosen line */
     d.ce_lo = 0;

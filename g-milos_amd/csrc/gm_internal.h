@@ -46,6 +46,12 @@ struct gm_dsetup {
     int32_t ce_lo, ce_hi;
     double  ce_l0, ce_geff;
     double  lambda[GM_MAX_LAMBDA];           /* absolute wavelengths (A)    */
+    /* macroturbulence / stray light (kernels built with EXT only) */
+    int32_t ext;                             /* 11-term physics active       */
+    int32_t use_slight;
+    double  mac_l0;                          /* lambda0 of the first line    */
+    float   psf_full[GM_MAX_LAMBDA];         /* all nlambda PSF taps         */
+    float   slight[4 * GM_MAX_LAMBDA];       /* stray light [4][nlambda]     */
 };
 
 /* Per-pixel LM state, structure-of-arrays over the pixel capacity. */
@@ -89,6 +95,7 @@ struct gm_lanecfg {
     int ppw;   /* pixels per 64-lane workgroup                               */
     int rows;  /* staged rows per pixel incl. zero padding                   */
     int padlo; /* zero rows before wavelength 0                              */
+    int ext;   /* macroturbulence / stray-light kernels                      */
 };
 
 gm_lanecfg gm_lane_config(const gm_dsetup &h);

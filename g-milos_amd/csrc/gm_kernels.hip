@@ -352,6 +352,73 @@ __device__ __forceinline__ T group_sum(T v)
     return v;
 }
 
+/* Macroturbulence kernel of one pixel on the wavelength grid
+ * (fgauss.cu:16-63): exp(-x^2/2), x = (lambda - lambda[n/2]) / sigma,
+ * sigma = lambda0 mac / c, normalised to unit sum (FP64 as the reference),
+ * then correlated with all nlambda PSF taps when the PSF is on
+ * (direct_convolution_double, mil_sinrf.cu:104-107).  Called by every lane
+ * of the workgroup (two barriers); gk, gc are the pixel group's LDS buffers.
+ * gc receives the combined kernel; returns its sum.                         */
+template <int LPP, int NLPL>
+__device__ float mac_kernel(const gm_dsetup *__restrict__ S, double mac, float *gk, float *gc,
+                            int k0)
+{
+    const int n = S->nlambda;
+    const double ild = S->mac_l0 * (mac > 1e-4 ? mac : 1e-4) / 2.99792458e5;
+    const double lc = S->lambda[n / 2];
+    double g[NLPL], part = 0.0;
+#pragma unroll
+    for (int s = 0; s < NLPL; ++s) {
+        const int k = k0 + s * LPP;
+        double v = 0.0;
+        if (k < n) {
+            const double x = (S->lambda[k] - lc) / ild;
+            const double t = 0.5 * (x * x);
+            v = (t < 1e30) ? exp(-t) : 0.0;
+        }
+        g[s] = v;
+        part += v;
+    }
+    const double sum = group_sum<double, LPP>(part);
+#pragma unroll
+    for (int s = 0; s < NLPL; ++s) {
+        const int k = k0 + s * LPP;
+        if (k < n) gk[k] = (float)(g[s] / sum);
+    }
+    __syncthreads();
+    const int c = n / 2;
+    float tot = 0.f;
+#pragma unroll
+    for (int s = 0; s < NLPL; ++s) {
+        const int k = k0 + s * LPP;
+        if (k < n) {
+            float v = 0.f;
+            if (S->use_psf) {
+                for (int j = 0; j < n; ++j) {
+                    const int i = k + j - c;
+                    if (i >= 0 && i < n) v += S->psf_full[j] * gk[i];
+                }
+            } else {
+                v = gk[k];
+            }
+            gc[k] = v;
+            tot += v;
+        }
+    }
+    __syncthreads();
+    return group_sum<float, LPP>(tot);
+}
+
+/* Circular correlation row index of convolution.cu:90-118: output m takes
+ * tap j from sample (m + n/2 - j) mod n.                                     */
+__device__ __forceinline__ int circ_row(int m, int c, int j, int n)
+{
+    int r = m + c - j;
+    r += (r < 0) ? n : 0;
+    r -= (r >= n) ? n : 0;
+    return r;
+}
+
 /* The 30 of the 36 evaluated values (per Stokes: synth, d/d{eta0, B, vlos,
  * dopp, a, gamma, phi}, d/dS1) that k_rf convolves: all but d/dphi of I and
  * V (zero) and the four d/dS1. */
@@ -393,7 +460,7 @@ __device__ __forceinline__ LmStep<T> lm_step(const gm_dsetup *__restrict__ S, T 
 /* ------------------------------------------------------------------------ */
 /* Synthesis + response functions + PSF + normal equations + chi^2 for a list
  * of pixels; wavelength-parallel mapping.                                   */
-template <int LPP, int NLPL, bool MIXED>
+template <int LPP, int NLPL, bool MIXED, bool EXT>
 __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
     const gm_dsetup *__restrict__ S, const double *__restrict__ models,
     const float *__restrict__ pre, int64_t mfs, int64_t mps, const float *__restrict__ obs,
@@ -413,6 +480,9 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
     const int region4 = max(rows * 9, (LPP / 2) * (RSTR / 4));
     float4 *conv = lds4 + h * region4;
     float *red = reinterpret_cast<float *>(conv);
+    /* EXT: macroturbulence kernel buffers after the PPW regions */
+    float *gk = reinterpret_cast<float *>(lds4 + PPW * region4) + h * 2 * (LPP * NLPL);
+    float *gc = gk + LPP * NLPL;
     const int use_psf = S->use_psf;
     const int dlo = S->psf_dlo, dhi = S->psf_dhi;
     const uint32_t fm = S->freemask;
@@ -469,6 +539,17 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
                 row[j] = make_float4(c[4 * j], c[4 * j + 1], c[4 * j + 2], c[4 * j + 3]);
         }
         __syncthreads();
+        /* EXT: this pixel's macroturbulence and filling factor */
+        bool mac_on = false;
+        float alv = 1.f, gsum = 1.f;
+        if (EXT) {
+            const double mac = models[9 * mfs + pix * mps];
+            alv = (float)models[10 * mfs + pix * mps];
+            mac_on = mac > 0.0001;
+            gsum = mac_kernel<LPP, NLPL>(S, mac, gk, gc, k0);
+        }
+        const float s0c = (EXT && mac_on) ? gsum : 1.f;
+        const float s0rf = (EXT && S->use_slight) ? alv : 1.f;
         const float inv_s1 = 1.f / pp.S1;
 
         float acc[GM_RED_N];
@@ -479,7 +560,25 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
         for (int s = 0; s < NLPL; ++s) {
             const int k = k0 + s * LPP;
             float z[32];
-            if (use_psf) {
+            if (EXT && mac_on) {
+                /* macroturbulence: circular correlation with the combined
+                 * kernel, no Ic rule (me_der.cu:143-152) */
+#pragma unroll
+                for (int j = 0; j < 32; ++j) z[j] = 0.f;
+                const int kk = k < n ? k : n - 1;
+                for (int j = 0; j < n; ++j) {
+                    const float tap = gc[j];
+                    const float4 *row = conv + (padlo + circ_row(kk, n / 2, j, n)) * 9;
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        const float4 x = row[q];
+                        z[4 * q] += tap * x.x;
+                        z[4 * q + 1] += tap * x.y;
+                        z[4 * q + 2] += tap * x.z;
+                        z[4 * q + 3] += tap * x.w;
+                    }
+                }
+            } else if (use_psf) {
 #pragma unroll
                 for (int j = 0; j < 32; ++j) z[j] = 0.f;
                 for (int d = dlo; d <= dhi; ++d) {
@@ -526,11 +625,20 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
             for (int i = 0; i < 30; ++i) y[kStagedSlot[i]] = z[i];
             y[7] = 0.f;
             y[34] = 0.f;
-            y[8] = (y[0] - pp.S0) * inv_s1;
+            y[8] = (y[0] - pp.S0 * s0c) * inv_s1;
             y[17] = y[9] * inv_s1;
             y[26] = y[18] * inv_s1;
             y[35] = y[27] * inv_s1;
             if (k < n) {
+                if (EXT && S->use_slight) {
+                    /* stray light (mil_sinrf.cu:133-140, me_der.cu:191-199) */
+#pragma unroll
+                    for (int st = 0; st < 4; ++st) {
+                        y[st * 9] = alv * y[st * 9] + (1.f - alv) * S->slight[st * n + k];
+#pragma unroll
+                        for (int p = 1; p < 9; ++p) y[st * 9 + p] *= alv;
+                    }
+                }
                 float g[4][9];
                 float r[4];
 #pragma unroll
@@ -539,7 +647,7 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
 #pragma unroll
                     for (int p = 0; p < 9; ++p) {
                         const int js = staged_slot(p);
-                        float gv = (js >= 0) ? y[st * 9 + js] : (st == 0 ? 1.f : 0.f);
+                        float gv = (js >= 0) ? y[st * 9 + js] : (st == 0 ? s0rf : 0.f);
                         g[st][p] = ((fm >> p) & 1u) ? gv : 0.f;
                     }
                 }
@@ -615,7 +723,7 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
 }
 
 /* Synthesis + PSF + chi^2 only (rejected steps need nothing more). */
-template <int LPP, int NLPL, bool MIXED>
+template <int LPP, int NLPL, bool MIXED, bool EXT>
 __global__ __launch_bounds__(64) void k_synth(
     const gm_dsetup *__restrict__ S, const double *__restrict__ models,
     const float *__restrict__ pre, int64_t mfs, int64_t mps, const float *__restrict__ obs,
@@ -631,6 +739,9 @@ __global__ __launch_bounds__(64) void k_synth(
     const int n = S->nlambda;
     const int64_t cnt = count ? (int64_t)(*count) : n_direct;
     float4 *conv = lds4 + h * rows;
+    /* EXT: macroturbulence kernel buffers after the PPW row blocks */
+    float *gk = reinterpret_cast<float *>(lds4 + PPW * rows) + h * 2 * (LPP * NLPL);
+    float *gc = gk + LPP * NLPL;
     const int use_psf = S->use_psf;
     const int dlo = S->psf_dlo, dhi = S->psf_dhi;
     const int nfree = S->nfree_chi;
@@ -670,13 +781,33 @@ __global__ __launch_bounds__(64) void k_synth(
                                       : make_float4(0.f, 0.f, 0.f, 0.f);
         }
         __syncthreads();
+        bool mac_on = false;
+        float alv = 1.f;
+        if (EXT) {
+            const double mac = models[9 * mfs + pix * mps];
+            alv = (float)models[10 * mfs + pix * mps];
+            mac_on = mac > 0.0001;
+            mac_kernel<LPP, NLPL>(S, mac, gk, gc, k0);
+        }
         double chi = 0.0;
         float chif = 0.f;
 #pragma unroll
         for (int s = 0; s < NLPL; ++s) {
             const int k = k0 + s * LPP;
             float4 y;
-            if (use_psf) {
+            if (EXT && mac_on) {
+                /* macroturbulence (mil_sinrf.cu:101-116): circular, no Ic rule */
+                y = make_float4(0.f, 0.f, 0.f, 0.f);
+                const int kk = k < n ? k : n - 1;
+                for (int j = 0; j < n; ++j) {
+                    const float tap = gc[j];
+                    const float4 x = conv[padlo + circ_row(kk, n / 2, j, n)];
+                    y.x += tap * x.x;
+                    y.y += tap * x.y;
+                    y.z += tap * x.z;
+                    y.w += tap * x.w;
+                }
+            } else if (use_psf) {
                 y = make_float4(0.f, 0.f, 0.f, 0.f);
                 for (int d = dlo; d <= dhi; ++d) {
                     const float tap = S->psf_tap[d - dlo];
@@ -690,6 +821,12 @@ __global__ __launch_bounds__(64) void k_synth(
                 y.x += end_max(conv[padlo].x, conv[padlo + n - 1].x) * edge;
             } else {
                 y = conv[padlo + k];
+            }
+            if (EXT && S->use_slight && k < n) {
+                y.x = alv * y.x + (1.f - alv) * S->slight[k];
+                y.y = alv * y.y + (1.f - alv) * S->slight[n + k];
+                y.z = alv * y.z + (1.f - alv) * S->slight[2 * n + k];
+                y.w = alv * y.w + (1.f - alv) * S->slight[3 * n + k];
             }
             if (k < n && obs) {
                 const float r0 = y.x - ob[s][0], r1 = y.y - ob[s][1], r2 = y.z - ob[s][2],
@@ -1436,6 +1573,12 @@ __global__ __launch_bounds__(64) void k_init_state(const gm_dsetup *__restrict__
             if (init_mode == GM_INIT_PER_PIXEL) {
 #pragma unroll
                 for (int f = 0; f < GM_NMODEL; ++f) m[f] = init[p * GM_NMODEL + f];
+                /* the 9-parameter kernels have no macroturbulence or stray
+                 * light: such contexts keep the set-up's mac and alfa */
+                if (!S->ext) {
+                    m[9] = S->init_model[9];
+                    m[10] = S->init_model[10];
+                }
             } else {
 #pragma unroll
                 for (int f = 0; f < GM_NMODEL; ++f) m[f] = S->init_model[f];
@@ -1552,16 +1695,21 @@ void launch_rf_t(const gm_lanecfg &lc, const gm_dsetup *dS, const double *models
                  float *synth, float *rf, int mixed, hipStream_t s)
 {
     const int region4 = (lc.rows * 9 > (LPP / 2) * 15) ? lc.rows * 9 : (LPP / 2) * 15;
-    const size_t shm = (size_t)lc.ppw * region4 * sizeof(float4);
+    const size_t shm = (size_t)lc.ppw * region4 * sizeof(float4) +
+                       (lc.ext ? (size_t)lc.ppw * 2 * LPP * NLPL * sizeof(float) : 0);
     const int g = grid_for(grid_items, lc.ppw, kMaxBlocks);
-    if (mixed)
-        hipLaunchKernelGGL((k_rf<LPP, NLPL, true>), dim3(g), dim3(64), shm, s, dS, models, pre, mfs,
-                           mps, obs, list, count, n_direct, alpha, beta, aes, aps, bes, bps, chi, synth, rf,
-                           lc.rows, lc.padlo);
-    else
-        hipLaunchKernelGGL((k_rf<LPP, NLPL, false>), dim3(g), dim3(64), shm, s, dS, models, pre, mfs,
-                           mps, obs, list, count, n_direct, alpha, beta, aes, aps, bes, bps, chi, synth, rf,
-                           lc.rows, lc.padlo);
+#define GM_RF_CASE(MX, EX)                                                                       \
+    hipLaunchKernelGGL((k_rf<LPP, NLPL, MX, EX>), dim3(g), dim3(64), shm, s, dS, models, pre, mfs, \
+                       mps, obs, list, count, n_direct, alpha, beta, aes, aps, bes, bps, chi, synth,  \
+                       rf, lc.rows, lc.padlo)
+    if (lc.ext) {
+        if (mixed) GM_RF_CASE(true, true);
+        else GM_RF_CASE(false, true);
+    } else {
+        if (mixed) GM_RF_CASE(true, false);
+        else GM_RF_CASE(false, false);
+    }
+#undef GM_RF_CASE
 }
 
 template <int LPP, int NLPL>
@@ -1570,14 +1718,20 @@ void launch_synth_t(const gm_lanecfg &lc, const gm_dsetup *dS, const double *mod
                     const int32_t *count, int64_t n_direct, int64_t grid_items, double *chi,
                     float *synth, int mixed, hipStream_t s)
 {
-    const size_t shm = (size_t)lc.ppw * lc.rows * sizeof(float4);
+    const size_t shm = (size_t)lc.ppw * lc.rows * sizeof(float4) +
+                       (lc.ext ? (size_t)lc.ppw * 2 * LPP * NLPL * sizeof(float) : 0);
     const int g = grid_for(grid_items, lc.ppw, kMaxBlocks);
-    if (mixed)
-        hipLaunchKernelGGL((k_synth<LPP, NLPL, true>), dim3(g), dim3(64), shm, s, dS, models, pre, mfs,
-                           mps, obs, list, count, n_direct, chi, synth, lc.rows, lc.padlo);
-    else
-        hipLaunchKernelGGL((k_synth<LPP, NLPL, false>), dim3(g), dim3(64), shm, s, dS, models,
-                           pre, mfs, mps, obs, list, count, n_direct, chi, synth, lc.rows, lc.padlo);
+#define GM_SYNTH_CASE(MX, EX)                                                                     \
+    hipLaunchKernelGGL((k_synth<LPP, NLPL, MX, EX>), dim3(g), dim3(64), shm, s, dS, models, pre,  \
+                       mfs, mps, obs, list, count, n_direct, chi, synth, lc.rows, lc.padlo)
+    if (lc.ext) {
+        if (mixed) GM_SYNTH_CASE(true, true);
+        else GM_SYNTH_CASE(false, true);
+    } else {
+        if (mixed) GM_SYNTH_CASE(true, false);
+        else GM_SYNTH_CASE(false, false);
+    }
+#undef GM_SYNTH_CASE
 }
 
 } // namespace

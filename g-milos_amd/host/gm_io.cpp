@@ -302,6 +302,39 @@ extern "C" int32_t gmio_read_per(const char *path, int32_t nlambda, float *sp)
     return k == nlambda ? 0 : -2;
 }
 
+extern "C" int32_t gmio_file_readable(const char *path)
+{
+    if (!path || !path[0]) return 0;
+    FILE *fp = std::fopen(path, "r");
+    if (!fp) return 0;
+    std::fclose(fp);
+    return 1;
+}
+
+/* Stray-light profile in .per form (readPerStrayLightFile,
+ * utilsFits.c:1532-1575): the second column must equal the truncated grid
+ * offset of each row.  Returns 0, -1 (no file), -2 (short file) or -3
+ * (offsets differ, where the reference exits).                              */
+extern "C" int32_t gmio_read_stray_per(const char *path, int32_t nlambda, const double *off,
+                                       float *sp)
+{
+    FILE *fp = std::fopen(path, "r");
+    if (!fp) return -1;
+    char line[512];
+    int32_t k = 0, rc = 0;
+    while (k < nlambda && std::fgets(line, sizeof line, fp)) {
+        float a[6];
+        if (std::sscanf(line, "%e %e %e %e %e %e", &a[0], &a[1], &a[2], &a[3], &a[4], &a[5]) != 6)
+            continue;
+        if (a[1] != (float)std::trunc(off[k])) rc = -3;
+        for (int s = 0; s < 4; ++s) sp[s * nlambda + k] = a[2 + s];
+        ++k;
+    }
+    std::fclose(fp);
+    if (rc) return rc;
+    return k == nlambda ? 0 : -2;
+}
+
 /* "%d\t%f\t%e\t%e\t%e\t%e\n" per wavelength (gmilos.cu:589, 885). */
 extern "C" int32_t gmio_write_per(const char *path, int32_t line_index, int32_t nlambda,
                                   const double *off, const float *sp)

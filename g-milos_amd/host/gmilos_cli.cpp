@@ -64,6 +64,10 @@ int main(int argc, char **argv)
     double init[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1.0};
     if (cfg.initial_guess_model[0] && gmio_read_mod(cfg.initial_guess_model, init) != 0)
         return fail("cannot read initial model ", cfg.initial_guess_model);
+    /* gmilos.cu:163-166 */
+    if (init[10] < 1 && !gmio_file_readable(cfg.stray_light_file))
+        return fail("filling factor in the initial model is less than 1 and the stray light "
+                    "file cannot be accessed: ", cfg.stray_light_file);
 
     /* wavelengths: .grid offsets (gmilos.cu:171-196) or a FITS file of
      * absolute wavelengths (gmilos.cu:197-209) */
@@ -134,6 +138,27 @@ int main(int argc, char **argv)
     s.precision = precision;
     s.solver = solver;
     s.rf_reference_rules = exact_rf ? 0 : 1;
+    /* stray light: read for the synthesis mode when the file exists
+     * (gmilos.cu:434-461) and, for inversions, only with a free filling
+     * factor (gmilos.cu:688-716, 916-944), which is not built here */
+    if (cfg.stray_light_file[0] && (cfg.number_of_cycles == 0 || cfg.fix[10])) {
+        const char *se = gmio_file_ext(cfg.stray_light_file);
+        if (!std::strcmp(se, ".per")) {
+            std::vector<float> sl(4 * n);
+            const int rc = gmio_read_stray_per(cfg.stray_light_file, n, off.data(), sl.data());
+            if (rc == -3)
+                return fail("the observed profiles and the stray light profiles do not have the "
+                            "same wavelengths: ", cfg.stray_light_file);
+            if (rc == 0) {
+                s.use_slight = 1;
+                for (int k = 0; k < 4 * n; ++k) s.slight[k] = sl[k];
+            }
+        } else if (!std::strcmp(se, ".fits")) {
+            return fail("stray-light FITS maps are not supported: ", cfg.stray_light_file);
+        }
+    }
+    if (cfg.number_of_cycles != 0 && cfg.fix[10])
+        return fail("a free filling factor (10/11-term inversion) is not supported", "");
 
     gm_ctx *ctx = nullptr;
     gm_status st = gm_ctx_create(&s, &ctx);

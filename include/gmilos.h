@@ -99,7 +99,21 @@ typedef struct {
     int32_t rf_reference_rules;      /* 1 = reference response-function scaling  */
     double  psf_tail_tol;            /* drop PSF taps whose total tail mass is
                                         below this fraction (0 keeps all taps)  */
+    int32_t use_slight;              /* mix a stray-light profile:
+                                        I = alfa I_mag + (1 - alfa) I_stray
+                                        (mil_sinrf.cu:133-140; slight of lm_mils) */
+    float   slight[4 * GM_MAX_LAMBDA]; /* stray light [4][nlambda], I,Q,U,V      */
 } gm_setup;
+
+/* Macroturbulence and stray light.  A context takes the reference's 11-term
+ * physics (circular convolution with the Gaussian macroturbulence kernel for
+ * pixels whose mac exceeds 1e-4, mil_sinrf.cu:101-116 and me_der.cu:119-156;
+ * stray-light mixing with alfa-scaled responses, me_der.cu:191-211) when its
+ * set-up has use_slight, fix[9], or an initial mac above 1e-4.  Other
+ * contexts run the 9-parameter kernels, which take mac = 0 and alfa = 1
+ * whatever a caller's model rows hold.  A free filling factor (fix[10],
+ * the 10- and 11-term solves of lm_mils / lm_mils_11) is not built:
+ * gm_ctx_create returns GM_ERR_UNSUPPORTED for it.                          */
 
 /* Outputs of an inversion (caller-owned device memory; synth and naccs may
  * be NULL).  Same content as the reference's d_vModels, d_vChisqrf,

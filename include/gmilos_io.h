@@ -73,6 +73,11 @@ int32_t gmio_count_lines(const char *path);
 int32_t gmio_read_per(const char *path, int32_t nlambda, float *spectro);
 int32_t gmio_write_per(const char *path, int32_t line_index, int32_t nlambda,
                        const double *offsets_mA, const float *spectro);
+/* Stray-light .per (readPerStrayLightFile, utilsFits.c:1532-1575): as
+ * gmio_read_per, and the offsets column must equal trunc(offsets_mA[k]);
+ * -3 when it does not. */
+int32_t gmio_read_stray_per(const char *path, int32_t nlambda, const double *offsets_mA,
+                            float *spectro);
 
 /* ---- FITS ---------------------------------------------------------------- */
 typedef struct gmio_cube gmio_cube;
@@ -105,6 +110,8 @@ int32_t gmio_fits_read_image(const char *path, int32_t *naxis, int64_t naxes[4],
 int32_t gmio_fits_read_lambda(const char *path, int32_t *line_index, double *lambda,
                               int32_t cap);
 
+/* 1 when the file can be opened for reading (access(F_OK) of the reference) */
+int32_t gmio_file_readable(const char *path);
 /* file_ext / get_basefilename of readConfig.c:2312-2349 */
 const char *gmio_file_ext(const char *path);
 void gmio_base_name(const char *path, char *out, int32_t cap);

@@ -62,6 +62,47 @@ def test_cli_wavelength_fits_file(refdir):
     assert np.abs(got[:, 2:] - ref[:, 2:]).max() < 2e-6
 
 
+def test_cli_synthesis_with_macroturbulence_and_stray_light(refdir, orc):
+    """run/synthesis.mtrol with v_mac = 1.5 km/s, filling factor 0.7 and a
+    .per stray-light profile (gmilos.cu:434-461): the written .per matches
+    the oracle's synthesis (CRISP PSF, circular macroturbulence kernel)."""
+    mod = open(refdir / "run" / "init.mod").read()
+    lines = [("1.5" if "v_mac" in l.lower() else "0.7" if "filling" in l.lower() else None, l)
+             for l in mod.splitlines()]
+    out = []
+    for v, l in lines:
+        if v is None:
+            out.append(l)
+        else:
+            key = l.split(":")[0]
+            out.append("%s:%s" % (key, v))
+    open(refdir / "run" / "macsl.mod", "w").write("\n".join(out) + "\n")
+    off = np.arange(-350.0, 665.0 + 1e-9, 35.0)
+    rng = np.random.default_rng(4)
+    stray = np.zeros((4, 30))
+    stray[0] = 0.9 - 0.3 * np.exp(-((off - 20.0) / 120.0) ** 2)
+    stray[1:] = rng.normal(0, 1e-3, size=(3, 30))
+    with open(refdir / "run" / "stray.per", "w") as f:
+        for k in range(30):
+            f.write("1\t%f\t%e\t%e\t%e\t%e\n" % (off[k], *stray[:, k]))
+    text = open(refdir / "run" / "synthesis.mtrol").read().splitlines()
+    assert text[2].startswith("Stray light file") and text[7].startswith("Initial guess model 1")
+    text[2] = "Stray light file             :run/stray.per     ! stray light"
+    text[7] = "Initial guess model 1     (*):run/macsl.mod !"
+    open(refdir / "run" / "synth_ms.mtrol", "w").write("\n".join(text) + "\n")
+    run(refdir, "run/synth_ms.mtrol")
+    got = np.loadtxt(refdir / "data" / "synthesis_gpu.per")      # <observed profiles>.per
+    m = read_mod(refdir / "run" / "macsl.mod")
+    assert m["mac"] == 1.5 and m["alfa"] == 0.7
+    from conftest import crisp_table
+    dl, val = crisp_table()
+    o = orc.make_setup([(6173.3356, (2, 1, 1.0, 2, 2, 0.0), 1.0)], off, psf=("table", dl, val))
+    orc.set_slight(o, stray)
+    ref = orc.synth(o, model_row(m))[0]
+    np.testing.assert_allclose(got[:, 1], off, atol=1e-6)
+    assert np.abs(got[:, 2:].T - ref).max() < 2e-6
+
+
 def test_cli_inverts_reference_example(refdir):
     run(refdir, "run/invert.mtrol")
     m = read_mod(refdir / "data" / "synthesis_gpu_model.mod")

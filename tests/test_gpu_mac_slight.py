@@ -1,0 +1,142 @@
+"""GPU parity of the macroturbulence / stray-light kernels (the reference's
+11-term physics with mac and alfa fixed: mil_sinrf.cu:101-140,
+me_der.cu:119-211) against the FP64 oracle, with the tolerances of
+test_gpu_parity.py, and the 9-parameter contexts' contract that per-pixel
+mac / alfa are ignored there."""
+import numpy as np
+import pytest
+
+from test_gpu_parity import DP_TYP, _assert_medians, pair, to_dev
+from workload import observations, param_errors
+
+pytestmark = pytest.mark.gpu
+
+GAUSS60 = ("gauss", 60.0)
+
+
+def stray_profile(n=30, seed=3):
+    """A quiet-Sun-like stray-light profile: a broad shallow line in I,
+    small Q, U, V."""
+    rng = np.random.default_rng(seed)
+    x = np.linspace(-1.0, 1.9, n)
+    prof = np.zeros((4, n))
+    prof[0] = 0.95 - 0.35 * np.exp(-((x - 0.0) / 0.35) ** 2)
+    prof[1:] = rng.normal(0, 2e-3, size=(3, n))
+    return prof
+
+
+def ext_pair(gm, orc, psf, mac=1.0, alfa=1.0, slight=None, **kw):
+    p, o = pair(gm, orc, "single", psf, init=dict(mac=mac, alfa=alfa), **kw)
+    if slight is not None:
+        prof = np.asarray(slight, dtype=np.float32).reshape(-1)
+        p.use_slight = 1
+        for k, v in enumerate(prof):
+            p.slight[k] = float(v)
+        orc.set_slight(o, slight)
+    return p, o
+
+
+def mac_models(gm, n, seed, slight=False):
+    m = gm.random_models(n, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    m[:, 9] = rng.uniform(0.0, 3.0, n)
+    m[: n // 4, 9] = rng.choice([0.0, 5e-5, 1e-4], n // 4)     # the 1e-4 switch
+    if slight:
+        m[:, 10] = rng.uniform(0.3, 1.0, n)
+    return m
+
+
+@pytest.mark.parametrize("psf", [None, GAUSS60])
+@pytest.mark.parametrize("with_slight", [False, True])
+def test_ext_synthesis_matches_oracle(gm, orc, cuda, psf, with_slight):
+    L = stray_profile() if with_slight else None
+    p, o = ext_pair(gm, orc, psf, slight=L)
+    models = mac_models(gm, 256, 31, slight=with_slight)
+    ctx = gm.Context(p)
+    got = ctx.synth(to_dev(models, cuda)).cpu().numpy()
+    ref = orc.synth(o, models)
+    ic = (models[:, 7] + models[:, 8])[:, None, None]
+    tol = 5e-6 * ic + 2e-5 * np.abs(ref)
+    assert (np.abs(got - ref) <= tol).all(), np.abs(got - ref).max()
+
+
+@pytest.mark.parametrize("psf", [None, GAUSS60])
+@pytest.mark.parametrize("with_slight", [False, True])
+def test_ext_response_functions_match_oracle(gm, orc, cuda, psf, with_slight):
+    L = stray_profile() if with_slight else None
+    p, o = ext_pair(gm, orc, psf, slight=L, rf_reference_rules=0)
+    models = mac_models(gm, 128, 32, slight=with_slight)
+    ctx = gm.Context(p)
+    sp, rf = ctx.synth_rf(to_dev(models, cuda))
+    rf = rf.cpu().numpy()
+    sp_ref, rf_ref = orc.synth_rf(o, models)
+    np.testing.assert_allclose(sp.cpu().numpy(), sp_ref, atol=2e-5)
+    scale = np.abs(rf_ref).max(axis=-1, keepdims=True)
+    ic = (models[:, 7] + models[:, 8])[:, None, None, None]
+    step = np.array(DP_TYP + (1.0,))[None, None, :, None]
+    tol = np.broadcast_to(5e-4 * scale + 2e-7 * ic / step, rf.shape)
+    bad = np.abs(rf - rf_ref) > tol
+    if bad.any():
+        for i in np.argwhere(bad)[:8]:
+            i = tuple(i)
+            print("mismatch", i, rf[i], rf_ref[i], tol[i], "model", models[i[0]].round(4))
+    assert not bad.any(), np.argwhere(bad)[:5]
+    assert np.all(rf[:, :, 9, :] == 0)          # filling factor fixed: slot 9 zeroed
+
+
+def test_ext_normal_equations_match_oracle(gm, orc, cuda):
+    L = stray_profile()
+    p, o = ext_pair(gm, orc, GAUSS60, slight=L)
+    truth = mac_models(gm, 128, 33, slight=True)
+    obs = observations(orc, o, truth, seed=33)
+    models = mac_models(gm, 128, 34, slight=True)
+    ctx = gm.Context(p)
+    a, b, c = ctx.normal_equations(to_dev(models, cuda), to_dev(obs, cuda))
+    a, b, c = a.cpu().numpy(), b.cpu().numpy(), c.cpu().numpy()
+    sp, rf = orc.synth_rf(o, models)
+    for i in range(len(models)):
+        A, B = orc.covarm(o, obs[i], sp[i], rf[i])
+        A9 = gm.unpack_alpha(a[i])
+        assert np.abs(A9 - A[:9, :9]).max() <= 1e-3 * np.abs(A).max()
+        assert np.abs(b[i] - B[:9]).max() <= 2e-3 * np.abs(B).max()
+        assert c[i] == pytest.approx(orc.chisqr(o, obs[i], sp[i]), rel=2e-5)
+
+
+@pytest.mark.parametrize("mac,alfa,with_slight", [(1.2, 1.0, False), (0.8, 0.75, True)])
+def test_ext_inversion_matches_oracle(gm, orc, cuda, mac, alfa, with_slight):
+    """Fixed macroturbulence (the reference's 10-term run with mac fixed) and
+    a fixed filling factor with stray light, inverted from the classical
+    estimates: medians of |d| against the oracle's inversion."""
+    L = stray_profile() if with_slight else None
+    p, o = ext_pair(gm, orc, GAUSS60, mac=mac, alfa=alfa, slight=L)
+    truth = gm.random_models(512, seed=35)
+    truth[:, 9] = mac
+    truth[:, 10] = alfa
+    obs = observations(orc, o, truth, seed=35)
+    ctx = gm.Context(p)
+    got = {k: v.cpu().numpy() for k, v in ctx.invert(to_dev(obs, cuda)).items()}
+    ref = orc.invert(o, obs)
+    d = param_errors(got["model"], ref["model"])
+    _assert_medians(d, "mac %.1f alfa %.2f" % (mac, alfa))
+    assert np.all(got["model"][:, 9] == mac) and np.all(got["model"][:, 10] == alfa)
+    # accuracy against the truth is that of the plain path
+    dt = param_errors(got["model"], truth)
+    assert np.median(dt[:, 1]) < 20.0
+
+
+def test_plain_context_ignores_per_pixel_mac_alfa(gm, orc, cuda):
+    """A 9-parameter context (no macroturbulence, no stray light) keeps the
+    set-up's mac and alfa for per-pixel start models."""
+    p, o = pair(gm, orc, "single", GAUSS60, init_mode=gm.GM_INIT_PER_PIXEL)
+    truth = gm.random_models(256, seed=36)
+    obs = observations(orc, o, truth, seed=36)
+    start = truth.copy()
+    start[:, 1] += 100.0
+    ctx = gm.Context(p)
+    a = {k: v.cpu().numpy() for k, v in ctx.invert(to_dev(obs, cuda), init=to_dev(start, cuda)).items()}
+    start[:, 9] = 2.0
+    start[:, 10] = 0.5
+    b = {k: v.cpu().numpy() for k, v in ctx.invert(to_dev(obs, cuda), init=to_dev(start, cuda)).items()}
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k])
+    assert np.all(b["model"][:, 9] == 0.0) and np.all(b["model"][:, 10] == 1.0)

@@ -61,6 +61,8 @@ def io():
                                        ctypes.c_int64]
     L.gmio_fits_read_lambda.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32), vp,
                                         ctypes.c_int32]
+    L.gmio_read_stray_per.argtypes = [ctypes.c_char_p, ctypes.c_int32, vp, vp]
+    L.gmio_file_readable.argtypes = [ctypes.c_char_p]
     L.gmio_file_ext.argtypes = [ctypes.c_char_p]
     L.gmio_file_ext.restype = ctypes.c_char_p
     L.gmio_base_name.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int32]
@@ -258,3 +260,23 @@ def test_fits_wavelength_file(io, tmp_path):
     write_fits(path, lam, -64)
     assert io.gmio_fits_read_lambda(path.encode(), ctypes.byref(idx), _p(out), 64) < 0
     assert io.gmio_fits_read_lambda(b"missing.fits", ctypes.byref(idx), _p(out), 64) < 0
+
+
+def test_stray_light_per(io, tmp_path):
+    """readPerStrayLightFile (utilsFits.c:1532-1575): four Stokes columns, the
+    offsets column checked against the truncated grid offsets."""
+    off = np.arange(-350.0, 665.0 + 1e-9, 35.0)
+    sp = np.arange(120, dtype=np.float32).reshape(4, 30) / 100.0
+    path = str(tmp_path / "s.per")
+    with open(path, "w") as f:
+        for k in range(30):
+            f.write("1\t%f\t%e\t%e\t%e\t%e\n" % (off[k], *sp[:, k]))
+    got = np.zeros((4, 30), dtype=np.float32)
+    assert io.gmio_read_stray_per(path.encode(), 30, _p(off), _p(got)) == 0
+    np.testing.assert_array_equal(got, sp)
+    shifted = off + 1.0
+    assert io.gmio_read_stray_per(path.encode(), 30, _p(shifted), _p(got)) == -3
+    assert io.gmio_read_stray_per(path.encode(), 31, _p(np.zeros(31)), _p(np.zeros((4, 31), np.float32))) < 0
+    assert io.gmio_read_stray_per(b"missing.per", 30, _p(off), _p(got)) == -1
+    assert io.gmio_file_readable(path.encode()) == 1
+    assert io.gmio_file_readable(b"missing.per") == 0 and io.gmio_file_readable(b"") == 0
