@@ -271,6 +271,14 @@ static double end_max(const double *x, int n)
  * rf_reference_rules = 1 reproduces the reference's magneto-optical Voigt
  * derivative scale (me_der.cu:71-74 store w*dF where shi = 2*w*F, i.e. half
  * the derivative of shi); 0 gives the exact derivative.                     */
+/* NTERMS of the reference kernel the set-up corresponds to: lm_mils_11
+ * (11) runs when macroturbulence and filling factor are both free
+ * (gmilos.cu:1234-1239), lm_mils (10) otherwise.                           */
+int gmo_nterms(const gmo_setup *s)
+{
+    return s->nterms == 11 ? 11 : 10;
+}
+
 static void synth_core(const gmo_setup *s, const double *m, double *spectra,
                        double *rf, int alpha_rf_mixed)
 {
@@ -293,8 +301,7 @@ static void synth_core(const gmo_setup *s, const double *m, double *spectra,
     double shi_scale = s->rf_reference_rules ? 1.0 : 2.0;
 
     double I[GMO_MAXL] = {0}, Q[GMO_MAXL] = {0}, U[GMO_MAXL] = {0}, V[GMO_MAXL] = {0};
-    /* rf layout while building: rfl[p][stokes][l] */
-    static const int NT = GMO_NT;
+    const int NT = gmo_nterms(s);
 
     for (int l = 0; l < n; l++) {
         /* FO-weighted profiles per component type t = 0 pi, 1 blue, 2 red */
@@ -442,7 +449,8 @@ static void synth_core(const gmo_setup *s, const double *m, double *spectra,
         rf[3 * NT * n + 7 * n + l] = 0;
         rf[0 * NT * n + 6 * n + l] = 0;
         rf[3 * NT * n + 6 * n + l] = 0;
-        for (int st = 0; st < 4; st++) rf[st * NT * n + 9 * n + l] = 0;
+        for (int st = 0; st < 4; st++)
+            for (int p = 9; p < NT; p++) rf[st * NT * n + p * n + l] = 0;
     }
 
     /* macroturbulence (mac > 1e-4): circular convolution with the Gaussian
@@ -494,7 +502,8 @@ static void synth_core(const gmo_setup *s, const double *m, double *spectra,
     }
     /* stray light: I = alfa I_mag + (1 - alfa) I_stray for every Stokes
      * parameter (mil_sinrf.cu:133-140); every response is scaled by alfa and
-     * slot 9 becomes d/dalfa = I_mag - I_stray (me_der.cu:191-211).  The
+     * the last slot (9 for NTERMS 10, 10 for 11) becomes d/dalfa =
+     * I_mag - I_stray (me_der.cu:191-211).  The
      * reference's LM loop passes the mixed spectrum as "I_mag" after the
      * first evaluation (milosUtils.cu:673), which gives alfa (I_mag - I_stray);
      * alpha_rf_mixed reproduces that.                                       */
@@ -509,15 +518,16 @@ static void synth_core(const gmo_setup *s, const double *m, double *spectra,
                 for (int p = 0; p < NT; p++)
                     for (int l = 0; l < n; l++) {
                         double *r = rf + st * NT * n + p * n + l;
-                        *r = (p == 9) ? base[st * n + l] - s->slight[st * n + l] : *r * al;
+                        *r = (p == NT - 1) ? base[st * n + l] - s->slight[st * n + l] : *r * al;
                     }
         }
     }
-    /* zero the RFs of fixed parameters: milosUtils.cu:243-271 (NTERMS=10,
-     * slot 9 follows the filling-factor flag fix[10]).                      */
+    /* zero the RFs of fixed parameters: milosUtils.cu:243-271 (NTERMS=10:
+     * slot 9 follows the filling-factor flag fix[10]; 11: slot p follows
+     * fix[p]).                                                              */
     if (rf) {
         for (int p = 0; p < NT; p++) {
-            int free_p = (p < 9) ? s->fix[p] : s->fix[10];
+            int free_p = (p < 9 || NT == 11) ? s->fix[p] : s->fix[10];
             if (free_p) continue;
             for (int st = 0; st < 4; st++)
                 for (int l = 0; l < n; l++) rf[st * NT * n + p * n + l] = 0;
@@ -547,8 +557,8 @@ void gmo_synth_rf2(const gmo_setup *s, const double *model, double *spectra,
 void gmo_covarm(const gmo_setup *s, const float *obs, const double *synth,
                 const double *rf, double *alpha, double *beta)
 {
-    const int n = s->nlambda, NT = GMO_NT;
-    double AP[4][GMO_NT * GMO_NT], BT[4][GMO_NT];
+    const int n = s->nlambda, NT = gmo_nterms(s);
+    double AP[4][11 * 11], BT[4][11];
     for (int j = 0; j < 4; j++) {
         double w = s->weight[j], wd = s->weight[j] / s->noise;
         for (int i = 0; i < NT; i++)
@@ -676,7 +686,9 @@ void gmo_apply_delta(const gmo_setup *s, const double *model,
     if (fix[7]) out[GMO_S0] = model[GMO_S0] - delta[7];
     if (fix[8]) out[GMO_S1] = model[GMO_S1] - delta[8];
     if (fix[9]) out[GMO_MAC] = model[GMO_MAC] - delta[9];
-    if (fix[10]) out[GMO_ALFA] = model[GMO_ALFA] - delta[9];
+    /* AplicaDelta (11 terms, milosUtils.cu:30-91) / AplicaDeltaf (10,
+     * milosUtils.cu:131-138) */
+    if (fix[10]) out[GMO_ALFA] = model[GMO_ALFA] - delta[gmo_nterms(s) == 11 ? 10 : 9];
 }
 
 /* Physical ranges, milosUtils.cu:160-238. */
@@ -822,12 +834,12 @@ int gmo_lm_pixel(const gmo_setup *s, const float *obs, const double *init,
                  double *model_out, double *chisqr_out, int *iter_out,
                  float *synth_out, int *nacc_out)
 {
-    const int n = s->nlambda, NT = GMO_NT;
+    const int n = s->nlambda, NT = gmo_nterms(s);
     const int nfree = (n * GMO_NPARMS) - NT;
     double best[GMO_NMODEL], model[GMO_NMODEL];
     double synth[4 * GMO_MAXL];
     double *rf = (double *)malloc(sizeof(double) * 4 * NT * n);
-    double alpha[GMO_NT * GMO_NT], covar[GMO_NT * GMO_NT], beta[GMO_NT], delta[GMO_NT];
+    double alpha[11 * 11], covar[11 * 11], beta[11], delta[11];
     if (!rf) return -1;
 
     if (init) {
@@ -850,7 +862,7 @@ int gmo_lm_pixel(const gmo_setup *s, const float *obs, const double *init,
     double flambda = s->ilambda, factor = 1.0;
     int iter = 0, clanda = 0, nacc = 0;
     do {
-        memcpy(covar, alpha, sizeof covar);
+        memcpy(covar, alpha, sizeof(double) * NT * NT);
         for (int j = 0; j < NT; j++) covar[j * (NT + 1)] = alpha[j * (NT + 1)] * (1.0 + flambda);
         gmo_svd_solve(covar, beta, NT, delta, NULL);
         gmo_apply_delta(s, best, delta, model);

@@ -58,6 +58,8 @@ typedef struct {
     int     rf_reference_rules;      /* 1: reproduce reference me_der rules  */
     int     use_slight;              /* stray-light profile mixed in          */
     double  slight[4 * GMO_MAXL];    /* stray light, I,Q,U,V blocks of n      */
+    int     nterms;                  /* reference NTERMS: 10 (lm_mils, also 0)
+                                        or 11 (lm_mils_11: mac and alfa free) */
 } gmo_setup;
 
 /* ---- atomic / PSF set-up ---------------------------------------------- */
@@ -80,7 +82,8 @@ void gmo_conv_circ(double *x, int n, const double *h);
 void gmo_mac_kernels(const gmo_setup *s, double mac, double *G, double *Gd);
 /* synthesis: spectra[4*n] (I block, Q, U, V) */
 void gmo_synth(const gmo_setup *s, const double *model, double *spectra);
-/* synthesis + response functions rf[s*GMO_NT*n + p*n + l] */
+/* synthesis + response functions rf[s*NT*n + p*n + l], NT = nterms (10
+ * unless the set-up selects the 11-term layout) */
 void gmo_synth_rf(const gmo_setup *s, const double *model, double *spectra,
                   double *rf);
 /* as gmo_synth_rf; alpha_rf_mixed = 1 takes the filling-factor response
@@ -92,6 +95,7 @@ void gmo_covarm(const gmo_setup *s, const float *obs, const double *synth,
                 const double *rf, double *alpha, double *beta);
 double gmo_chisqr(const gmo_setup *s, const float *obs, const double *synth,
                   int nfree);
+int  gmo_nterms(const gmo_setup *s);
 /* pseudo-inverse solve of a symmetric n x n system via an SVD (n <= 11) */
 void gmo_svd_solve(const double *A, const double *b, int n, double *x,
                    double *w_out);

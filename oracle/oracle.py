@@ -42,7 +42,8 @@ class gmo_setup(ctypes.Structure):
                 ("logclambda", ctypes.c_int), ("fix", ctypes.c_int * 11),
                 ("init_model", ctypes.c_double * NMODEL), ("use_ce", ctypes.c_int),
                 ("ce_line", ctypes.c_int), ("rf_reference_rules", ctypes.c_int),
-                ("use_slight", ctypes.c_int), ("slight", ctypes.c_double * (4 * MAXL))]
+                ("use_slight", ctypes.c_int), ("slight", ctypes.c_double * (4 * MAXL)),
+                ("nterms", ctypes.c_int)]
 
 
 _lib = None
@@ -69,6 +70,8 @@ def lib():
         L.gmo_conv_circ.argtypes = [vp, i, vp]
         L.gmo_mac_kernels.argtypes = [P(gmo_setup), d, vp, vp]
         L.gmo_synth_rf2.argtypes = [P(gmo_setup), vp, vp, vp, i]
+        L.gmo_nterms.argtypes = [P(gmo_setup)]
+        L.gmo_nterms.restype = i
         L.gmo_synth.argtypes = [P(gmo_setup), vp, vp]
         L.gmo_synth_rf.argtypes = [P(gmo_setup), vp, vp, vp]
         L.gmo_covarm.argtypes = [P(gmo_setup), vp, vp, vp, vp, vp]
@@ -120,6 +123,12 @@ def voigt(a, u):
     H, F = ctypes.c_double(), ctypes.c_double()
     lib().gmo_voigt(float(a), float(u), ctypes.byref(H), ctypes.byref(F))
     return H.value, F.value
+
+
+def nterms(s):
+    """NTERMS of the reference kernel: 11 (lm_mils_11) when mac and alfa are
+    both free, else 10 (gmilos.cu:1234-1239)."""
+    return lib().gmo_nterms(ctypes.byref(s))
 
 
 def conv_circ(x, h):
@@ -223,6 +232,7 @@ def make_setup(lines, offsets_mA, psf=None, init=None, free=None, miter=50, nois
     s.use_ce = int(use_ce)
     s.ce_line = int(ce_line)
     s.rf_reference_rules = int(rf_reference_rules)
+    s.nterms = 11 if (s.fix[9] and s.fix[10]) else 10
     return s
 
 
@@ -242,8 +252,9 @@ def synth(s, models):
 def synth_rf(s, models, alpha_rf_mixed=0):
     models = np.ascontiguousarray(np.atleast_2d(models), dtype=np.float64)
     n = s.nlambda
+    nt = nterms(s)
     sp = np.zeros((models.shape[0], 4, n))
-    rf = np.zeros((models.shape[0], 4, NT, n))
+    rf = np.zeros((models.shape[0], 4, nt, n))
     for p in range(models.shape[0]):
         lib().gmo_synth_rf2(ctypes.byref(s), _p(models[p]), _p(sp[p]), _p(rf[p]),
                             int(alpha_rf_mixed))
@@ -262,16 +273,17 @@ def covarm(s, obs, synth_, rf):
     obs = np.ascontiguousarray(obs, dtype=np.float32)
     synth_ = np.ascontiguousarray(synth_, dtype=np.float64)
     rf = np.ascontiguousarray(rf, dtype=np.float64)
-    alpha = np.zeros(NT * NT)
-    beta = np.zeros(NT)
+    nt = nterms(s)
+    alpha = np.zeros(nt * nt)
+    beta = np.zeros(nt)
     lib().gmo_covarm(ctypes.byref(s), _p(obs), _p(synth_), _p(rf), _p(alpha), _p(beta))
-    return alpha.reshape(NT, NT), beta
+    return alpha.reshape(nt, nt), beta
 
 
 def chisqr(s, obs, synth_, nfree=None):
     obs = np.ascontiguousarray(obs, dtype=np.float32)
     synth_ = np.ascontiguousarray(synth_, dtype=np.float64)
-    nfree = nfree if nfree is not None else 4 * s.nlambda - NT
+    nfree = nfree if nfree is not None else 4 * s.nlambda - nterms(s)
     return lib().gmo_chisqr(ctypes.byref(s), _p(obs), _p(synth_), nfree)
 
 

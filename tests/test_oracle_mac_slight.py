@@ -147,3 +147,49 @@ def test_stray_light_mixing_and_responses(orc):
     s_fix = orc.set_slight(setup(orc, psf=("gauss", 60.0)), L)
     _, rf_fix = orc.synth_rf(s_fix, m)
     assert np.abs(rf_fix[0][:, 9]).max() == 0
+
+
+def test_eleven_term_layout_finite_difference(orc):
+    """lm_mils_11 layout (mac and alfa free): slot 9 is d/dmac scaled by alfa,
+    slot 10 d/dalfa (me_der.cu:191-207), against central differences of the
+    mixed synthesis."""
+    L = np.zeros((4, 30))
+    L[0] = 0.9 - 0.3 * np.exp(-((GRID - 20.0) / 120.0) ** 2)
+    s = orc.set_slight(setup(orc, psf=("gauss", 60.0), rf_reference_rules=0, free=FIELDS), L)
+    assert orc.nterms(s) == 11
+    m = _model(mac=1.4, alfa=0.7)
+    sp, rf = orc.synth_rf(s, m)
+    assert rf.shape == (1, 4, 11, 30)
+    for p in range(11):
+        h = max(abs(m[p]) * 1e-6, 1e-7)
+        mp, mm = m.copy(), m.copy()
+        mp[p] += h
+        mm[p] -= h
+        fd = (orc.synth(s, mp)[0] - orc.synth(s, mm)[0]) / (2 * h)
+        err = np.abs(fd - rf[0][:, p, :]).max() / np.abs(fd).max()
+        assert err < 1e-3, (p, err)
+    A, b = orc.covarm(s, sp[0].astype(np.float32), sp[0], rf[0])
+    assert A.shape == (11, 11) and b.shape == (11,)
+
+
+@pytest.mark.parametrize("free", [FIELDS[:9] + ["alfa"], FIELDS])
+def test_filling_factor_lm_recovers_truth(orc, free):
+    """Noise-free closed loop with stray light: the 10-term run (filling
+    factor free, mac fixed at its true value) and the 11-term run (both
+    free) recover the synthesising model."""
+    L = np.zeros((4, 30))
+    L[0] = 0.95 - 0.4 * np.exp(-((GRID - 40.0) / 110.0) ** 2)
+    truth = _model(eta0=9, B=900, vlos=0.4, dopp=0.032, aa=0.15, gm=65, az=35, S0=0.3,
+                   S1=0.65, mac=1.2, alfa=0.75)
+    init = dict(eta0=8, B=400, vlos=0.25, dopp=0.034, aa=0.06, gm=30, az=30, S0=0.15,
+                S1=0.85, mac=1.2 if "mac" not in free else 0.8, alfa=0.9)
+    s = orc.set_slight(setup(orc, psf=("gauss", 60.0), free=free, init=init, miter=200,
+                             use_ce=1), L)
+    assert orc.nterms(s) == (11 if "mac" in free else 10)
+    obs = orc.synth(s, truth)[0].astype(np.float32)
+    r = orc.lm_pixel(s, obs)
+    assert r["chisqr"] < 1e-3
+    np.testing.assert_allclose(r["model"][[1, 5, 6, 10]], truth[[1, 5, 6, 10]], rtol=2e-3,
+                               atol=0.05)
+    if "mac" in free:
+        assert abs(r["model"][9] - 1.2) < 0.02
