@@ -408,6 +408,11 @@ class Context:
         _check(self.lib.gm_ctx_create(ctypes.byref(setup), ctypes.byref(h)), "gm_ctx_create")
         self.h = h
         self.nlambda = setup.nlambda
+        # a free filling factor solves eleven slots (mac, alfa in 9, 10); the
+        # reference layout of the responses has NTERMS = 11 only when mac is
+        # free too (lm_mils_11), else 10
+        self.np = 11 if setup.fix[10] else 9
+        self.nterms = 11 if (setup.fix[9] and setup.fix[10]) else 10
 
     def close(self):
         if self.h:
@@ -473,7 +478,8 @@ class Context:
         import torch
         npix = models.shape[0]
         sp = torch.empty((npix, 4, self.nlambda), dtype=torch.float32, device=models.device)
-        rf = torch.empty((npix, 4, 10, self.nlambda), dtype=torch.float32, device=models.device)
+        rf = torch.empty((npix, 4, self.nterms, self.nlambda), dtype=torch.float32,
+                         device=models.device)
         _check(self.lib.gm_synth_rf(self.h, ctypes.c_void_p(models.data_ptr()), npix,
                                     ctypes.c_void_p(sp.data_ptr()), ctypes.c_void_p(rf.data_ptr()),
                                     self._stream()), "gm_synth_rf")
@@ -483,8 +489,9 @@ class Context:
         import torch
         npix = models.shape[0]
         dev = models.device
-        a = torch.empty((npix, 45), dtype=torch.float32, device=dev)
-        b = torch.empty((npix, 9), dtype=torch.float32, device=dev)
+        na = self.np * (self.np + 1) // 2
+        a = torch.empty((npix, na), dtype=torch.float32, device=dev)
+        b = torch.empty((npix, self.np), dtype=torch.float32, device=dev)
         c = torch.empty(npix, dtype=torch.float64, device=dev)
         _check(self.lib.gm_normal_equations(self.h, ctypes.c_void_p(models.data_ptr()),
                                             ctypes.c_void_p(obs.data_ptr()), npix,
@@ -497,7 +504,7 @@ class Context:
     def step_solve(self, alpha, beta, flambda):
         import torch
         npix = alpha.shape[0]
-        d = torch.empty((npix, 9), dtype=torch.float64, device=alpha.device)
+        d = torch.empty((npix, self.np), dtype=torch.float64, device=alpha.device)
         _check(self.lib.gm_step_solve(self.h, ctypes.c_void_p(alpha.data_ptr()),
                                       ctypes.c_void_p(beta.data_ptr()),
                                       ctypes.c_void_p(flambda.data_ptr()), npix,
@@ -736,21 +743,22 @@ class Server:
             pass
 
 
-def packed_index(p, q):
-    """Index of (p, q) in the packed 9x9 upper triangle."""
+def packed_index(p, q, n=9):
+    """Index of (p, q) in the packed n x n upper triangle (n = 9 or 11)."""
     if p > q:
         p, q = q, p
-    return p * 9 - (p * (p - 1)) // 2 + (q - p)
+    return p * n - (p * (p - 1)) // 2 + (q - p)
 
 
 def unpack_alpha(a):
-    """[..., 45] packed upper triangle -> [..., 9, 9] symmetric."""
+    """[..., 45] or [..., 66] packed upper triangle -> [..., n, n] symmetric."""
     a = np.asarray(a)
-    out = np.zeros(a.shape[:-1] + (9, 9), dtype=a.dtype)
-    for p in range(9):
-        for q in range(p, 9):
-            out[..., p, q] = a[..., packed_index(p, q)]
-            out[..., q, p] = a[..., packed_index(p, q)]
+    n = 11 if a.shape[-1] == 66 else 9
+    out = np.zeros(a.shape[:-1] + (n, n), dtype=a.dtype)
+    for p in range(n):
+        for q in range(p, n):
+            out[..., p, q] = a[..., packed_index(p, q, n)]
+            out[..., q, p] = a[..., packed_index(p, q, n)]
     return out
 
 

@@ -86,6 +86,7 @@ gm_lanecfg gm_lane_config(const gm_dsetup &h)
     lc.padlo = padlo;
     lc.rows = lc.lpp * lc.nlpl + padlo + padhi;
     lc.ext = h.ext;
+    lc.np = h.np;
     return lc;
 }
 
@@ -101,8 +102,9 @@ static gm_status build_dsetup(const gm_setup &s, gm_dsetup &d)
     if (s.nlines < 1 || s.nlines > GM_MAX_LINES) return GM_ERR_INVALID;
     if (!(s.noise > 0)) return GM_ERR_INVALID;
     if (s.miter < 0 || s.miter > kMaxRounds) return GM_ERR_INVALID;
-    /* a free filling factor needs the 10-/11-term solve (lm_mils_11) */
-    if (s.fix[10]) return GM_ERR_UNSUPPORTED;
+    /* a free filling factor needs the stray light it is the fraction of
+     * (the reference reads the profile only then, gmilos.cu:688-716) */
+    if (s.fix[10] && !s.use_slight) return GM_ERR_INVALID;
     if (s.use_slight)
         for (int k = 0; k < 4 * n; ++k)
             if (!std::isfinite(s.slight[k])) return GM_ERR_INVALID;
@@ -185,6 +187,16 @@ static gm_status build_dsetup(const gm_setup &s, gm_dsetup &d)
     for (int k = 0; k < n; ++k) d.lambda[k] = s.lambda[k];
     d.use_slight = s.use_slight ? 1 : 0;
     d.ext = (d.use_slight || s.fix[9] || s.init_model[9] > 0.0001) ? 1 : 0;
+    /* free filling factor: eleven solved slots; the reference runs
+     * lm_mils_11 when mac is free too, else the 10-term lm_mils
+     * (gmilos.cu:1234-1239), which sets nfree and the RF layout */
+    d.np = s.fix[10] ? 11 : 9;
+    d.nt_ref = (s.fix[9] && s.fix[10]) ? 11 : 10;
+    d.nfree_chi = 4 * n - d.nt_ref;
+    if (d.np == 11) {
+        if (d.nt_ref == 11) d.freemask |= (1u << 9);
+        d.freemask |= (1u << 10);
+    }
     d.mac_l0 = s.lambda0[0];
     for (int k = 0; k < n; ++k) d.psf_full[k] = d.use_psf ? (float)s.psf[k] : 0.f;
     if (d.use_slight)
@@ -283,7 +295,8 @@ static gm_status ensure_state(gm_ctx *c, int64_t npix)
     if (npix <= c->st.cap) return GM_OK;
     int64_t cap = ((npix + 255) / 256) * 256;
     const size_t per = sizeof(double) * (2 * GM_NMODEL + 5) +
-                       sizeof(float) * (GM_ALPHA_N + 9 + 2 * GM_NPRE) + sizeof(int32_t) * (2 + 4);
+                       sizeof(float) * (GM_ALPHA_MAX + GM_BETA_MAX + 2 * GM_NPRE) +
+                       sizeof(int32_t) * (2 + 4);
     const size_t bytes = per * (size_t)cap + 256 * 22 + 64 + sizeof(int32_t) * 3 * (kMaxRounds + 2) + 256;
     if (c->ws) {
         hipDeviceSynchronize();
@@ -303,8 +316,8 @@ static gm_status ensure_state(gm_ctx *c, int64_t npix)
     st.cap = cap;
     st.best = (double *)take(sizeof(double) * GM_NMODEL * cap);
     st.trial = (double *)take(sizeof(double) * GM_NMODEL * cap);
-    st.alpha = (float *)take(sizeof(float) * GM_ALPHA_N * cap);
-    st.beta = (float *)take(sizeof(float) * 9 * cap);
+    st.alpha = (float *)take(sizeof(float) * GM_ALPHA_MAX * cap);
+    st.beta = (float *)take(sizeof(float) * GM_BETA_MAX * cap);
     st.pre_best = (float *)take(sizeof(float) * GM_NPRE * cap);
     st.pre_trial = (float *)take(sizeof(float) * GM_NPRE * cap);
     st.chi_trial = (double *)take(sizeof(double) * cap);
@@ -389,7 +402,7 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
         const int nxt = 1 - cur;
         kbegin(2);
         gm_launch_solve(c->dS, st, st.list[cur], &cnt[cur], npix, mixed, solver, &cnt[nxt],
-                        &cnt[2], hist + 3 * r, st.list[3], &cnt[3], s);
+                        &cnt[2], hist + 3 * r, st.list[3], &cnt[3], s, lc.np);
         kend();
         kbegin(0);
         gm_launch_synth(lc, c->dS, st.trial, st.pre_trial, st.cap, obs, st.list[cur], &cnt[cur], 0,
@@ -400,8 +413,10 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
                          st.list[2], &cnt[2], &cnt[3], nullptr, nullptr, mixed, s);
         kend();
         kbegin(1);
+        /* alpha_first = 0: the filling-factor response of the reference's
+         * loop (milosUtils.cu:673) */
         gm_launch_rf(lc, c->dS, st.best, st.pre_best, st.cap, obs, st.list[2], &cnt[2], 0, npix, st.alpha,
-                     st.beta, st.cap, nullptr, nullptr, nullptr, mixed, s);
+                     st.beta, st.cap, nullptr, nullptr, nullptr, mixed, s, 0);
         kend();
         cur = nxt;
     }
@@ -487,7 +502,7 @@ extern "C" gm_status gm_step_solve(gm_ctx *c, const float *alpha, const float *b
     if (npix == 0) return GM_OK;
     if (!alpha || !beta || !flambda || !delta) return GM_ERR_INVALID;
     gm_launch_step_solve(alpha, beta, flambda, npix, delta, c->setup.precision == GM_PREC_MIXED,
-                         c->setup.solver, (hipStream_t)stream);
+                         c->setup.solver, (hipStream_t)stream, c->lc.np);
     return hip_ok(hipGetLastError());
 }
 

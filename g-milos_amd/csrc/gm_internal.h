@@ -10,6 +10,8 @@
 #include "gmilos.h"
 
 #define GM_ALPHA_N   45          /* packed upper triangle of the 9x9 system */
+#define GM_ALPHA_MAX 66          /* ... of the 11x11 system (free filling factor) */
+#define GM_BETA_MAX  11
 #define GM_NPRE      19          /* per-pixel FP32 synthesis constants      */
 #define GM_RED_N     55          /* 45 alpha + 9 beta + 1 chi^2             */
 #define GM_SLOTS     36          /* staged values per wavelength: per Stokes
@@ -48,6 +50,10 @@ struct gm_dsetup {
     double  lambda[GM_MAX_LAMBDA];           /* absolute wavelengths (A)    */
     /* macroturbulence / stray light (kernels built with EXT only) */
     int32_t ext;                             /* 11-term physics active       */
+    int32_t np;                              /* solved slots: 9, or 11 with a
+                                                free filling factor (slot 9
+                                                mac, slot 10 alfa)          */
+    int32_t nt_ref;                          /* reference NTERMS: 10 or 11   */
     int32_t use_slight;
     double  mac_l0;                          /* lambda0 of the first line    */
     float   psf_full[GM_MAX_LAMBDA];         /* all nlambda PSF taps         */
@@ -96,6 +102,7 @@ struct gm_lanecfg {
     int rows;  /* staged rows per pixel incl. zero padding                   */
     int padlo; /* zero rows before wavelength 0                              */
     int ext;   /* macroturbulence / stray-light kernels                      */
+    int np;    /* 9 or 11 solved slots                                       */
 };
 
 gm_lanecfg gm_lane_config(const gm_dsetup &h);
@@ -108,7 +115,8 @@ void gm_launch_rf(const gm_lanecfg &lc, const gm_dsetup *dS, const double *model
                   const float *pre, int64_t model_stride, const float *obs, const int32_t *list,
                   const int32_t *count, int64_t n_direct, int64_t grid_items,
                   float *alpha, float *beta, int64_t ab_stride, double *chi_out,
-                  float *synth_out, float *rf_out, int mixed, hipStream_t st);
+                  float *synth_out, float *rf_out, int mixed, hipStream_t st,
+                  int alpha_first = 1);
 void gm_launch_synth(const gm_lanecfg &lc, const gm_dsetup *dS, const double *models,
                      const float *pre, int64_t model_stride, const float *obs, const int32_t *list,
                      const int32_t *count, int64_t n_direct, int64_t grid_items,
@@ -116,10 +124,10 @@ void gm_launch_synth(const gm_lanecfg &lc, const gm_dsetup *dS, const double *mo
 void gm_launch_solve(const gm_dsetup *dS, gm_state st, const int32_t *list,
                      const int32_t *count, int64_t grid_items, int mixed, int solver,
                      int32_t *reset0, int32_t *reset1, int32_t *hist,
-                     int32_t *fb_list, int32_t *fb_count, hipStream_t stream);
+                     int32_t *fb_list, int32_t *fb_count, hipStream_t stream, int np = 9);
 void gm_launch_step_solve(const float *alpha, const float *beta, const double *flambda,
                           int64_t n, double *delta, int mixed, int solver,
-                          hipStream_t stream);
+                          hipStream_t stream, int np = 9);
 void gm_launch_decide(const gm_dsetup *dS, gm_state st, const int32_t *list,
                       const int32_t *count, int64_t grid_items, int32_t *next_list,
                       int32_t *next_count, int32_t *acc_list, int32_t *acc_count,

@@ -359,27 +359,46 @@ __device__ __forceinline__ T group_sum(T v)
  * (direct_convolution_double, mil_sinrf.cu:104-107).  Called by every lane
  * of the workgroup (two barriers); gk, gc are the pixel group's LDS buffers.
  * gc receives the combined kernel; returns its sum.                         */
-template <int LPP, int NLPL>
+template <int LPP, int NLPL, bool DER = false>
 __device__ float mac_kernel(const gm_dsetup *__restrict__ S, double mac, float *gk, float *gc,
-                            int k0)
+                            int k0, float *gdk = nullptr, float *gd = nullptr)
 {
     const int n = S->nlambda;
-    const double ild = S->mac_l0 * (mac > 1e-4 ? mac : 1e-4) / 2.99792458e5;
+    const double macv = mac > 1e-4 ? mac : 1e-4;
+    const double ild = S->mac_l0 * macv / 2.99792458e5;
     const double lc = S->lambda[n / 2];
-    double g[NLPL], part = 0.0;
+    double g[NLPL], x2[NLPL], part = 0.0;
 #pragma unroll
     for (int s = 0; s < NLPL; ++s) {
         const int k = k0 + s * LPP;
         double v = 0.0;
+        x2[s] = 0.0;
         if (k < n) {
             const double x = (S->lambda[k] - lc) / ild;
-            const double t = 0.5 * (x * x);
+            x2[s] = x * x;
+            const double t = 0.5 * x2[s];
             v = (t < 1e30) ? exp(-t) : 0.0;
         }
         g[s] = v;
         part += v;
     }
     const double sum = group_sum<double, LPP>(part);
+    if constexpr (DER) {
+        /* d/dmac of the normalised kernel: G (x^2 - 1) / mac as
+         * me_der.cu:124-131, or G (x^2 - sum G x^2) / mac exactly */
+        double m2 = 1.0;
+        if (S->shi_scale != 1.0f) {
+            double pm = 0.0;
+#pragma unroll
+            for (int s = 0; s < NLPL; ++s) pm += g[s] / sum * x2[s];
+            m2 = group_sum<double, LPP>(pm);
+        }
+#pragma unroll
+        for (int s = 0; s < NLPL; ++s) {
+            const int k = k0 + s * LPP;
+            if (k < n) gdk[k] = (float)(g[s] / sum / macv * (x2[s] - m2));
+        }
+    }
 #pragma unroll
     for (int s = 0; s < NLPL; ++s) {
         const int k = k0 + s * LPP;
@@ -392,16 +411,21 @@ __device__ float mac_kernel(const gm_dsetup *__restrict__ S, double mac, float *
     for (int s = 0; s < NLPL; ++s) {
         const int k = k0 + s * LPP;
         if (k < n) {
-            float v = 0.f;
+            float v = 0.f, vd = 0.f;
             if (S->use_psf) {
                 for (int j = 0; j < n; ++j) {
                     const int i = k + j - c;
-                    if (i >= 0 && i < n) v += S->psf_full[j] * gk[i];
+                    if (i >= 0 && i < n) {
+                        v += S->psf_full[j] * gk[i];
+                        if constexpr (DER) vd += S->psf_full[j] * gdk[i];
+                    }
                 }
             } else {
                 v = gk[k];
+                if constexpr (DER) vd = gdk[k];
             }
             gc[k] = v;
+            if constexpr (DER) gd[k] = vd;
             tot += v;
         }
     }
@@ -460,17 +484,25 @@ __device__ __forceinline__ LmStep<T> lm_step(const gm_dsetup *__restrict__ S, T 
 /* ------------------------------------------------------------------------ */
 /* Synthesis + response functions + PSF + normal equations + chi^2 for a list
  * of pixels; wavelength-parallel mapping.                                   */
-template <int LPP, int NLPL, bool MIXED, bool EXT>
+template <int LPP, int NLPL, bool MIXED, int XM>
 __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
     const gm_dsetup *__restrict__ S, const double *__restrict__ models,
     const float *__restrict__ pre, int64_t mfs, int64_t mps, const float *__restrict__ obs,
     const int32_t *__restrict__ list, const int32_t *__restrict__ count,
     int64_t n_direct, float *__restrict__ alpha, float *__restrict__ beta,
     int64_t aes, int64_t aps, int64_t bes, int64_t bps, double *__restrict__ chi_out,
-    float *__restrict__ synth_out, float *__restrict__ rf_out, int rows, int padlo)
+    float *__restrict__ synth_out, float *__restrict__ rf_out, int rows, int padlo,
+    int alpha_first)
 {
+    /* XM: 0 the 9-parameter kernel, 1 with macroturbulence / stray light,
+     * 2 the eleven-slot system (slot 9 mac, slot 10 filling factor) */
+    constexpr bool EXT = XM >= 1;
+    constexpr bool P11 = XM == 2;
+    constexpr int NP = P11 ? 11 : 9;
+    constexpr int NA = NP * (NP + 1) / 2;
+    constexpr int NRED = NA + NP + 1;            /* alpha, beta, chi^2         */
     constexpr int PPW = 64 / LPP;
-    constexpr int RSTR = 60;                   /* reduction row stride (floats) */
+    constexpr int RSTR = P11 ? 80 : 60;        /* reduction row stride (floats) */
     extern __shared__ float4 lds4[];
     const int lane = threadIdx.x;
     const int h = lane / LPP;
@@ -481,8 +513,10 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
     float4 *conv = lds4 + h * region4;
     float *red = reinterpret_cast<float *>(conv);
     /* EXT: macroturbulence kernel buffers after the PPW regions */
-    float *gk = reinterpret_cast<float *>(lds4 + PPW * region4) + h * 2 * (LPP * NLPL);
+    float *gk = reinterpret_cast<float *>(lds4 + PPW * region4) + h * (P11 ? 4 : 2) * (LPP * NLPL);
     float *gc = gk + LPP * NLPL;
+    float *gdk = gc + LPP * NLPL;                /* P11: d/dmac kernel          */
+    float *gd = gdk + LPP * NLPL;
     const int use_psf = S->use_psf;
     const int dlo = S->psf_dlo, dhi = S->psf_dhi;
     const uint32_t fm = S->freemask;
@@ -546,20 +580,21 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
             const double mac = models[9 * mfs + pix * mps];
             alv = (float)models[10 * mfs + pix * mps];
             mac_on = mac > 0.0001;
-            gsum = mac_kernel<LPP, NLPL>(S, mac, gk, gc, k0);
+            gsum = mac_kernel<LPP, NLPL, P11>(S, mac, gk, gc, k0, gdk, gd);
         }
         const float s0c = (EXT && mac_on) ? gsum : 1.f;
         const float s0rf = (EXT && S->use_slight) ? alv : 1.f;
         const float inv_s1 = 1.f / pp.S1;
 
-        float acc[GM_RED_N];
+        float acc[NRED];
 #pragma unroll
-        for (int e = 0; e < GM_RED_N; ++e) acc[e] = 0.f;
+        for (int e = 0; e < NRED; ++e) acc[e] = 0.f;
         double chi = 0.0;
 #pragma unroll
         for (int s = 0; s < NLPL; ++s) {
             const int k = k0 + s * LPP;
             float z[32];
+            float zm[4] = {0.f, 0.f, 0.f, 0.f};  /* P11: d/dmac per Stokes */
             if (EXT && mac_on) {
                 /* macroturbulence: circular correlation with the combined
                  * kernel, no Ic rule (me_der.cu:143-152) */
@@ -576,6 +611,15 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
                         z[4 * q + 1] += tap * x.y;
                         z[4 * q + 2] += tap * x.z;
                         z[4 * q + 3] += tap * x.w;
+                    }
+                    if (P11) {
+                        /* unconvolved spectra (staged positions 0, 7, 15,
+                         * 23) with the kernel derivative, me_der.cu:139-141 */
+                        const float td = gd[j];
+                        zm[0] += td * row[0].x;
+                        zm[1] += td * row[1].w;
+                        zm[2] += td * row[3].w;
+                        zm[3] += td * row[5].w;
                     }
                 }
             } else if (use_psf) {
@@ -630,16 +674,24 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
             y[26] = y[18] * inv_s1;
             y[35] = y[27] * inv_s1;
             if (k < n) {
+                float ra[4] = {0.f, 0.f, 0.f, 0.f};  /* P11: d/dalfa per Stokes */
                 if (EXT && S->use_slight) {
-                    /* stray light (mil_sinrf.cu:133-140, me_der.cu:191-199) */
+                    /* stray light (mil_sinrf.cu:133-140, me_der.cu:191-207):
+                     * d/dalfa from the magnetic spectrum at the first
+                     * evaluation and from the mixed one afterwards, as the
+                     * reference's LM loop passes it (milosUtils.cu:673) */
 #pragma unroll
                     for (int st = 0; st < 4; ++st) {
-                        y[st * 9] = alv * y[st * 9] + (1.f - alv) * S->slight[st * n + k];
+                        const float ls = S->slight[st * n + k];
+                        const float pure = y[st * 9];
+                        y[st * 9] = alv * pure + (1.f - alv) * ls;
+                        if (P11) ra[st] = (alpha_first ? pure : y[st * 9]) - ls;
 #pragma unroll
                         for (int p = 1; p < 9; ++p) y[st * 9 + p] *= alv;
+                        if (P11) zm[st] *= alv;
                     }
                 }
-                float g[4][9];
+                float g[4][NP];
                 float r[4];
 #pragma unroll
                 for (int st = 0; st < 4; ++st) {
@@ -650,6 +702,10 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
                         float gv = (js >= 0) ? y[st * 9 + js] : (st == 0 ? s0rf : 0.f);
                         g[st][p] = ((fm >> p) & 1u) ? gv : 0.f;
                     }
+                    if constexpr (P11) {
+                        g[st][9] = ((fm >> 9) & 1u) ? zm[st] : 0.f;
+                        g[st][10] = ((fm >> 10) & 1u) ? ra[st] : 0.f;
+                    }
                 }
 #pragma unroll
                 for (int st = 0; st < 4 && obs; ++st) {
@@ -657,16 +713,16 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
                     if (MIXED)
                         chi += (double)ws * ((double)r[st] * (double)r[st]);
                     else
-                        acc[54] += (r[st] * r[st]) * ws;
+                        acc[NRED - 1] += (r[st] * r[st]) * ws;
                     int e = 0;
 #pragma unroll
-                    for (int p = 0; p < 9; ++p) {
+                    for (int p = 0; p < NP; ++p) {
                         const float wg = ws * g[st][p];
 #pragma unroll
-                        for (int q = p; q < 9; ++q) acc[e++] += wg * g[st][q];
+                        for (int q = p; q < NP; ++q) acc[e++] += wg * g[st][q];
                     }
 #pragma unroll
-                    for (int p = 0; p < 9; ++p) acc[45 + p] += (ws * r[st]) * g[st][p];
+                    for (int p = 0; p < NP; ++p) acc[NA + p] += (ws * r[st]) * g[st][p];
                 }
                 if (synth_out && valid) {
                     float *so = synth_out + pix * 4 * n + k;
@@ -674,11 +730,26 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
                     for (int st = 0; st < 4; ++st) so[st * n] = y[st * 9];
                 }
                 if (rf_out && valid) {
-                    float *ro = rf_out + pix * 4 * 10 * n + k;
+                    if constexpr (P11) {
+                        /* reference layout [4][NTERMS][n]: NTERMS 10 keeps
+                         * the filling factor in slot 9, 11 has mac in 9 and
+                         * alfa in 10 */
+                        const int nto = S->nt_ref;
+                        float *ro = rf_out + pix * 4 * nto * n + k;
 #pragma unroll
-                    for (int st = 0; st < 4; ++st)
+                        for (int st = 0; st < 4; ++st) {
 #pragma unroll
-                        for (int p = 0; p < 10; ++p) ro[(st * 10 + p) * n] = (p < 9) ? g[st][p] : 0.f;
+                            for (int p = 0; p < 9; ++p) ro[(st * nto + p) * n] = g[st][p];
+                            ro[(st * nto + 9) * n] = (nto == 11) ? g[st][9] : g[st][10];
+                            if (nto == 11) ro[(st * nto + 10) * n] = g[st][10];
+                        }
+                    } else {
+                        float *ro = rf_out + pix * 4 * 10 * n + k;
+#pragma unroll
+                        for (int st = 0; st < 4; ++st)
+#pragma unroll
+                            for (int p = 0; p < 10; ++p) ro[(st * 10 + p) * n] = (p < 9) ? g[st][p] : 0.f;
+                    }
                 }
             }
         }
@@ -686,27 +757,29 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
          * the pixel's lanes onto the lower half with lane swaps, then sum the
          * HL remaining rows through LDS (aliasing the staged rows) */
         constexpr int HL = LPP / 2;
-        const float chi_part = acc[54];          /* FP32 chi^2 term of this lane */
+        const float chi_part = acc[NRED - 1];    /* FP32 chi^2 term of this lane */
 #pragma unroll
-        for (int e = 0; e < GM_RED_N; ++e) acc[e] += __shfl_xor(acc[e], HL, 64);
+        for (int e = 0; e < NRED; ++e) acc[e] += __shfl_xor(acc[e], HL, 64);
         __syncthreads();
         if (k0 < HL) {
             float4 *rrow = reinterpret_cast<float4 *>(red + k0 * RSTR);
 #pragma unroll
-            for (int j = 0; j < 14; ++j)
-                rrow[j] = make_float4(acc[4 * j], acc[4 * j + 1], acc[4 * j + 2],
-                                      (4 * j + 3 < GM_RED_N) ? acc[4 * j + 3] : 0.f);
+            for (int j = 0; j < (NRED + 3) / 4; ++j)
+                rrow[j] = make_float4((4 * j < NRED) ? acc[4 * j] : 0.f,
+                                      (4 * j + 1 < NRED) ? acc[4 * j + 1] : 0.f,
+                                      (4 * j + 2 < NRED) ? acc[4 * j + 2] : 0.f,
+                                      (4 * j + 3 < NRED) ? acc[4 * j + 3] : 0.f);
         }
         __syncthreads();
-        for (int e = k0; e < GM_RED_N - 1; e += LPP) {
+        for (int e = k0; e < NRED - 1; e += LPP) {
             float sum = 0.f;
 #pragma unroll 8
             for (int l = 0; l < HL; ++l) sum += red[l * RSTR + e];
             if (valid) {
-                if (e < 45) {
+                if (e < NA) {
                     if (alpha) alpha[e * aes + pix * aps] = sum;
                 } else {
-                    if (beta) beta[(e - 45) * bes + pix * bps] = sum;
+                    if (beta) beta[(e - NA) * bes + pix * bps] = sum;
                 }
             }
         }
@@ -1193,9 +1266,9 @@ __device__ __forceinline__ void load_damped(const float *__restrict__ alpha,
 }
 
 /* AplicaDelta + check (milosUtils.cu:30-238), double model. */
-template <typename T>
+template <typename T, int N = 9>
 __device__ __forceinline__ void apply_delta_check(const gm_dsetup *__restrict__ S,
-                                                  const double *best, const T (&d)[9],
+                                                  const double *best, const T (&d)[N],
                                                   double *m)
 {
     const int32_t *fix = S->fix;
@@ -1221,6 +1294,14 @@ __device__ __forceinline__ void apply_delta_check(const gm_dsetup *__restrict__ 
     }
     if (fix[7]) m[7] = best[7] - d[7];
     if (fix[8]) m[8] = best[8] - d[8];
+    if constexpr (N == 11) {
+        /* 11 slots: mac in 9, filling factor in 10 (AplicaDelta,
+         * milosUtils.cu:67-71; the 10-term AplicaDeltaf takes the filling
+         * factor from its slot 9, milosUtils.cu:131-138, which is slot 10
+         * here) */
+        if (fix[9]) m[9] = best[9] - d[9];
+        if (fix[10]) m[10] = best[10] - d[10];
+    }
     /* check(): milosUtils.cu:160-238 */
     if (m[1] < 0) { m[1] = -m[1]; m[5] = 180.0 - m[5]; }
     if (m[1] > 5000) m[1] = 5000;
@@ -1253,6 +1334,271 @@ __device__ __forceinline__ void apply_to_trial(const gm_dsetup *__restrict__ S, 
 #pragma unroll
     for (int f = 0; f < GM_NMODEL; ++f) st.trial[f * cap + pix] = m[f];
     store_pre(st.pre_trial, cap, pix, make_pre(m));
+}
+
+/* ---- eleven-slot systems (free filling factor) ----------------------------
+ * Slots 0-8 as above, 9 = mac, 10 = filling factor.  The reference solves
+ * them as its 10-term (lm_mils, alfa free) or 11-term (lm_mils_11, mac and
+ * alfa free) system; a slot of a fixed parameter holds a zero row, which the
+ * pseudo-inverse drops, so both are one 11-slot solve here.               */
+template <int N>
+__host__ __device__ constexpr int utn(int i, int j)
+{
+    return (i <= j) ? (i * N - (i * (i - 1)) / 2 + (j - i))
+                    : (j * N - (j * (j - 1)) / 2 + (i - j));
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void load_damped_n(const float *__restrict__ alpha,
+                                              const float *__restrict__ beta, int64_t es,
+                                              T flambda, T (&a)[N * (N + 1) / 2], T (&b)[N])
+{
+#pragma unroll
+    for (int e = 0; e < N * (N + 1) / 2; ++e) a[e] = (T)alpha[e * es];
+#pragma unroll
+    for (int p = 0; p < N; ++p) {
+        b[p] = (T)beta[p * es];
+        a[utn<N>(p, p)] = a[utn<N>(p, p)] * (T(1) + flambda);
+    }
+}
+
+/* LDL^T with the trace(A^-1) < 1e12 certificate (see solve_ldl), N slots. */
+template <typename T, int N>
+__device__ __forceinline__ bool solve_ldl_n(const T (&a)[N * (N + 1) / 2], const T (&b)[N],
+                                            T (&x)[N])
+{
+    T L[N * (N + 1) / 2], d[N], id[N];
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const T ajj = a[utn<N>(j, j)];
+        T dj = ajj;
+#pragma unroll
+        for (int k = 0; k < j; ++k) dj -= L[utn<N>(j, k)] * L[utn<N>(j, k)] * d[k];
+        const bool null_row = (ajj == T(0));
+        const bool pos = dj > T(0);
+        ok = ok && (null_row || pos);
+        const bool use = pos && !null_row;
+        d[j] = use ? dj : T(0);
+        id[j] = use ? T(1) / dj : T(0);
+#pragma unroll
+        for (int i = j + 1; i < N; ++i) {
+            T lij = a[utn<N>(i, j)];
+#pragma unroll
+            for (int k = 0; k < j; ++k) lij -= L[utn<N>(i, k)] * L[utn<N>(j, k)] * d[k];
+            L[utn<N>(i, j)] = lij * id[j];
+        }
+    }
+    T z[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        T zi = b[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) zi -= L[utn<N>(i, k)] * z[k];
+        z[i] = zi;
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) z[i] *= id[i];
+#pragma unroll
+    for (int i = N - 1; i >= 0; --i) {
+        T xi = z[i];
+#pragma unroll
+        for (int k = i + 1; k < N; ++k) xi -= L[utn<N>(k, i)] * x[k];
+        x[i] = xi;
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+#pragma unroll
+        for (int i = j + 1; i < N; ++i) {
+            T mij = L[utn<N>(i, j)];
+#pragma unroll
+            for (int k = j + 1; k < i; ++k) mij += L[utn<N>(i, k)] * L[utn<N>(k, j)];
+            L[utn<N>(i, j)] = -mij;
+        }
+    T tr = T(0);
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        T row = T(1);
+#pragma unroll
+        for (int k = 0; k < j; ++k) row += L[utn<N>(j, k)] * L[utn<N>(j, k)];
+        tr += id[j] * row;
+    }
+    return ok && (tr < T(1e12));
+}
+
+/* Cyclic Jacobi eigen-decomposition in round-robin order for odd N (step K:
+ * pairs ((K+i) mod N, (K-i) mod N), i = 1..(N-1)/2, index K waits), then the
+ * pseudo-inverse x = V diag(1/w, |w| > 1e-12) V^T b (milosUtils.cu:296-322).
+ * Same scheme as jacobi_step / solve_svd; used for the pixels whose LDL^T
+ * certificate fails.                                                       */
+template <typename T, int N, int K>
+__device__ __forceinline__ void jacobi_step_n(T (&a)[N * (N + 1) / 2], T (&v)[N * N])
+{
+    constexpr int H = (N - 1) / 2;
+    T c[H], sn[H], t[H];
+    int P[H], Q[H];
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        P[i] = (K + 1 + i) % N;
+        Q[i] = (K + N - 1 - i) % N;
+    }
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        const T app = a[utn<N>(P[i], P[i])], aqq = a[utn<N>(Q[i], Q[i])];
+        const T apq = a[utn<N>(P[i], Q[i])];
+        const T dd = aqq - app;
+        const T den = fabs(dd) + sqrt(dd * dd + T(4) * apq * apq);
+        T ti = (den > T(0)) ? (T(2) * apq) / den : T(0);
+        ti = (dd < T(0)) ? -ti : ti;
+        t[i] = ti;
+        c[i] = rsq<T>(T(1) + ti * ti);
+        sn[i] = ti * c[i];
+    }
+#pragma unroll
+    for (int i = 0; i < H; ++i)
+#pragma unroll
+        for (int j = i + 1; j < H; ++j) {
+            const T m00 = a[utn<N>(P[i], P[j])], m01 = a[utn<N>(P[i], Q[j])];
+            const T m10 = a[utn<N>(Q[i], P[j])], m11 = a[utn<N>(Q[i], Q[j])];
+            const T n00 = c[j] * m00 - sn[j] * m01, n01 = sn[j] * m00 + c[j] * m01;
+            const T n10 = c[j] * m10 - sn[j] * m11, n11 = sn[j] * m10 + c[j] * m11;
+            a[utn<N>(P[i], P[j])] = c[i] * n00 - sn[i] * n10;
+            a[utn<N>(P[i], Q[j])] = c[i] * n01 - sn[i] * n11;
+            a[utn<N>(Q[i], P[j])] = sn[i] * n00 + c[i] * n10;
+            a[utn<N>(Q[i], Q[j])] = sn[i] * n01 + c[i] * n11;
+        }
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        const T akp = a[utn<N>(K, P[i])], akq = a[utn<N>(K, Q[i])];
+        a[utn<N>(K, P[i])] = c[i] * akp - sn[i] * akq;
+        a[utn<N>(K, Q[i])] = sn[i] * akp + c[i] * akq;
+    }
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        const T apq = a[utn<N>(P[i], Q[i])];
+        a[utn<N>(P[i], P[i])] -= t[i] * apq;
+        a[utn<N>(Q[i], Q[i])] += t[i] * apq;
+        a[utn<N>(P[i], Q[i])] = T(0);
+    }
+#pragma unroll
+    for (int r = 0; r < N; ++r)
+#pragma unroll
+        for (int i = 0; i < H; ++i) {
+            const T vp = v[r * N + P[i]], vq = v[r * N + Q[i]];
+            v[r * N + P[i]] = c[i] * vp - sn[i] * vq;
+            v[r * N + Q[i]] = sn[i] * vp + c[i] * vq;
+        }
+}
+
+template <typename T, int N, int K>
+struct JacobiSweep {
+    static __device__ __forceinline__ void run(T (&a)[N * (N + 1) / 2], T (&v)[N * N])
+    {
+        jacobi_step_n<T, N, K>(a, v);
+        JacobiSweep<T, N, K + 1>::run(a, v);
+    }
+};
+template <typename T, int N>
+struct JacobiSweep<T, N, N> {
+    static __device__ __forceinline__ void run(T (&)[N * (N + 1) / 2], T (&)[N * N]) {}
+};
+
+template <typename T, int N, int MAXSWEEP>
+__device__ __forceinline__ void solve_svd_n(T (&a)[N * (N + 1) / 2], const T (&b)[N], T (&x)[N])
+{
+    T v[N * N];
+#pragma unroll
+    for (int i = 0; i < N * N; ++i) v[i] = (i % (N + 1) == 0) ? T(1) : T(0);
+    const T eps2 = (sizeof(T) == 8) ? T(1e-30) : T(1e-14);
+    for (int sweep = 0; sweep < MAXSWEEP; ++sweep) {
+        T off = 0, dg = 0;
+#pragma unroll
+        for (int p = 0; p < N; ++p) {
+            dg += a[utn<N>(p, p)] * a[utn<N>(p, p)];
+#pragma unroll
+            for (int q = p + 1; q < N; ++q) off += a[utn<N>(p, q)] * a[utn<N>(p, q)];
+        }
+        if (off <= eps2 * dg) break;
+        JacobiSweep<T, N, 0>::run(a, v);
+    }
+    T aux[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        T sum = 0;
+#pragma unroll
+        for (int k = 0; k < N; ++k) sum += b[k] * v[k * N + j];
+        const T w = fabs(a[utn<N>(j, j)]);
+        aux[j] = (w > T(1e-12)) ? sum / w : T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        T sum = 0;
+#pragma unroll
+        for (int k = 0; k < N; ++k) sum += v[i * N + k] * aux[k];
+        x[i] = sum;
+    }
+}
+
+/* Step solve of the eleven-slot system for the active list, FP64 (the
+ * reference's 11-term solve is FP64 svdcmp): LDL^T where it provably equals
+ * the pseudo-inverse, the Jacobi SVD otherwise, then AplicaDelta + check. */
+__global__ __launch_bounds__(64) void k_solve11(const gm_dsetup *__restrict__ S, gm_state st,
+                                                const int32_t *__restrict__ list,
+                                                const int32_t *__restrict__ count,
+                                                int32_t *__restrict__ reset0,
+                                                int32_t *__restrict__ reset1,
+                                                int32_t *__restrict__ hist)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (hist) {
+            hist[0] = *count;
+            hist[1] = 0;
+            if (reset1) hist[-1] = *reset1;
+        }
+        if (reset0) *reset0 = 0;
+        if (reset1) *reset1 = 0;
+    }
+    const int64_t cnt = *count;
+    const int64_t cap = st.cap;
+    for (int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x; i < cnt;
+         i += (int64_t)gridDim.x * 64) {
+        const int64_t pix = list[i];
+        double a[66], b[11], x[11];
+        load_damped_n<double, 11>(st.alpha + pix, st.beta + pix, cap, st.flambda[pix], a, b);
+        double a2[66];
+#pragma unroll
+        for (int e = 0; e < 66; ++e) a2[e] = a[e];
+        if (!solve_ldl_n<double, 11>(a, b, x)) solve_svd_n<double, 11, 14>(a2, b, x);
+        double best[GM_NMODEL], m[GM_NMODEL];
+#pragma unroll
+        for (int f = 0; f < GM_NMODEL; ++f) {
+            best[f] = st.best[f * cap + pix];
+            m[f] = st.trial[f * cap + pix];
+        }
+        apply_delta_check<double, 11>(S, best, x, m);
+#pragma unroll
+        for (int f = 0; f < GM_NMODEL; ++f) st.trial[f * cap + pix] = m[f];
+        store_pre(st.pre_trial, cap, pix, make_pre(m));
+    }
+}
+
+/* Piece entry: delta [n][11] from alpha [n][66], beta [n][11]. */
+__global__ __launch_bounds__(64) void k_step_solve11(const float *__restrict__ alpha,
+                                                     const float *__restrict__ beta,
+                                                     const double *__restrict__ flambda,
+                                                     int64_t n, double *__restrict__ delta)
+{
+    for (int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * 64) {
+        double a[66], b[11], x[11];
+        load_damped_n<double, 11>(alpha + i * 66, beta + i * 11, 1, flambda[i], a, b);
+        double a2[66];
+#pragma unroll
+        for (int e = 0; e < 66; ++e) a2[e] = a[e];
+        if (!solve_ldl_n<double, 11>(a, b, x)) solve_svd_n<double, 11, 14>(a2, b, x);
+#pragma unroll
+        for (int p = 0; p < 11; ++p) delta[i * 11 + p] = x[p];
+    }
 }
 
 /* Step solve for the active list.  SOLVER = GM_SOLVE_SVD: LDL^T where it
@@ -1692,22 +2038,27 @@ void launch_rf_t(const gm_lanecfg &lc, const gm_dsetup *dS, const double *models
                  const float *pre, int64_t mfs, int64_t mps, const float *obs, const int32_t *list,
                  const int32_t *count, int64_t n_direct, int64_t grid_items, float *alpha,
                  float *beta, int64_t aes, int64_t aps, int64_t bes, int64_t bps, double *chi,
-                 float *synth, float *rf, int mixed, hipStream_t s)
+                 float *synth, float *rf, int mixed, hipStream_t s, int alpha_first)
 {
-    const int region4 = (lc.rows * 9 > (LPP / 2) * 15) ? lc.rows * 9 : (LPP / 2) * 15;
+    const int xm = lc.np == 11 ? 2 : (lc.ext ? 1 : 0);
+    const int red4 = (LPP / 2) * (xm == 2 ? 20 : 15);
+    const int region4 = (lc.rows * 9 > red4) ? lc.rows * 9 : red4;
     const size_t shm = (size_t)lc.ppw * region4 * sizeof(float4) +
-                       (lc.ext ? (size_t)lc.ppw * 2 * LPP * NLPL * sizeof(float) : 0);
+                       (size_t)lc.ppw * (xm == 2 ? 4 : (xm == 1 ? 2 : 0)) * LPP * NLPL * sizeof(float);
     const int g = grid_for(grid_items, lc.ppw, kMaxBlocks);
-#define GM_RF_CASE(MX, EX)                                                                       \
-    hipLaunchKernelGGL((k_rf<LPP, NLPL, MX, EX>), dim3(g), dim3(64), shm, s, dS, models, pre, mfs, \
+#define GM_RF_CASE(MX, X)                                                                        \
+    hipLaunchKernelGGL((k_rf<LPP, NLPL, MX, X>), dim3(g), dim3(64), shm, s, dS, models, pre, mfs,  \
                        mps, obs, list, count, n_direct, alpha, beta, aes, aps, bes, bps, chi, synth,  \
-                       rf, lc.rows, lc.padlo)
-    if (lc.ext) {
-        if (mixed) GM_RF_CASE(true, true);
-        else GM_RF_CASE(false, true);
+                       rf, lc.rows, lc.padlo, alpha_first)
+    if (xm == 2) {
+        if (mixed) GM_RF_CASE(true, 2);
+        else GM_RF_CASE(false, 2);
+    } else if (xm == 1) {
+        if (mixed) GM_RF_CASE(true, 1);
+        else GM_RF_CASE(false, 1);
     } else {
-        if (mixed) GM_RF_CASE(true, false);
-        else GM_RF_CASE(false, false);
+        if (mixed) GM_RF_CASE(true, 0);
+        else GM_RF_CASE(false, 0);
     }
 #undef GM_RF_CASE
 }
@@ -1748,28 +2099,30 @@ void gm_launch_rf(const gm_lanecfg &lc, const gm_dsetup *dS, const double *model
                   const float *pre, int64_t model_stride, const float *obs, const int32_t *list,
                   const int32_t *count, int64_t n_direct, int64_t grid_items, float *alpha,
                   float *beta, int64_t ab_stride, double *chi_out, float *synth_out,
-                  float *rf_out, int mixed, hipStream_t st)
+                  float *rf_out, int mixed, hipStream_t st, int alpha_first)
 {
     int64_t mfs, mps;
     split_strides(model_stride, mfs, mps);
     /* alpha/beta: SoA over the state capacity (entry stride ab_stride, pixel
-     * stride 1) or, with ab_stride 0, the caller's AoS [npix][45] / [npix][9] */
+     * stride 1) or, with ab_stride 0, the caller's AoS [npix][45] / [npix][9]
+     * ([npix][66] / [npix][11] for eleven slots) */
+    const int np = lc.np == 11 ? 11 : 9;
     const int64_t aes = ab_stride > 0 ? ab_stride : 1;
-    const int64_t aps = ab_stride > 0 ? 1 : 45;
+    const int64_t aps = ab_stride > 0 ? 1 : np * (np + 1) / 2;
     const int64_t bes = ab_stride > 0 ? ab_stride : 1;
-    const int64_t bps = ab_stride > 0 ? 1 : 9;
+    const int64_t bps = ab_stride > 0 ? 1 : np;
     if (lc.lpp == 32 && lc.nlpl == 1)
         launch_rf_t<32, 1>(lc, dS, models, pre, mfs, mps, obs, list, count, n_direct, grid_items, alpha,
-                           beta, aes, aps, bes, bps, chi_out, synth_out, rf_out, mixed, st);
+                           beta, aes, aps, bes, bps, chi_out, synth_out, rf_out, mixed, st, alpha_first);
     else if (lc.lpp == 64 && lc.nlpl == 1)
         launch_rf_t<64, 1>(lc, dS, models, pre, mfs, mps, obs, list, count, n_direct, grid_items, alpha,
-                           beta, aes, aps, bes, bps, chi_out, synth_out, rf_out, mixed, st);
+                           beta, aes, aps, bes, bps, chi_out, synth_out, rf_out, mixed, st, alpha_first);
     else if (lc.lpp == 64 && lc.nlpl == 2)
         launch_rf_t<64, 2>(lc, dS, models, pre, mfs, mps, obs, list, count, n_direct, grid_items, alpha,
-                           beta, aes, aps, bes, bps, chi_out, synth_out, rf_out, mixed, st);
+                           beta, aes, aps, bes, bps, chi_out, synth_out, rf_out, mixed, st, alpha_first);
     else
         launch_rf_t<64, 4>(lc, dS, models, pre, mfs, mps, obs, list, count, n_direct, grid_items, alpha,
-                           beta, aes, aps, bes, bps, chi_out, synth_out, rf_out, mixed, st);
+                           beta, aes, aps, bes, bps, chi_out, synth_out, rf_out, mixed, st, alpha_first);
 }
 
 void gm_launch_synth(const gm_lanecfg &lc, const gm_dsetup *dS, const double *models,
@@ -1796,9 +2149,15 @@ void gm_launch_synth(const gm_lanecfg &lc, const gm_dsetup *dS, const double *mo
 void gm_launch_solve(const gm_dsetup *dS, gm_state st, const int32_t *list,
                      const int32_t *count, int64_t grid_items, int mixed, int solver,
                      int32_t *reset0, int32_t *reset1, int32_t *hist, int32_t *fb_list,
-                     int32_t *fb_count, hipStream_t s)
+                     int32_t *fb_count, hipStream_t s, int np)
 {
     const int g = grid_for(grid_items, 64, kMaxBlocks);
+    if (np == 11) {
+        /* eleven slots: FP64 LDL^T / Jacobi in one kernel, no fallback list */
+        hipLaunchKernelGGL(k_solve11, dim3(g), dim3(64), 0, s, dS, st, list, count, reset0, reset1,
+                           hist);
+        return;
+    }
 #define GM_SOLVE_CASE(MX, SV)                                                                    \
     hipLaunchKernelGGL((k_solve<MX, SV>), dim3(g), dim3(64), 0, s, dS, st, list, count, reset0,  \
                        reset1, hist, fb_list, fb_count)
@@ -1824,9 +2183,14 @@ void gm_launch_solve(const gm_dsetup *dS, gm_state st, const int32_t *list,
 }
 
 void gm_launch_step_solve(const float *alpha, const float *beta, const double *flambda,
-                          int64_t n, double *delta, int mixed, int solver, hipStream_t s)
+                          int64_t n, double *delta, int mixed, int solver, hipStream_t s,
+                          int np)
 {
     const int g = grid_for(n, 64, kMaxBlocks);
+    if (np == 11) {
+        hipLaunchKernelGGL(k_step_solve11, dim3(g), dim3(64), 0, s, alpha, beta, flambda, n, delta);
+        return;
+    }
 #define GM_STEP_CASE(MX, SV)                                                                     \
     hipLaunchKernelGGL((k_step_solve<MX, SV>), dim3(g), dim3(64), 0, s, alpha, beta, flambda, n, delta)
     if (mixed) {

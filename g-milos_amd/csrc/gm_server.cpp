@@ -177,7 +177,7 @@ static gm_status build_engine(gm_server *srv, const gm_setup *setup, Engine **ou
     e->max_tiles = srv->cfg.max_tiles;
     const int64_t nl4 = 4 * (int64_t)e->nl;
     e->bytes_per_px = (int64_t)(sizeof(double) * (2 * GM_NMODEL + 5) +
-                                sizeof(float) * (GM_ALPHA_N + 9 + 2 * GM_NPRE) +
+                                sizeof(float) * (GM_ALPHA_MAX + GM_BETA_MAX + 2 * GM_NPRE) +
                                 sizeof(int32_t) * (2 + 4 + 3) + sizeof(float) * 2 * nl4);
     int64_t cap = srv->cfg.pool_pixels;
     if (cap <= 0) cap = std::min<int64_t>(srv->cfg.hbm_budget_bytes / e->bytes_per_px, (int64_t)1 << 22);
@@ -187,7 +187,7 @@ static gm_status build_engine(gm_server *srv, const gm_setup *setup, Engine **ou
     auto rnd = [](size_t b) { return (b + 255) & ~(size_t)255; };
     size_t bytes = 0;
     bytes += rnd(sizeof(double) * GM_NMODEL * cap) * 2;
-    bytes += rnd(sizeof(float) * GM_ALPHA_N * cap) + rnd(sizeof(float) * 9 * cap);
+    bytes += rnd(sizeof(float) * GM_ALPHA_MAX * cap) + rnd(sizeof(float) * GM_BETA_MAX * cap);
     bytes += rnd(sizeof(float) * GM_NPRE * cap) * 2;
     bytes += rnd(sizeof(double) * cap) * 5 + rnd(sizeof(int32_t) * cap) * 2;
     bytes += rnd(sizeof(int32_t) * cap) * 4;                       /* lists      */
@@ -202,8 +202,8 @@ static gm_status build_engine(gm_server *srv, const gm_setup *setup, Engine **ou
     s.cap = cap;
     s.best = (double *)take(sizeof(double) * GM_NMODEL * cap);
     s.trial = (double *)take(sizeof(double) * GM_NMODEL * cap);
-    s.alpha = (float *)take(sizeof(float) * GM_ALPHA_N * cap);
-    s.beta = (float *)take(sizeof(float) * 9 * cap);
+    s.alpha = (float *)take(sizeof(float) * GM_ALPHA_MAX * cap);
+    s.beta = (float *)take(sizeof(float) * GM_BETA_MAX * cap);
     s.pre_best = (float *)take(sizeof(float) * GM_NPRE * cap);
     s.pre_trial = (float *)take(sizeof(float) * GM_NPRE * cap);
     s.chi_trial = (double *)take(sizeof(double) * cap);
@@ -297,13 +297,13 @@ static void launch_round(Engine &e)
     gm_launch_set_count(&cnt[5], 0, s);
     gm_launch_set_count(e.completed_dev, 0, s);
     gm_launch_solve(e.dS, st, st.list[cur], &cnt[cur], grid, e.mixed, e.solver, &cnt[nxt], &cnt[2],
-                    nullptr, st.list[3], &cnt[3], s);
+                    nullptr, st.list[3], &cnt[3], s, e.lc.np);
     gm_launch_synth(e.lc, e.dS, st.trial, st.pre_trial, st.cap, e.obs, st.list[cur], &cnt[cur], 0, grid,
                     st.chi_trial, e.synth, e.mixed, s);
     gm_launch_decide(e.dS, st, st.list[cur], &cnt[cur], grid, st.list[nxt], &cnt[nxt], st.list[2],
                      &cnt[2], &cnt[3], e.done, &cnt[5], e.mixed, s);
     gm_launch_rf(e.lc, e.dS, st.best, st.pre_best, st.cap, e.obs, st.list[2], &cnt[2], 0, grid, st.alpha,
-                 st.beta, st.cap, nullptr, nullptr, nullptr, e.mixed, s);
+                 st.beta, st.cap, nullptr, nullptr, nullptr, e.mixed, s, 0);
     gm_launch_retire(st, e.done, &cnt[5], grid, e.synth, 4 * e.nl, e.slot_tile, e.tiles_dev,
                      e.completed_dev, s);
     (void)hipMemcpyAsync(e.completed_host[slot], e.completed_dev, sizeof(int32_t) * (e.max_tiles + 1),

@@ -140,7 +140,7 @@ int main(int argc, char **argv)
     s.rf_reference_rules = exact_rf ? 0 : 1;
     /* stray light: read for the synthesis mode when the file exists
      * (gmilos.cu:434-461) and, for inversions, only with a free filling
-     * factor (gmilos.cu:688-716, 916-944), which is not built here */
+     * factor (gmilos.cu:688-716, 916-944) */
     if (cfg.stray_light_file[0] && (cfg.number_of_cycles == 0 || cfg.fix[10])) {
         const char *se = gmio_file_ext(cfg.stray_light_file);
         if (!std::strcmp(se, ".per")) {
@@ -157,8 +157,9 @@ int main(int argc, char **argv)
             return fail("stray-light FITS maps are not supported: ", cfg.stray_light_file);
         }
     }
-    if (cfg.number_of_cycles != 0 && cfg.fix[10])
-        return fail("a free filling factor (10/11-term inversion) is not supported", "");
+    if (cfg.number_of_cycles > 0 && cfg.fix[10] && !s.use_slight)
+        return fail("a free filling factor needs a stray light profile (.per): ",
+                    cfg.stray_light_file);
 
     gm_ctx *ctx = nullptr;
     gm_status st = gm_ctx_create(&s, &ctx);

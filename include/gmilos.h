@@ -111,9 +111,13 @@ typedef struct {
  * stray-light mixing with alfa-scaled responses, me_der.cu:191-211) when its
  * set-up has use_slight, fix[9], or an initial mac above 1e-4.  Other
  * contexts run the 9-parameter kernels, which take mac = 0 and alfa = 1
- * whatever a caller's model rows hold.  A free filling factor (fix[10],
- * the 10- and 11-term solves of lm_mils / lm_mils_11) is not built:
- * gm_ctx_create returns GM_ERR_UNSUPPORTED for it.                          */
+ * whatever a caller's model rows hold.
+ * A free filling factor (fix[10], which needs use_slight) solves eleven
+ * slots in FP64: 0-8 as the 9-parameter system, 9 = mac (free only with
+ * fix[9], the reference's lm_mils_11; otherwise the 10-term lm_mils), 10 =
+ * alfa.  nfree is 4 nlambda - NTERMS (10 or 11).  Pieces of such a context
+ * use alpha [npix][66], beta [npix][11], delta [npix][11] and responses
+ * [npix][4][NTERMS][nlambda] in the reference's slot order.                 */
 
 /* Outputs of an inversion (caller-owned device memory; synth and naccs may
  * be NULL).  Same content as the reference's d_vModels, d_vChisqrf,

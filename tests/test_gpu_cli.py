@@ -103,6 +103,37 @@ def test_cli_synthesis_with_macroturbulence_and_stray_light(refdir, orc):
     assert np.abs(got[:, 2:].T - ref).max() < 2e-6
 
 
+def test_cli_inverts_filling_factor_with_stray_light(refdir):
+    """Synthesise run/init.mod with filling factor 0.75 and a stray-light
+    .per, then invert it from run/guess.mod with "Invert stray light factor"
+    on (fix[10], the reference's 10-term run with alfa free): the model and
+    the filling factor are recovered."""
+    off = np.arange(-350.0, 665.0 + 1e-9, 35.0)
+    stray = np.zeros((4, 30))
+    stray[0] = 0.95 - 0.35 * np.exp(-((off - 40.0) / 110.0) ** 2)
+    with open(refdir / "run" / "stray.per", "w") as f:
+        for k in range(30):
+            f.write("1\t%f\t%e\t%e\t%e\t%e\n" % (off[k], *stray[:, k]))
+    mod = open(refdir / "run" / "init.mod").read().splitlines()
+    mod = [("filling factor: 0.75" if l.lower().startswith("filling") else l) for l in mod]
+    open(refdir / "run" / "ff.mod", "w").write("\n".join(mod) + "\n")
+    text = open(refdir / "run" / "synthesis.mtrol").read().splitlines()
+    text[2] = "Stray light file             :run/stray.per     ! stray light"
+    text[7] = "Initial guess model 1     (*):run/ff.mod !"
+    open(refdir / "run" / "synth_ff.mtrol", "w").write("\n".join(text) + "\n")
+    run(refdir, "run/synth_ff.mtrol")
+    text = open(refdir / "run" / "invert.mtrol").read().splitlines()
+    assert text[35].startswith("Invert stray light factor")
+    text[2] = "Stray light file             :run/stray.per     ! stray light"
+    text[35] = "Invert stray light factor?   :1                 ! (0 or blank=no, 1=yes)"
+    open(refdir / "run" / "inv_ff.mtrol", "w").write("\n".join(text) + "\n")
+    run(refdir, "run/inv_ff.mtrol", "--mixed")
+    m = read_mod(refdir / "data" / "synthesis_gpu_model.mod")
+    truth = read_mod(refdir / "run" / "ff.mod")
+    np.testing.assert_allclose(model_row(m)[:9], model_row(truth)[:9], rtol=5e-3, atol=2e-3)
+    assert abs(m["alfa"] - 0.75) < 1e-3
+
+
 def test_cli_inverts_reference_example(refdir):
     run(refdir, "run/invert.mtrol")
     m = read_mod(refdir / "data" / "synthesis_gpu_model.mod")

@@ -140,3 +140,133 @@ def test_plain_context_ignores_per_pixel_mac_alfa(gm, orc, cuda):
     for k in a:
         np.testing.assert_array_equal(a[k], b[k])
     assert np.all(b["model"][:, 9] == 0.0) and np.all(b["model"][:, 10] == 1.0)
+
+
+# ---- free filling factor: the 10-term (lm_mils) and 11-term (lm_mils_11) runs
+FREE10 = ("eta0", "B", "vlos", "dopp", "aa", "gm", "az", "S0", "S1", "alfa")
+FREE11 = FREE10 + ("mac",)
+
+
+def ff_models(gm, n, seed):
+    m = gm.random_models(n, seed=seed)
+    rng = np.random.default_rng(seed + 2)
+    m[:, 9] = rng.uniform(0.3, 2.5, n)
+    m[:, 10] = rng.uniform(0.4, 1.0, n)
+    return m
+
+
+@pytest.mark.parametrize("free", [FREE10, FREE11])
+def test_free_filling_factor_responses_match_oracle(gm, orc, cuda, free):
+    """Reference layout [4][NTERMS][n]: NTERMS 10 keeps d/dalfa in slot 9,
+    11 has alfa-scaled d/dmac in 9 and d/dalfa in 10 (me_der.cu:191-207)."""
+    L = stray_profile()
+    p, o = ext_pair(gm, orc, GAUSS60, slight=L, free=free, rf_reference_rules=0)
+    nt = 11 if "mac" in free else 10
+    assert orc.nterms(o) == nt
+    models = ff_models(gm, 128, 41)
+    ctx = gm.Context(p)
+    sp, rf = ctx.synth_rf(to_dev(models, cuda))
+    rf = rf.cpu().numpy()
+    assert rf.shape == (128, 4, nt, 30)
+    sp_ref, rf_ref = orc.synth_rf(o, models)
+    np.testing.assert_allclose(sp.cpu().numpy(), sp_ref, atol=2e-5)
+    scale = np.abs(rf_ref).max(axis=-1, keepdims=True)
+    ic = (models[:, 7] + models[:, 8])[:, None, None, None]
+    step = np.array(DP_TYP + (0.01,) * (nt - 9))[None, None, :, None]
+    tol = np.broadcast_to(5e-4 * scale + 2e-7 * ic / step, rf.shape)
+    bad = np.abs(rf - rf_ref) > tol
+    if bad.any():
+        for i in np.argwhere(bad)[:8]:
+            i = tuple(i)
+            print("mismatch", i, rf[i], rf_ref[i], tol[i])
+    assert not bad.any(), np.argwhere(bad)[:5]
+
+
+@pytest.mark.parametrize("free", [FREE10, FREE11])
+def test_free_filling_factor_normal_equations_match_oracle(gm, orc, cuda, free):
+    L = stray_profile()
+    p, o = ext_pair(gm, orc, GAUSS60, slight=L, free=free)
+    nt = orc.nterms(o)
+    truth = ff_models(gm, 96, 42)
+    obs = observations(orc, o, truth, seed=42)
+    models = ff_models(gm, 96, 43)
+    ctx = gm.Context(p)
+    a, b, c = ctx.normal_equations(to_dev(models, cuda), to_dev(obs, cuda))
+    a, b, c = a.cpu().numpy(), b.cpu().numpy(), c.cpu().numpy()
+    assert a.shape == (96, 66) and b.shape == (96, 11)
+    sp, rf = orc.synth_rf(o, models)
+    # oracle slot -> eleven-slot index (NTERMS 10: alfa in slot 9 -> 10)
+    idx = list(range(9)) + ([9, 10] if nt == 11 else [10])
+    for i in range(len(models)):
+        A, B = orc.covarm(o, obs[i], sp[i], rf[i])
+        A11 = gm.unpack_alpha(a[i])
+        assert np.abs(A11[np.ix_(idx, idx)] - A).max() <= 1e-3 * np.abs(A).max()
+        assert np.abs(b[i][idx] - B).max() <= 2e-3 * np.abs(B).max()
+        if nt == 10:
+            assert np.all(A11[9] == 0) and np.all(b[i][9] == 0)
+        assert c[i] == pytest.approx(orc.chisqr(o, obs[i], sp[i]), rel=2e-5)
+
+
+def test_eleven_slot_step_solve_matches_oracle(gm, orc, cuda):
+    """FP64 LDL^T / Jacobi pseudo-inverse of 11x11 systems: well-conditioned
+    damped ones, zero rows, and exactly singular integer ones (A = J^T J / 64
+    with dependent columns of small-integer J, exact in float32)."""
+    L = stray_profile()
+    p, _ = ext_pair(gm, orc, None, slight=L, free=FREE11)
+    ctx = gm.Context(p)
+    rng = np.random.default_rng(44)
+    n = 192
+    packed = np.zeros((n, 66), dtype=np.float32)
+    lam = np.zeros(n)
+    for i in range(n):
+        if i % 2 == 0:
+            J = rng.normal(size=(120, 11)) * rng.uniform(0.01, 10, 11)
+            M = J.T @ J
+            if i % 6 == 0:
+                k = (i // 6) % 11
+                M[k, :] = M[:, k] = 0.0
+            lam[i] = 10.0 ** rng.uniform(-4, 2)
+        else:
+            J = rng.integers(-3, 4, size=(24, 11)).astype(np.float64)
+            J[:, 10] = J[:, 0] + J[:, 3]
+            if i % 4 == 1:
+                J[:, 9] = J[:, 1] - J[:, 2]
+            J = J[:, rng.permutation(11)]
+            M = J.T @ J / 64.0
+        for q in range(11):
+            for r in range(q, 11):
+                packed[i, gm.packed_index(q, r, 11)] = M[q, r]
+    b = (rng.normal(size=(n, 11)) * 10).astype(np.float32)
+    d = ctx.step_solve(to_dev(packed, cuda), to_dev(b, cuda), to_dev(lam, cuda)).cpu().numpy()
+    for i in range(n):
+        A = gm.unpack_alpha(packed[i]).astype(np.float64)
+        A[np.diag_indices(11)] *= 1.0 + lam[i]
+        x, _ = orc.svd_solve(A, b[i].astype(np.float64))
+        if i % 2:
+            xp = np.linalg.pinv(A, rcond=1e-10) @ b[i].astype(np.float64)
+            assert np.abs(x - xp).max() <= 1e-8 * np.abs(xp).max()
+        assert np.abs(d[i] - x).max() <= 1e-6 * np.abs(x).max() + 1e-12, (i, d[i], x)
+
+
+@pytest.mark.parametrize("free", [FREE10, FREE11])
+def test_free_filling_factor_inversion_matches_oracle(gm, orc, cuda, free):
+    """Inversions with the filling factor (and mac) free against the oracle's
+    10-/11-term LM: medians of |d| for B, gamma, phi, vlos and alfa."""
+    L = stray_profile()
+    p, o = ext_pair(gm, orc, GAUSS60, mac=1.0, alfa=0.9, slight=L, free=free)
+    truth = ff_models(gm, 384, 45)
+    if "mac" not in free:
+        truth[:, 9] = 1.0
+    obs = observations(orc, o, truth, seed=45)
+    ctx = gm.Context(p)
+    got = {k: v.cpu().numpy() for k, v in ctx.invert(to_dev(obs, cuda)).items()}
+    ref = orc.invert(o, obs)
+    d = param_errors(got["model"], ref["model"])
+    _assert_medians(d, "free " + ",".join(free[9:]))
+    print("median |d alfa| %.3g, |d mac| %.3g" % (np.median(d[:, 10]), np.median(d[:, 9])))
+    assert np.median(d[:, 10]) <= 1e-3
+    if "mac" in free:
+        assert np.median(d[:, 9]) <= 1e-2
+    else:
+        assert np.all(got["model"][:, 9] == 1.0)
+    assert np.median(np.abs(got["chisqr"] / ref["chisqr"] - 1)) < 1e-3

@@ -44,8 +44,20 @@ def _wait_all(srv, tickets, timeout_s=120):
     return got
 
 
-def test_server_tile_matches_invert(gm, cuda):
-    setup = _setups(gm)[0]
+def _ext_setup(gm):
+    """Macroturbulence fixed at 0.8 km/s, filling factor free against a
+    stray-light profile: the eleven-slot kernels."""
+    x = np.linspace(-1.0, 1.9, 30)
+    stray = np.zeros((4, 30))
+    stray[0] = 0.95 - 0.35 * np.exp(-(x / 0.35) ** 2)
+    free = ("eta0", "B", "vlos", "dopp", "aa", "gm", "az", "S0", "S1", "alfa")
+    return gm.setup_single_line(psf_fwhm=60.0, precision=gm.GM_PREC_MIXED, free=free,
+                                init=dict(mac=0.8, alfa=0.9), slight=stray)
+
+
+@pytest.mark.parametrize("kind", ["plain", "ext"])
+def test_server_tile_matches_invert(gm, cuda, kind):
+    setup = _setups(gm)[0] if kind == "plain" else _ext_setup(gm)
     ctx = gm.Context(setup)
     obs = _obs(gm, ctx, 3000, seed=41)
     ref = ctx.invert(obs)
