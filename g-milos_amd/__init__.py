@@ -132,7 +132,7 @@ EXPORTED = ("gm_ctx_create", "gm_ctx_destroy", "gm_status_string", "gm_ctx_set_t
             "gm_server_create", "gm_server_add_setup", "gm_server_submit", "gm_server_poll",
             "gm_server_get_stats", "gm_server_clock_ms", "gm_server_destroy",
             "gm_prior_default", "gm_draw_models", "gm_add_noise", "gm_mlp_fit",
-            "gm_invert_guided")
+            "gm_invert_guided", "gm_ctx_set_stray_map")
 
 _lib = None
 
@@ -158,6 +158,7 @@ def load_library(path=LIB_PATH):
         "gm_ctx_destroy": ([vp], i32),
         "gm_status_string": ([i32], ctypes.c_char_p),
         "gm_ctx_set_timing": ([vp, i32], i32),
+        "gm_ctx_set_stray_map": ([vp, vp, i64], i32),
         "gm_last_stats": ([vp, P(gm_stats)], i32),
         "gm_invert": ([vp, vp, i64, vp, P(gm_out), vp], i32),
         "gm_synth": ([vp, vp, i64, vp, vp], i32),
@@ -429,6 +430,20 @@ class Context:
     def _stream():
         import torch
         return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def set_stray_map(self, stray_map):
+        """Per-pixel stray light: float32 [npix, 4, nlambda] CUDA tensor
+        (kept referenced here), or None for the set-up's profile."""
+        if stray_map is None:
+            _check(self.lib.gm_ctx_set_stray_map(self.h, None, 0), "gm_ctx_set_stray_map")
+            self._stray_map = None
+            return
+        import torch
+        assert stray_map.dtype == torch.float32 and stray_map.is_contiguous()
+        assert stray_map.shape[1:] == (4, self.nlambda)
+        _check(self.lib.gm_ctx_set_stray_map(self.h, ctypes.c_void_p(stray_map.data_ptr()),
+                                             stray_map.shape[0]), "gm_ctx_set_stray_map")
+        self._stray_map = stray_map
 
     def set_timing(self, on):
         _check(self.lib.gm_ctx_set_timing(self.h, 1 if on else 0), "gm_ctx_set_timing")

@@ -49,6 +49,7 @@ struct gm_ctx {
     int timing = 0;
     std::vector<Events> kev;         /* per-kernel events when timing is on   */
     gm_stats stats;
+    int64_t slight_map_npix = 0;     /* pixels covered by the stray-light map */
 };
 
 static gm_status hip_ok(hipError_t e)
@@ -269,6 +270,24 @@ extern "C" gm_status gm_ctx_destroy(gm_ctx *c)
     return GM_OK;
 }
 
+extern "C" gm_status gm_ctx_set_stray_map(gm_ctx *c, const float *map, int64_t npix)
+{
+    if (!c || (map && npix <= 0)) return GM_ERR_INVALID;
+    if (map && !c->host_ds.use_slight) return GM_ERR_INVALID;
+    /* the device copy of the set-up is read by work in flight */
+    GM_HIP(hipDeviceSynchronize());
+    GM_HIP(hipMemcpy(&c->dS->slight_map, &map, sizeof(map), hipMemcpyHostToDevice));
+    c->host_ds.slight_map = map;
+    c->slight_map_npix = map ? npix : 0;
+    return GM_OK;
+}
+
+/* A stray-light map must cover every pixel of a call. */
+static bool map_covers(const gm_ctx *c, int64_t npix)
+{
+    return !c->host_ds.slight_map || npix <= c->slight_map_npix;
+}
+
 extern "C" gm_status gm_ctx_set_timing(gm_ctx *c, int32_t enable)
 {
     if (!c) return GM_ERR_INVALID;
@@ -354,6 +373,7 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
     }
     if (!obs || npix > (int64_t)0x7fffffff) return GM_ERR_INVALID;
     if (c->setup.init_mode == GM_INIT_PER_PIXEL && !init) return GM_ERR_INVALID;
+    if (!map_covers(c, npix)) return GM_ERR_INVALID;
     std::memset(&c->stats, 0, sizeof c->stats);
     c->stats.npix = npix;
     c->stats.psf_taps = c->host_ds.use_psf ? (c->host_ds.psf_dhi - c->host_ds.psf_dlo + 1) : 0;
@@ -463,6 +483,7 @@ extern "C" gm_status gm_synth(gm_ctx *c, const double *models, int64_t npix, flo
                               void *stream)
 {
     if (!c || npix < 0) return GM_ERR_INVALID;
+    if (!map_covers(c, npix)) return GM_ERR_INVALID;
     if (npix == 0) return GM_OK;
     if (!models || !spectra) return GM_ERR_INVALID;
     gm_launch_synth(c->lc, c->dS, models, nullptr, 0, nullptr, nullptr, nullptr, npix, npix, nullptr,
@@ -474,6 +495,7 @@ extern "C" gm_status gm_synth_rf(gm_ctx *c, const double *models, int64_t npix, 
                                  float *rf, void *stream)
 {
     if (!c || npix < 0) return GM_ERR_INVALID;
+    if (!map_covers(c, npix)) return GM_ERR_INVALID;
     if (npix == 0) return GM_OK;
     if (!models) return GM_ERR_INVALID;
     gm_launch_rf(c->lc, c->dS, models, nullptr, 0, nullptr, nullptr, nullptr, npix, npix, nullptr, nullptr,
@@ -486,6 +508,7 @@ extern "C" gm_status gm_normal_equations(gm_ctx *c, const double *models, const 
                                          void *stream)
 {
     if (!c || npix < 0) return GM_ERR_INVALID;
+    if (!map_covers(c, npix)) return GM_ERR_INVALID;
     if (npix == 0) return GM_OK;
     if (!models || !obs) return GM_ERR_INVALID;
     const int mixed = c->setup.precision == GM_PREC_MIXED;

@@ -221,6 +221,10 @@ extern "C" gm_status gm_invert_guided(gm_ctx *learned, gm_ctx *plain, gm_mlp *ne
     if (sl.init_mode != GM_INIT_PER_PIXEL || sp.init_mode == GM_INIT_PER_PIXEL ||
         sl.nlambda != sp.nlambda)
         return GM_ERR_INVALID;
+    /* the fallback inverts a gathered subset: per-pixel stray-light maps
+     * would be indexed by the wrong pixel */
+    if (gm_ctx_host_setup(learned).slight_map || gm_ctx_host_setup(plain).slight_map)
+        return GM_ERR_INVALID;
     gm_guided_stats st{};
     st.npix = npix;
     if (npix == 0) {

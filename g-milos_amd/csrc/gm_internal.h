@@ -58,6 +58,8 @@ struct gm_dsetup {
     double  mac_l0;                          /* lambda0 of the first line    */
     float   psf_full[GM_MAX_LAMBDA];         /* all nlambda PSF taps         */
     float   slight[4 * GM_MAX_LAMBDA];       /* stray light [4][nlambda]     */
+    const float *slight_map;                 /* per-pixel [npix][4][nlambda]
+                                                or NULL (mapStrayLight)     */
 };
 
 /* Per-pixel LM state, structure-of-arrays over the pixel capacity. */

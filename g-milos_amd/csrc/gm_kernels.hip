@@ -682,7 +682,8 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
                      * reference's LM loop passes it (milosUtils.cu:673) */
 #pragma unroll
                     for (int st = 0; st < 4; ++st) {
-                        const float ls = S->slight[st * n + k];
+                        const float ls = S->slight_map ? S->slight_map[pix * 4 * n + st * n + k]
+                                                       : S->slight[st * n + k];
                         const float pure = y[st * 9];
                         y[st * 9] = alv * pure + (1.f - alv) * ls;
                         if (P11) ra[st] = (alpha_first ? pure : y[st * 9]) - ls;
@@ -896,10 +897,11 @@ __global__ __launch_bounds__(64) void k_synth(
                 y = conv[padlo + k];
             }
             if (EXT && S->use_slight && k < n) {
-                y.x = alv * y.x + (1.f - alv) * S->slight[k];
-                y.y = alv * y.y + (1.f - alv) * S->slight[n + k];
-                y.z = alv * y.z + (1.f - alv) * S->slight[2 * n + k];
-                y.w = alv * y.w + (1.f - alv) * S->slight[3 * n + k];
+                const float *sl = S->slight_map ? S->slight_map + pix * 4 * n : S->slight;
+                y.x = alv * y.x + (1.f - alv) * sl[k];
+                y.y = alv * y.y + (1.f - alv) * sl[n + k];
+                y.z = alv * y.z + (1.f - alv) * sl[2 * n + k];
+                y.w = alv * y.w + (1.f - alv) * sl[3 * n + k];
             }
             if (k < n && obs) {
                 const float r0 = y.x - ob[s][0], r1 = y.y - ob[s][1], r2 = y.z - ob[s][2],

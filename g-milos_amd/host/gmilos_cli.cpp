@@ -141,6 +141,7 @@ int main(int argc, char **argv)
     /* stray light: read for the synthesis mode when the file exists
      * (gmilos.cu:434-461) and, for inversions, only with a free filling
      * factor (gmilos.cu:688-716, 916-944) */
+    bool use_stray_map = false;
     if (cfg.stray_light_file[0] && (cfg.number_of_cycles == 0 || cfg.fix[10])) {
         const char *se = gmio_file_ext(cfg.stray_light_file);
         if (!std::strcmp(se, ".per")) {
@@ -154,7 +155,40 @@ int main(int argc, char **argv)
                 for (int k = 0; k < 4 * n; ++k) s.slight[k] = sl[k];
             }
         } else if (!std::strcmp(se, ".fits")) {
-            return fail("stray-light FITS maps are not supported: ", cfg.stray_light_file);
+            /* readFitsStrayLightFile (utilsFits.c:1578-1897): a 2-D float
+             * image is one profile (wavelength the longer axis), a 4-D one a
+             * per-pixel map laid out like the observed cube */
+            int32_t naxis = 0;
+            int64_t naxes[4] = {1, 1, 1, 1};
+            if (gmio_fits_read_image(cfg.stray_light_file, &naxis, naxes, nullptr, 0) != 0)
+                return fail("cannot read stray light FITS (float32): ", cfg.stray_light_file);
+            if (naxis == 2) {
+                const int64_t nl = naxes[0] > naxes[1] ? naxes[0] : naxes[1];
+                const int64_t ns = naxes[0] > naxes[1] ? naxes[1] : naxes[0];
+                if (nl != n || ns != 4) {
+                    std::printf("The number of wavelengths is different in the stray light file: "
+                                "%lld and the grid %d; stray light is not used.\n",
+                                (long long)nl, n);
+                } else {
+                    std::vector<float> raw(4 * n);
+                    gmio_fits_read_image(cfg.stray_light_file, &naxis, naxes, raw.data(), 4 * n);
+                    s.use_slight = 1;
+                    for (int st = 0; st < 4; ++st)
+                        for (int k = 0; k < n; ++k)
+                            s.slight[st * n + k] = (naxes[0] == nl) ? raw[st * n + k] : raw[k * 4 + st];
+                }
+            } else if (naxis == 4) {
+                if (cfg.number_of_cycles == 0 ||
+                    std::strcmp(gmio_file_ext(cfg.observed_profiles), ".fits")) {
+                    std::printf("A 4-D stray light map needs a FITS cube to invert; stray light "
+                                "is not used.\n");
+                } else {
+                    use_stray_map = true;
+                    s.use_slight = 1;          /* the map replaces the profile */
+                }
+            } else {
+                return fail("stray light FITS must have 2 or 4 axes: ", cfg.stray_light_file);
+            }
         }
     }
     if (cfg.number_of_cycles > 0 && cfg.fix[10] && !s.use_slight)
@@ -210,6 +244,25 @@ int main(int argc, char **argv)
     Dev dobs(sizeof(float) * obs.size()), dmod(sizeof(double) * 11 * npix);
     if (!dobs.p || !dmod.p) return fail("device allocation failed");
     (void)hipMemcpy(dobs.p, obs.data(), sizeof(float) * obs.size(), hipMemcpyHostToDevice);
+    /* per-pixel stray light (mapStrayLight, gmilos.cu:1212-1226) */
+    Dev dmap(use_stray_map ? sizeof(float) * obs.size() : 0);
+    if (use_stray_map) {
+        gmio_cube *mc = nullptr;
+        int32_t mr = 0, mcn = 0, ml = 0;
+        if (gmio_fits_read_cube(cfg.stray_light_file, n, &mc) != 0)
+            return fail("cannot read the stray light map: ", cfg.stray_light_file);
+        gmio_cube_dims(mc, &mr, &mcn, &ml);
+        if (mr != nrows || mcn != ncols) {
+            gmio_cube_free(mc);
+            return fail("the stray light map and the observed cube differ in size: ",
+                        cfg.stray_light_file);
+        }
+        (void)hipMemcpy(dmap.p, gmio_cube_data(mc), sizeof(float) * obs.size(),
+                        hipMemcpyHostToDevice);
+        gmio_cube_free(mc);
+        st = gm_ctx_set_stray_map(ctx, (const float *)dmap.p, npix);
+        if (st != GM_OK) return fail("stray light map: ", gm_status_string(st));
+    }
     std::vector<double> models(11 * npix);
 
     /* ---- classical estimates (gmilos.cu:309-431) -------------------------- */

@@ -153,6 +153,13 @@ typedef struct gm_ctx gm_ctx;
 gm_status gm_ctx_create(const gm_setup *setup, gm_ctx **ctx);
 gm_status gm_ctx_destroy(gm_ctx *ctx);
 const char *gm_status_string(gm_status st);
+/* Per-pixel stray light (mapStrayLight of lm_mils, gmilos.cu:1212-1226):
+ * map is device memory [npix][4][nlambda] that replaces the set-up's
+ * profile for pixel p of every later gm_invert / gm_synth / gm_synth_rf /
+ * gm_normal_equations call on this context (calls with more pixels than the
+ * map covers are refused); NULL reverts to the profile.  The context's
+ * set-up must have use_slight.  The caller keeps the map alive meanwhile. */
+gm_status gm_ctx_set_stray_map(gm_ctx *ctx, const float *map, int64_t npix);
 /* Enables per-kernel-class event timing in gm_stats (adds syncs). */
 gm_status gm_ctx_set_timing(gm_ctx *ctx, int32_t enable);
 gm_status gm_last_stats(const gm_ctx *ctx, gm_stats *stats);

@@ -103,6 +103,36 @@ def test_cli_synthesis_with_macroturbulence_and_stray_light(refdir, orc):
     assert np.abs(got[:, 2:].T - ref).max() < 2e-6
 
 
+def test_cli_stray_light_fits_profile(refdir):
+    """A 2-D FITS stray-light profile (readFitsStrayLightFile, utilsFits.c:
+    1599-1630; wavelength on either axis) gives the same synthesis as the
+    .per profile."""
+    off = np.arange(-350.0, 665.0 + 1e-9, 35.0)
+    stray = np.zeros((4, 30), dtype=np.float32)
+    stray[0] = 0.9 - 0.3 * np.exp(-((off - 20.0) / 120.0) ** 2)
+    stray[3] = 1e-3 * np.sin(off / 100.0)
+    with open(refdir / "run" / "stray.per", "w") as f:
+        for k in range(30):
+            f.write("1\t%f\t%e\t%e\t%e\t%e\n" % (off[k], *stray[:, k]))
+    write_fits(str(refdir / "run" / "stray_a.fits"), stray, -32)          # naxes (30, 4)
+    write_fits(str(refdir / "run" / "stray_b.fits"), stray.T.copy(), -32)  # naxes (4, 30)
+    mod = open(refdir / "run" / "init.mod").read().splitlines()
+    mod = [("filling factor: 0.6" if l.lower().startswith("filling") else l) for l in mod]
+    open(refdir / "run" / "ff.mod", "w").write("\n".join(mod) + "\n")
+    res = []
+    for name in ("stray.per", "stray_a.fits", "stray_b.fits"):
+        text = open(refdir / "run" / "synthesis.mtrol").read().splitlines()
+        text[2] = "Stray light file             :run/%s     ! stray light" % name
+        text[7] = "Initial guess model 1     (*):run/ff.mod !"
+        open(refdir / "run" / "s.mtrol", "w").write("\n".join(text) + "\n")
+        run(refdir, "run/s.mtrol")
+        res.append(np.loadtxt(refdir / "data" / "synthesis_gpu.per"))
+    np.testing.assert_array_equal(res[1], res[2])
+    np.testing.assert_allclose(res[0], res[1], atol=1e-6)   # .per holds 7 digits
+    plain = np.loadtxt(os.path.join(GOLDEN, "synthesis_gpu.per"))
+    assert np.abs(res[0][:, 2] - plain[:, 2]).max() > 1e-2       # the stray light is in
+
+
 def test_cli_inverts_filling_factor_with_stray_light(refdir):
     """Synthesise run/init.mod with filling factor 0.75 and a stray-light
     .per, then invert it from run/guess.mod with "Invert stray light factor"

@@ -270,3 +270,44 @@ def test_free_filling_factor_inversion_matches_oracle(gm, orc, cuda, free):
     else:
         assert np.all(got["model"][:, 9] == 1.0)
     assert np.median(np.abs(got["chisqr"] / ref["chisqr"] - 1)) < 1e-3
+
+
+def test_stray_light_map_matches_oracle(gm, orc, cuda):
+    """Per-pixel stray light (mapStrayLight, gmilos.cu:1212-1226): synthesis
+    and an inversion with the filling factor free, each pixel against the
+    oracle with that pixel's profile; calls beyond the map are refused."""
+    import torch
+    L = stray_profile()
+    p, o = ext_pair(gm, orc, GAUSS60, mac=0.6, alfa=0.9, slight=L, free=FREE10)
+    npix = 48
+    rng = np.random.default_rng(46)
+    maps = (L[None] * rng.uniform(0.8, 1.2, size=(npix, 1, 1)) +
+            rng.normal(0, 1e-3, size=(npix, 4, 30))).astype(np.float32)
+    truth = ff_models(gm, npix, 47)
+    truth[:, 9] = 0.6
+    ctx = gm.Context(p)
+    dmap = to_dev(maps, cuda)
+    ctx.set_stray_map(dmap)
+    got_sp = ctx.synth(to_dev(truth, cuda)).cpu().numpy()
+    obs = np.zeros((npix, 4, 30), dtype=np.float32)
+    noise = np.random.default_rng(48).normal(size=(npix, 4, 30))
+    for i in range(npix):
+        orc.set_slight(o, maps[i])
+        ref = orc.synth(o, truth[i])[0]
+        ic = truth[i, 7] + truth[i, 8]
+        assert np.abs(got_sp[i] - ref).max() <= 5e-6 * ic + 2e-5 * np.abs(ref).max()
+        obs[i] = (ref + 1e-3 * ic * noise[i]).astype(np.float32)
+    got = {k: v.cpu().numpy() for k, v in ctx.invert(to_dev(obs, cuda)).items()}
+    d = np.zeros((npix, 11))
+    for i in range(npix):
+        orc.set_slight(o, maps[i])
+        r = orc.lm_pixel(o, obs[i])
+        d[i] = param_errors(got["model"][i:i + 1], r["model"][None])[0]
+    _assert_medians(d, "stray-light map")
+    assert np.median(d[:, 10]) <= 1e-3
+    with pytest.raises(RuntimeError):
+        ctx.invert(to_dev(np.concatenate([obs, obs[:1]]), cuda))
+    ctx.set_stray_map(None)
+    back = ctx.synth(to_dev(truth, cuda)).cpu().numpy()
+    orc.set_slight(o, L)
+    np.testing.assert_allclose(back, orc.synth(o, truth), atol=2e-5)
