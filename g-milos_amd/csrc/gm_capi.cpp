@@ -314,7 +314,7 @@ static gm_status ensure_state(gm_ctx *c, int64_t npix)
     if (npix <= c->st.cap) return GM_OK;
     int64_t cap = ((npix + 255) / 256) * 256;
     const size_t per = sizeof(double) * (2 * GM_NMODEL + 5) +
-                       sizeof(float) * (GM_ALPHA_MAX + GM_BETA_MAX + 2 * GM_NPRE) +
+                       sizeof(float) * (GM_AB_REC11 + 2 * GM_NPRE) +
                        sizeof(int32_t) * (2 + 4);
     const size_t bytes = per * (size_t)cap + 256 * 22 + 64 + sizeof(int32_t) * 3 * (kMaxRounds + 2) + 256;
     if (c->ws) {
@@ -335,8 +335,10 @@ static gm_status ensure_state(gm_ctx *c, int64_t npix)
     st.cap = cap;
     st.best = (double *)take(sizeof(double) * GM_NMODEL * cap);
     st.trial = (double *)take(sizeof(double) * GM_NMODEL * cap);
-    st.alpha = (float *)take(sizeof(float) * GM_ALPHA_MAX * cap);
-    st.beta = (float *)take(sizeof(float) * GM_BETA_MAX * cap);
+    const bool p11 = c->lc.np == 11;
+    st.ab_rec = p11 ? GM_AB_REC11 : GM_AB_REC9;
+    st.alpha = (float *)take(sizeof(float) * GM_AB_REC11 * cap);
+    st.beta = st.alpha + (p11 ? GM_AB_BETA11 : GM_AB_BETA9);
     st.pre_best = (float *)take(sizeof(float) * GM_NPRE * cap);
     st.pre_trial = (float *)take(sizeof(float) * GM_NPRE * cap);
     st.chi_trial = (double *)take(sizeof(double) * cap);
@@ -407,7 +409,7 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
     kend();
     kbegin(1);
     gm_launch_rf(lc, c->dS, st.best, st.pre_best, st.cap, obs, st.list[0], &cnt[0], 0, npix, st.alpha,
-                 st.beta, st.cap, st.ochisqr, out->synth, nullptr, mixed, s);
+                 st.beta, -st.ab_rec, st.ochisqr, out->synth, nullptr, mixed, s);
     kend();
     kbegin(3);
     gm_launch_init_chi(st, 0, npix, s);
@@ -436,7 +438,7 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
         /* alpha_first = 0: the filling-factor response of the reference's
          * loop (milosUtils.cu:673) */
         gm_launch_rf(lc, c->dS, st.best, st.pre_best, st.cap, obs, st.list[2], &cnt[2], 0, npix, st.alpha,
-                     st.beta, st.cap, nullptr, nullptr, nullptr, mixed, s, 0);
+                     st.beta, -st.ab_rec, nullptr, nullptr, nullptr, mixed, s, 0);
         kend();
         cur = nxt;
     }

@@ -12,6 +12,10 @@
 #define GM_ALPHA_N   45          /* packed upper triangle of the 9x9 system */
 #define GM_ALPHA_MAX 66          /* ... of the 11x11 system (free filling factor) */
 #define GM_BETA_MAX  11
+#define GM_AB_REC9   64          /* per-pixel alpha/beta record: 45 + 9 at 48 */
+#define GM_AB_REC11  80          /* ... 66 + 11 at 68                          */
+#define GM_AB_BETA9  48
+#define GM_AB_BETA11 68
 #define GM_NPRE      19          /* per-pixel FP32 synthesis constants      */
 #define GM_RED_N     55          /* 45 alpha + 9 beta + 1 chi^2             */
 #define GM_SLOTS     36          /* staged values per wavelength: per Stokes
@@ -67,8 +71,10 @@ struct gm_state {
     int64_t  cap;
     double  *best;        /* [11][cap] best accepted model                  */
     double  *trial;       /* [11][cap] last proposed model                  */
-    float   *alpha;       /* [45][cap] normal matrix at best                */
-    float   *beta;        /* [9][cap]                                        */
+    float   *alpha;       /* [cap][ab_rec] per-pixel records: packed normal
+                             matrix at best (45 or 66 floats) ...            */
+    float   *beta;        /* ... and the gradient at alpha + ab_beta         */
+    int32_t  ab_rec;      /* record stride in floats (64, or 80 for 11 slots) */
     float   *pre_best;    /* [GM_NPRE][cap] synthesis constants of best      */
     float   *pre_trial;   /* [GM_NPRE][cap] ... of trial                     */
     double  *chi_trial;   /* [cap] chi^2 of trial                           */

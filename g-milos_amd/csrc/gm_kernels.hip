@@ -1566,7 +1566,8 @@ __global__ __launch_bounds__(64) void k_solve11(const gm_dsetup *__restrict__ S,
          i += (int64_t)gridDim.x * 64) {
         const int64_t pix = list[i];
         double a[66], b[11], x[11];
-        load_damped_n<double, 11>(st.alpha + pix, st.beta + pix, cap, st.flambda[pix], a, b);
+        load_damped_n<double, 11>(st.alpha + pix * st.ab_rec, st.beta + pix * st.ab_rec, 1,
+                                  st.flambda[pix], a, b);
         double a2[66];
 #pragma unroll
         for (int e = 0; e < 66; ++e) a2[e] = a[e];
@@ -1635,7 +1636,8 @@ __global__ __launch_bounds__(64) void k_solve(const gm_dsetup *__restrict__ S,
          i += (int64_t)gridDim.x * 64) {
         const int64_t pix = list[i];
         T a[45], b[9], x[9];
-        load_damped<T>(st.alpha + pix, st.beta + pix, cap, (T)st.flambda[pix], a, b);
+        load_damped<T>(st.alpha + pix * st.ab_rec, st.beta + pix * st.ab_rec, 1,
+                       (T)st.flambda[pix], a, b);
         if (SOLVER == GM_SOLVE_JACOBI) {
             fb_list[atomicAdd(fb_count, 1)] = (int32_t)pix;
             continue;
@@ -1665,7 +1667,8 @@ __global__ __launch_bounds__(64) void k_solve_svd(const gm_dsetup *__restrict__ 
          i += ((int64_t)gridDim.x * 64) >> 1) {
         const int64_t pix = fb_list[i];
         T a[45], b[9], x[9];
-        load_damped<T>(st.alpha + pix, st.beta + pix, cap, (T)st.flambda[pix], a, b);
+        load_damped<T>(st.alpha + pix * st.ab_rec, st.beta + pix * st.ab_rec, 1,
+                       (T)st.flambda[pix], a, b);
         solve_svd_pair<T, (sizeof(T) == 8 ? 12 : 10)>(a, b, half, x);
         if (half == 0) apply_to_trial<T>(S, st, pix, x);
     }
@@ -2106,13 +2109,16 @@ void gm_launch_rf(const gm_lanecfg &lc, const gm_dsetup *dS, const double *model
     int64_t mfs, mps;
     split_strides(model_stride, mfs, mps);
     /* alpha/beta: SoA over the state capacity (entry stride ab_stride, pixel
-     * stride 1) or, with ab_stride 0, the caller's AoS [npix][45] / [npix][9]
+     * stride 1), per-pixel records of -ab_stride floats (the LM state: one
+     * pixel's normal equations are written and read as contiguous lines,
+     * however sparse the active pixels), or, with ab_stride 0, the caller's
+     * AoS [npix][45] / [npix][9]
      * ([npix][66] / [npix][11] for eleven slots) */
     const int np = lc.np == 11 ? 11 : 9;
     const int64_t aes = ab_stride > 0 ? ab_stride : 1;
-    const int64_t aps = ab_stride > 0 ? 1 : np * (np + 1) / 2;
+    const int64_t aps = ab_stride > 0 ? 1 : (ab_stride < 0 ? -ab_stride : np * (np + 1) / 2);
     const int64_t bes = ab_stride > 0 ? ab_stride : 1;
-    const int64_t bps = ab_stride > 0 ? 1 : np;
+    const int64_t bps = ab_stride > 0 ? 1 : (ab_stride < 0 ? -ab_stride : np);
     if (lc.lpp == 32 && lc.nlpl == 1)
         launch_rf_t<32, 1>(lc, dS, models, pre, mfs, mps, obs, list, count, n_direct, grid_items, alpha,
                            beta, aes, aps, bes, bps, chi_out, synth_out, rf_out, mixed, st, alpha_first);

@@ -177,7 +177,7 @@ static gm_status build_engine(gm_server *srv, const gm_setup *setup, Engine **ou
     e->max_tiles = srv->cfg.max_tiles;
     const int64_t nl4 = 4 * (int64_t)e->nl;
     e->bytes_per_px = (int64_t)(sizeof(double) * (2 * GM_NMODEL + 5) +
-                                sizeof(float) * (GM_ALPHA_MAX + GM_BETA_MAX + 2 * GM_NPRE) +
+                                sizeof(float) * (GM_AB_REC11 + 2 * GM_NPRE) +
                                 sizeof(int32_t) * (2 + 4 + 3) + sizeof(float) * 2 * nl4);
     int64_t cap = srv->cfg.pool_pixels;
     if (cap <= 0) cap = std::min<int64_t>(srv->cfg.hbm_budget_bytes / e->bytes_per_px, (int64_t)1 << 22);
@@ -187,7 +187,7 @@ static gm_status build_engine(gm_server *srv, const gm_setup *setup, Engine **ou
     auto rnd = [](size_t b) { return (b + 255) & ~(size_t)255; };
     size_t bytes = 0;
     bytes += rnd(sizeof(double) * GM_NMODEL * cap) * 2;
-    bytes += rnd(sizeof(float) * GM_ALPHA_MAX * cap) + rnd(sizeof(float) * GM_BETA_MAX * cap);
+    bytes += rnd(sizeof(float) * GM_AB_REC11 * cap);
     bytes += rnd(sizeof(float) * GM_NPRE * cap) * 2;
     bytes += rnd(sizeof(double) * cap) * 5 + rnd(sizeof(int32_t) * cap) * 2;
     bytes += rnd(sizeof(int32_t) * cap) * 4;                       /* lists      */
@@ -202,8 +202,10 @@ static gm_status build_engine(gm_server *srv, const gm_setup *setup, Engine **ou
     s.cap = cap;
     s.best = (double *)take(sizeof(double) * GM_NMODEL * cap);
     s.trial = (double *)take(sizeof(double) * GM_NMODEL * cap);
-    s.alpha = (float *)take(sizeof(float) * GM_ALPHA_MAX * cap);
-    s.beta = (float *)take(sizeof(float) * GM_BETA_MAX * cap);
+    const bool p11 = e->lc.np == 11;
+    s.ab_rec = p11 ? GM_AB_REC11 : GM_AB_REC9;
+    s.alpha = (float *)take(sizeof(float) * GM_AB_REC11 * cap);
+    s.beta = s.alpha + (p11 ? GM_AB_BETA11 : GM_AB_BETA9);
     s.pre_best = (float *)take(sizeof(float) * GM_NPRE * cap);
     s.pre_trial = (float *)take(sizeof(float) * GM_NPRE * cap);
     s.chi_trial = (double *)take(sizeof(double) * cap);
@@ -278,7 +280,7 @@ static void admit(Engine &e, const Pending &pd, int t, int64_t slot0)
                          s);
     gm_launch_set_count(&e.cnt[4], (int32_t)pd.npix, s);
     gm_launch_rf(e.lc, e.dS, e.st.best, e.st.pre_best, e.cap, e.obs, e.newlist, &e.cnt[4], 0, pd.npix, e.st.alpha,
-                 e.st.beta, e.cap, e.st.ochisqr, e.synth, nullptr, e.mixed, s);
+                 e.st.beta, -e.st.ab_rec, e.st.ochisqr, e.synth, nullptr, e.mixed, s);
     gm_launch_init_chi(e.st, slot0, pd.npix, s);
     gm_launch_append_range(e.st.list[e.cur], &e.cnt[e.cur], slot0, pd.npix, s);
     e.inflight_px += pd.npix;
@@ -303,7 +305,7 @@ static void launch_round(Engine &e)
     gm_launch_decide(e.dS, st, st.list[cur], &cnt[cur], grid, st.list[nxt], &cnt[nxt], st.list[2],
                      &cnt[2], &cnt[3], e.done, &cnt[5], e.mixed, s);
     gm_launch_rf(e.lc, e.dS, st.best, st.pre_best, st.cap, e.obs, st.list[2], &cnt[2], 0, grid, st.alpha,
-                 st.beta, st.cap, nullptr, nullptr, nullptr, e.mixed, s, 0);
+                 st.beta, -st.ab_rec, nullptr, nullptr, nullptr, e.mixed, s, 0);
     gm_launch_retire(st, e.done, &cnt[5], grid, e.synth, 4 * e.nl, e.slot_tile, e.tiles_dev,
                      e.completed_dev, s);
     (void)hipMemcpyAsync(e.completed_host[slot], e.completed_dev, sizeof(int32_t) * (e.max_tiles + 1),
