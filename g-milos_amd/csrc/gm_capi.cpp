@@ -313,8 +313,8 @@ static gm_status ensure_state(gm_ctx *c, int64_t npix)
 {
     if (npix <= c->st.cap) return GM_OK;
     int64_t cap = ((npix + 255) / 256) * 256;
-    const size_t per = sizeof(double) * (2 * GM_NMODEL + 5) +
-                       sizeof(float) * (GM_AB_REC11 + 2 * GM_NPRE) +
+    const size_t per = sizeof(double) * (2 * GM_MODEL_REC + 5) +
+                       sizeof(float) * (GM_AB_REC11 + 2 * GM_PRE_REC) +
                        sizeof(int32_t) * (2 + 4);
     const size_t bytes = per * (size_t)cap + 256 * 22 + 64 + sizeof(int32_t) * 3 * (kMaxRounds + 2) + 256;
     if (c->ws) {
@@ -333,14 +333,14 @@ static gm_status ensure_state(gm_ctx *c, int64_t npix)
     };
     gm_state &st = c->st;
     st.cap = cap;
-    st.best = (double *)take(sizeof(double) * GM_NMODEL * cap);
-    st.trial = (double *)take(sizeof(double) * GM_NMODEL * cap);
+    st.best = (double *)take(sizeof(double) * GM_MODEL_REC * cap);
+    st.trial = (double *)take(sizeof(double) * GM_MODEL_REC * cap);
     const bool p11 = c->lc.np == 11;
     st.ab_rec = p11 ? GM_AB_REC11 : GM_AB_REC9;
     st.alpha = (float *)take(sizeof(float) * GM_AB_REC11 * cap);
     st.beta = st.alpha + (p11 ? GM_AB_BETA11 : GM_AB_BETA9);
-    st.pre_best = (float *)take(sizeof(float) * GM_NPRE * cap);
-    st.pre_trial = (float *)take(sizeof(float) * GM_NPRE * cap);
+    st.pre_best = (float *)take(sizeof(float) * GM_PRE_REC * cap);
+    st.pre_trial = (float *)take(sizeof(float) * GM_PRE_REC * cap);
     st.chi_trial = (double *)take(sizeof(double) * cap);
     st.ochisqr = (double *)take(sizeof(double) * cap);
     st.chisqr0 = (double *)take(sizeof(double) * cap);
@@ -408,7 +408,7 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
     gm_launch_set_count(&cnt[3], 0, s);
     kend();
     kbegin(1);
-    gm_launch_rf(lc, c->dS, st.best, st.pre_best, st.cap, obs, st.list[0], &cnt[0], 0, npix, st.alpha,
+    gm_launch_rf(lc, c->dS, st.best, st.pre_best, -GM_MODEL_REC, obs, st.list[0], &cnt[0], 0, npix, st.alpha,
                  st.beta, -st.ab_rec, st.ochisqr, out->synth, nullptr, mixed, s);
     kend();
     kbegin(3);
@@ -427,7 +427,7 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
                         &cnt[2], hist + 3 * r, st.list[3], &cnt[3], s, lc.np);
         kend();
         kbegin(0);
-        gm_launch_synth(lc, c->dS, st.trial, st.pre_trial, st.cap, obs, st.list[cur], &cnt[cur], 0,
+        gm_launch_synth(lc, c->dS, st.trial, st.pre_trial, -GM_MODEL_REC, obs, st.list[cur], &cnt[cur], 0,
                         npix, st.chi_trial, out->synth, mixed, s);
         kend();
         kbegin(3);
@@ -437,7 +437,7 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
         kbegin(1);
         /* alpha_first = 0: the filling-factor response of the reference's
          * loop (milosUtils.cu:673) */
-        gm_launch_rf(lc, c->dS, st.best, st.pre_best, st.cap, obs, st.list[2], &cnt[2], 0, npix, st.alpha,
+        gm_launch_rf(lc, c->dS, st.best, st.pre_best, -GM_MODEL_REC, obs, st.list[2], &cnt[2], 0, npix, st.alpha,
                      st.beta, -st.ab_rec, nullptr, nullptr, nullptr, mixed, s, 0);
         kend();
         cur = nxt;

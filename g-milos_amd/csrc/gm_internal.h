@@ -17,6 +17,8 @@
 #define GM_AB_BETA9  48
 #define GM_AB_BETA11 68
 #define GM_NPRE      19          /* per-pixel FP32 synthesis constants      */
+#define GM_PRE_REC   20          /* ... stored as 80-byte per-pixel records  */
+#define GM_MODEL_REC 12          /* LM state models: 96-byte per-pixel records */
 #define GM_RED_N     55          /* 45 alpha + 9 beta + 1 chi^2             */
 #define GM_SLOTS     36          /* staged values per wavelength: per Stokes
                                     [synth, eta0, B, vlos, dopp, a, gamma, phi,
@@ -69,14 +71,14 @@ struct gm_dsetup {
 /* Per-pixel LM state, structure-of-arrays over the pixel capacity. */
 struct gm_state {
     int64_t  cap;
-    double  *best;        /* [11][cap] best accepted model                  */
-    double  *trial;       /* [11][cap] last proposed model                  */
+    double  *best;        /* [cap][GM_MODEL_REC] best accepted model        */
+    double  *trial;       /* [cap][GM_MODEL_REC] last proposed model        */
     float   *alpha;       /* [cap][ab_rec] per-pixel records: packed normal
                              matrix at best (45 or 66 floats) ...            */
     float   *beta;        /* ... and the gradient at alpha + ab_beta         */
     int32_t  ab_rec;      /* record stride in floats (64, or 80 for 11 slots) */
-    float   *pre_best;    /* [GM_NPRE][cap] synthesis constants of best      */
-    float   *pre_trial;   /* [GM_NPRE][cap] ... of trial                     */
+    float   *pre_best;    /* [cap][GM_PRE_REC] synthesis constants of best   */
+    float   *pre_trial;   /* [cap][GM_PRE_REC] ... of trial                  */
     double  *chi_trial;   /* [cap] chi^2 of trial                           */
     double  *ochisqr;     /* [cap] chi^2 of best                            */
     double  *chisqr0;     /* [cap] initial chi^2                            */

@@ -126,21 +126,30 @@ struct PixPre {
 
 static_assert(sizeof(PixPre) == GM_NPRE * sizeof(float), "PixPre layout");
 
-/* PixPre rows of the LM state, SoA [GM_NPRE][stride]. */
-__device__ __forceinline__ void store_pre(float *__restrict__ pre, int64_t stride, int64_t pix,
-                                          const PixPre &p)
+/* PixPre records of the LM state, [pix][GM_PRE_REC] (80 bytes, 16-byte
+ * aligned): a pixel's constants are one contiguous read however sparse the
+ * active pixels are. */
+__device__ __forceinline__ void store_pre(float *__restrict__ pre, int64_t pix, const PixPre &p)
 {
     const float *v = reinterpret_cast<const float *>(&p);
+    float4 *r = reinterpret_cast<float4 *>(pre + pix * GM_PRE_REC);
 #pragma unroll
-    for (int f = 0; f < GM_NPRE; ++f) pre[f * stride + pix] = v[f];
+    for (int q = 0; q < 4; ++q) r[q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+    r[4] = make_float4(v[16], v[17], v[18], 0.f);
 }
-__device__ __forceinline__ PixPre load_pre(const float *__restrict__ pre, int64_t stride,
-                                           int64_t pix)
+__device__ __forceinline__ PixPre load_pre(const float *__restrict__ pre, int64_t pix)
 {
     PixPre p;
     float *v = reinterpret_cast<float *>(&p);
+    const float4 *r = reinterpret_cast<const float4 *>(pre + pix * GM_PRE_REC);
 #pragma unroll
-    for (int f = 0; f < GM_NPRE; ++f) v[f] = pre[f * stride + pix];
+    for (int q = 0; q < 5; ++q) {
+        const float4 x = r[q];
+        v[4 * q] = x.x;
+        v[4 * q + 1] = x.y;
+        v[4 * q + 2] = x.z;
+        if (q < 4) v[4 * q + 3] = x.w;
+    }
     return p;
 }
 
@@ -529,7 +538,7 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
         const int64_t pix = list ? (int64_t)list[valid ? item : base] : (valid ? item : base);
         PixPre pp;
         if (pre) {
-            pp = load_pre(pre, mfs, pix);
+            pp = load_pre(pre, pix);
         } else {
             double m[GM_NMODEL];
             load_model(models, mfs, mps, pix, m);
@@ -830,7 +839,7 @@ __global__ __launch_bounds__(64) void k_synth(
         const int64_t pix = list ? (int64_t)list[valid ? item : base] : (valid ? item : base);
         PixPre pp;
         if (pre) {
-            pp = load_pre(pre, mfs, pix);
+            pp = load_pre(pre, pix);
         } else {
             double m[GM_NMODEL];
             load_model(models, mfs, mps, pix, m);
@@ -1329,13 +1338,13 @@ __device__ __forceinline__ void apply_to_trial(const gm_dsetup *__restrict__ S, 
     double best[GM_NMODEL], m[GM_NMODEL];
 #pragma unroll
     for (int f = 0; f < GM_NMODEL; ++f) {
-        best[f] = st.best[f * cap + pix];
-        m[f] = st.trial[f * cap + pix];
+        best[f] = st.best[pix * GM_MODEL_REC + f];
+        m[f] = st.trial[pix * GM_MODEL_REC + f];
     }
     apply_delta_check<T>(S, best, x, m);
 #pragma unroll
-    for (int f = 0; f < GM_NMODEL; ++f) st.trial[f * cap + pix] = m[f];
-    store_pre(st.pre_trial, cap, pix, make_pre(m));
+    for (int f = 0; f < GM_NMODEL; ++f) st.trial[pix * GM_MODEL_REC + f] = m[f];
+    store_pre(st.pre_trial, pix, make_pre(m));
 }
 
 /* ---- eleven-slot systems (free filling factor) ----------------------------
@@ -1575,13 +1584,13 @@ __global__ __launch_bounds__(64) void k_solve11(const gm_dsetup *__restrict__ S,
         double best[GM_NMODEL], m[GM_NMODEL];
 #pragma unroll
         for (int f = 0; f < GM_NMODEL; ++f) {
-            best[f] = st.best[f * cap + pix];
-            m[f] = st.trial[f * cap + pix];
+            best[f] = st.best[pix * GM_MODEL_REC + f];
+            m[f] = st.trial[pix * GM_MODEL_REC + f];
         }
         apply_delta_check<double, 11>(S, best, x, m);
 #pragma unroll
-        for (int f = 0; f < GM_NMODEL; ++f) st.trial[f * cap + pix] = m[f];
-        store_pre(st.pre_trial, cap, pix, make_pre(m));
+        for (int f = 0; f < GM_NMODEL; ++f) st.trial[pix * GM_MODEL_REC + f] = m[f];
+        store_pre(st.pre_trial, pix, make_pre(m));
     }
 }
 
@@ -1735,10 +1744,13 @@ __global__ __launch_bounds__(1024) void k_decide(const gm_dsetup *__restrict__ S
                                            (T)st.factor[pix], it);
             if (d.accept) {
 #pragma unroll
-                for (int f = 0; f < GM_NMODEL; ++f) st.best[f * cap + pix] = st.trial[f * cap + pix];
+                for (int q = 0; q < GM_MODEL_REC / 2; ++q)
+                    reinterpret_cast<double2 *>(st.best + pix * GM_MODEL_REC)[q] =
+                        reinterpret_cast<const double2 *>(st.trial + pix * GM_MODEL_REC)[q];
 #pragma unroll
-                for (int f = 0; f < GM_NPRE; ++f)
-                    st.pre_best[f * cap + pix] = st.pre_trial[f * cap + pix];
+                for (int q = 0; q < GM_PRE_REC / 4; ++q)
+                    reinterpret_cast<float4 *>(st.pre_best + pix * GM_PRE_REC)[q] =
+                        reinterpret_cast<const float4 *>(st.pre_trial + pix * GM_PRE_REC)[q];
                 ochisqr = chisqr;
             }
             if (S->logclambda) {
@@ -1937,10 +1949,10 @@ __global__ __launch_bounds__(64) void k_init_state(const gm_dsetup *__restrict__
             }
 #pragma unroll
             for (int f = 0; f < GM_NMODEL; ++f) {
-                st.best[f * cap + q] = m[f];
-                st.trial[f * cap + q] = m[f];
+                st.best[q * GM_MODEL_REC + f] = m[f];
+                st.trial[q * GM_MODEL_REC + f] = m[f];
             }
-            store_pre(st.pre_best, cap, q, make_pre(m));
+            store_pre(st.pre_best, q, make_pre(m));
             st.flambda[q] = S->ilambda;
             st.factor[q] = 1.0;
             st.iter[q] = 0;
@@ -1998,7 +2010,7 @@ __global__ __launch_bounds__(256) void k_retire(gm_state st, const int32_t *__re
         const int64_t idx = slot - g.slot0;
         if (g.synth)
             for (int k = lane; k < nl4; k += 64) g.synth[idx * nl4 + k] = synth[slot * nl4 + k];
-        if (lane < GM_NMODEL && g.model) g.model[idx * GM_NMODEL + lane] = st.best[lane * cap + slot];
+        if (lane < GM_NMODEL && g.model) g.model[idx * GM_NMODEL + lane] = st.best[slot * GM_MODEL_REC + lane];
         if (lane == 0) {
             if (g.chisqr) g.chisqr[idx] = (float)st.ochisqr[slot];
             if (g.iters) g.iters[idx] = st.iter[slot];
@@ -2018,7 +2030,7 @@ __global__ void k_finalize(gm_state st, int64_t npix, double *__restrict__ model
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npix;
          p += (int64_t)gridDim.x * blockDim.x) {
         if (model)
-            for (int f = 0; f < GM_NMODEL; ++f) model[p * GM_NMODEL + f] = st.best[f * cap + p];
+            for (int f = 0; f < GM_NMODEL; ++f) model[p * GM_NMODEL + f] = st.best[p * GM_MODEL_REC + f];
         if (chisqr) chisqr[p] = (float)st.ochisqr[p];
         if (iters) iters[p] = st.iter[p];
         if (naccs) naccs[p] = st.nacc[p];
@@ -2097,6 +2109,7 @@ void launch_synth_t(const gm_lanecfg &lc, const gm_dsetup *dS, const double *mod
 static void split_strides(int64_t model_stride, int64_t &mfs, int64_t &mps)
 {
     if (model_stride > 0) { mfs = model_stride; mps = 1; }
+    else if (model_stride < 0) { mfs = 1; mps = -model_stride; }   /* state records */
     else { mfs = 1; mps = GM_NMODEL; }
 }
 

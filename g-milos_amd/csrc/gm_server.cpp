@@ -176,8 +176,8 @@ static gm_status build_engine(gm_server *srv, const gm_setup *setup, Engine **ou
     e->init_mode = ps.init_mode;
     e->max_tiles = srv->cfg.max_tiles;
     const int64_t nl4 = 4 * (int64_t)e->nl;
-    e->bytes_per_px = (int64_t)(sizeof(double) * (2 * GM_NMODEL + 5) +
-                                sizeof(float) * (GM_AB_REC11 + 2 * GM_NPRE) +
+    e->bytes_per_px = (int64_t)(sizeof(double) * (2 * GM_MODEL_REC + 5) +
+                                sizeof(float) * (GM_AB_REC11 + 2 * GM_PRE_REC) +
                                 sizeof(int32_t) * (2 + 4 + 3) + sizeof(float) * 2 * nl4);
     int64_t cap = srv->cfg.pool_pixels;
     if (cap <= 0) cap = std::min<int64_t>(srv->cfg.hbm_budget_bytes / e->bytes_per_px, (int64_t)1 << 22);
@@ -186,9 +186,9 @@ static gm_status build_engine(gm_server *srv, const gm_setup *setup, Engine **ou
     e->cap = cap;
     auto rnd = [](size_t b) { return (b + 255) & ~(size_t)255; };
     size_t bytes = 0;
-    bytes += rnd(sizeof(double) * GM_NMODEL * cap) * 2;
+    bytes += rnd(sizeof(double) * GM_MODEL_REC * cap) * 2;
     bytes += rnd(sizeof(float) * GM_AB_REC11 * cap);
-    bytes += rnd(sizeof(float) * GM_NPRE * cap) * 2;
+    bytes += rnd(sizeof(float) * GM_PRE_REC * cap) * 2;
     bytes += rnd(sizeof(double) * cap) * 5 + rnd(sizeof(int32_t) * cap) * 2;
     bytes += rnd(sizeof(int32_t) * cap) * 4;                       /* lists      */
     bytes += rnd(sizeof(float) * nl4 * cap) * 2;                   /* obs, synth */
@@ -200,14 +200,14 @@ static gm_status build_engine(gm_server *srv, const gm_setup *setup, Engine **ou
     auto take = [&](size_t b) { char *r = p; p += rnd(b); return (void *)r; };
     gm_state &s = e->st;
     s.cap = cap;
-    s.best = (double *)take(sizeof(double) * GM_NMODEL * cap);
-    s.trial = (double *)take(sizeof(double) * GM_NMODEL * cap);
+    s.best = (double *)take(sizeof(double) * GM_MODEL_REC * cap);
+    s.trial = (double *)take(sizeof(double) * GM_MODEL_REC * cap);
     const bool p11 = e->lc.np == 11;
     s.ab_rec = p11 ? GM_AB_REC11 : GM_AB_REC9;
     s.alpha = (float *)take(sizeof(float) * GM_AB_REC11 * cap);
     s.beta = s.alpha + (p11 ? GM_AB_BETA11 : GM_AB_BETA9);
-    s.pre_best = (float *)take(sizeof(float) * GM_NPRE * cap);
-    s.pre_trial = (float *)take(sizeof(float) * GM_NPRE * cap);
+    s.pre_best = (float *)take(sizeof(float) * GM_PRE_REC * cap);
+    s.pre_trial = (float *)take(sizeof(float) * GM_PRE_REC * cap);
     s.chi_trial = (double *)take(sizeof(double) * cap);
     s.ochisqr = (double *)take(sizeof(double) * cap);
     s.chisqr0 = (double *)take(sizeof(double) * cap);
@@ -279,7 +279,7 @@ static void admit(Engine &e, const Pending &pd, int t, int64_t slot0)
     gm_launch_init_state(e.dS, e.st, e.obs, pd.init, e.init_mode, e.nl, slot0, pd.npix, e.newlist,
                          s);
     gm_launch_set_count(&e.cnt[4], (int32_t)pd.npix, s);
-    gm_launch_rf(e.lc, e.dS, e.st.best, e.st.pre_best, e.cap, e.obs, e.newlist, &e.cnt[4], 0, pd.npix, e.st.alpha,
+    gm_launch_rf(e.lc, e.dS, e.st.best, e.st.pre_best, -GM_MODEL_REC, e.obs, e.newlist, &e.cnt[4], 0, pd.npix, e.st.alpha,
                  e.st.beta, -e.st.ab_rec, e.st.ochisqr, e.synth, nullptr, e.mixed, s);
     gm_launch_init_chi(e.st, slot0, pd.npix, s);
     gm_launch_append_range(e.st.list[e.cur], &e.cnt[e.cur], slot0, pd.npix, s);
@@ -300,11 +300,11 @@ static void launch_round(Engine &e)
     gm_launch_set_count(e.completed_dev, 0, s);
     gm_launch_solve(e.dS, st, st.list[cur], &cnt[cur], grid, e.mixed, e.solver, &cnt[nxt], &cnt[2],
                     nullptr, st.list[3], &cnt[3], s, e.lc.np);
-    gm_launch_synth(e.lc, e.dS, st.trial, st.pre_trial, st.cap, e.obs, st.list[cur], &cnt[cur], 0, grid,
+    gm_launch_synth(e.lc, e.dS, st.trial, st.pre_trial, -GM_MODEL_REC, e.obs, st.list[cur], &cnt[cur], 0, grid,
                     st.chi_trial, e.synth, e.mixed, s);
     gm_launch_decide(e.dS, st, st.list[cur], &cnt[cur], grid, st.list[nxt], &cnt[nxt], st.list[2],
                      &cnt[2], &cnt[3], e.done, &cnt[5], e.mixed, s);
-    gm_launch_rf(e.lc, e.dS, st.best, st.pre_best, st.cap, e.obs, st.list[2], &cnt[2], 0, grid, st.alpha,
+    gm_launch_rf(e.lc, e.dS, st.best, st.pre_best, -GM_MODEL_REC, e.obs, st.list[2], &cnt[2], 0, grid, st.alpha,
                  st.beta, -st.ab_rec, nullptr, nullptr, nullptr, e.mixed, s, 0);
     gm_launch_retire(st, e.done, &cnt[5], grid, e.synth, 4 * e.nl, e.slot_tile, e.tiles_dev,
                      e.completed_dev, s);
