@@ -307,30 +307,74 @@ __device__ __forceinline__ void eval_lambda(const gm_dsetup *__restrict__ S,
     d_ru[6] = parcial3 * (pp.sinis * pp.sinda);
     d_rv[6] = 0.f;
 
-    /* chain rule through the ME solution: me_der.cu:317-373 */
+    /* chain rule through the ME solution (me_der.cu:317-373).  Every
+     * response is linear in its parameter's seven absorption-matrix
+     * derivatives (d eta_I,Q,U,V, d rho_Q,U,V), with coefficients that
+     * depend only on the wavelength: out[s][p] = sum_m C[s][m] d_m[p].  The
+     * coefficients are formed once per wavelength from the reference's
+     * expressions for dgp1..dgp6 and d(dt), then applied to the seven
+     * parameters.                                                          */
     const float dtaux = (-(pp.S1 * ah)) * dti * dti;
     const float etai_gp3 = etai * gp3;
     const float a2 = 2.f * etai;
     const float ext1 = a2 * etaq + etav * rhou - etau * rhov;
     const float ext2 = a2 * etau + etaq * rhov - etav * rhoq;
     const float ext3 = a2 * etav + etau * rhoq - etaq * rhou;
-    const float ext4 = a2 * gp1;
+    /* d(dt) = sum_m cdt[m] d_m, from ext4 ei + etai^2 dgp1 - 2 gp2 dgp2 */
+    const float t2e = 2.f * etai_2, t2g = 2.f * gp2;
+    float cdt[7];
+    cdt[0] = a2 * gp1 + t2e * etai;
+    cdt[1] = -(t2e * etaq + t2g * rhoq);
+    cdt[2] = -(t2e * etau + t2g * rhou);
+    cdt[3] = -(t2e * etav + t2g * rhov);
+    cdt[4] = t2e * rhoq - t2g * etaq;
+    cdt[5] = t2e * rhou - t2g * etau;
+    cdt[6] = t2e * rhov - t2g * etav;
+    /* the Delta-multiplied numerators' coefficients, per Stokes parameter */
+    float ca[4][7];
+    ca[0][0] = gp3 + a2 * etai;  ca[0][1] = 0.f;  ca[0][2] = 0.f;  ca[0][3] = 0.f;
+    ca[0][4] = a2 * rhoq;        ca[0][5] = a2 * rhou;  ca[0][6] = a2 * rhov;
+    ca[1][0] = ext1;
+    ca[1][1] = etai_2 + rhoq * rhoq;
+    ca[1][2] = rhoq * rhou - etai * rhov;
+    ca[1][3] = rhoq * rhov + etai * rhou;
+    ca[1][4] = gp2 + rhoq * etaq;
+    ca[1][5] = rhoq * etau + etai * etav;
+    ca[1][6] = rhoq * etav - etai * etau;
+    ca[2][0] = ext2;
+    ca[2][1] = rhou * rhoq + etai * rhov;
+    ca[2][2] = etai_2 + rhou * rhou;
+    ca[2][3] = rhou * rhov - etai * rhoq;
+    ca[2][4] = rhou * etaq - etai * etav;
+    ca[2][5] = gp2 + rhou * etau;
+    ca[2][6] = rhou * etav + etai * etaq;
+    ca[3][0] = ext3;
+    ca[3][1] = rhov * rhoq - etai * rhou;
+    ca[3][2] = rhov * rhou + etai * rhoq;
+    ca[3][3] = etai_2 + rhov * rhov;
+    ca[3][4] = rhov * etaq + etai * etau;
+    ca[3][5] = rhov * etau - etai * etaq;
+    ca[3][6] = gp2 + rhov * etav;
+    /* out = dtaux (Delta a - g d(dt)); I carries the opposite sign */
+    const float gq[4] = {etai_gp3, g4, g5, g6};
+    const float sg[4] = {-1.f, 1.f, 1.f, 1.f};
+    float C[4][7];
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+        const float u = sg[st] * dt * dtaux, v = sg[st] * gq[st] * dtaux;
+#pragma unroll
+        for (int m = 0; m < 7; ++m) C[st][m] = u * ca[st][m] - v * cdt[m];
+    }
 #pragma unroll
     for (int p = 0; p < 7; ++p) {
-        const float ei = d_ei[p], eq = d_eq[p], eu = d_eu[p], ev = d_ev[p];
-        const float rq = d_rq[p], ru = d_ru[p], rv = d_rv[p];
-        const float dgp1 = 2.f * (etai * ei - etaq * eq - etau * eu - etav * ev +
-                                  rhoq * rq + rhou * ru + rhov * rv);
-        const float dgp2 = rhoq * eq + etaq * rq + rhou * eu + etau * ru + rhov * ev + etav * rv;
-        const float d_dt = ext4 * ei + etai_2 * dgp1 - 2.f * gp2 * dgp2;
-        const float dgp3 = 2.f * (etai * ei + rhoq * rq + rhou * ru + rhov * rv);
-        const float dgp4 = ei * ext1 + etai_2 * eq + etai * (rhou * ev + etav * ru - rhov * eu - etau * rv);
-        const float dgp5 = ei * ext2 + etai_2 * eu + etai * (rhov * eq + etaq * rv - rhoq * ev - etav * rq);
-        const float dgp6 = ei * ext3 + etai_2 * ev + etai * (rhoq * eu + etau * rq - rhou * eq - etaq * ru);
-        out[0 + 1 + p] = -(((ei * gp3 + etai * dgp3) * dt - d_dt * etai_gp3) * dtaux);
-        out[9 + 1 + p] = ((dgp4 + rq * gp2 + rhoq * dgp2) * dt - d_dt * g4) * dtaux;
-        out[18 + 1 + p] = ((dgp5 + ru * gp2 + rhou * dgp2) * dt - d_dt * g5) * dtaux;
-        out[27 + 1 + p] = ((dgp6 + rv * gp2 + rhov * dgp2) * dt - d_dt * g6) * dtaux;
+        const float dm[7] = {d_ei[p], d_eq[p], d_eu[p], d_ev[p], d_rq[p], d_ru[p], d_rv[p]};
+#pragma unroll
+        for (int st = 0; st < 4; ++st) {
+            float o = 0.f;
+#pragma unroll
+            for (int m = 0; m < 7; ++m) o += C[st][m] * dm[m];
+            out[st * 9 + 1 + p] = o;
+        }
     }
     const float dti2 = -(dti * ah);
     out[0 + 8] = -dti2 * etai_gp3;
