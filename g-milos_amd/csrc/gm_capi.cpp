@@ -432,13 +432,16 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
         kend();
         kbegin(3);
         gm_launch_decide(c->dS, st, st.list[cur], &cnt[cur], npix, st.list[nxt], &cnt[nxt],
-                         st.list[2], &cnt[2], &cnt[3], nullptr, nullptr, mixed, s);
+                         st.list[2], &cnt[2], &cnt[3], nullptr, nullptr, mixed, s, 1);
         kend();
         kbegin(1);
         /* alpha_first = 0: the filling-factor response of the reference's
          * loop (milosUtils.cu:673) */
-        gm_launch_rf(lc, c->dS, st.best, st.pre_best, -GM_MODEL_REC, obs, st.list[2], &cnt[2], 0, npix, st.alpha,
-                     st.beta, -st.ab_rec, nullptr, nullptr, nullptr, mixed, s, 0);
+        /* accepted pixels: normal equations at the trial, which becomes the
+         * best state (copied by the RF kernel) */
+        gm_launch_rf(lc, c->dS, st.trial, st.pre_trial, -GM_MODEL_REC, obs, st.list[2], &cnt[2], 0, npix,
+                     st.alpha, st.beta, -st.ab_rec, nullptr, nullptr, nullptr, mixed, s, 0, st.best,
+                     st.pre_best);
         kend();
         cur = nxt;
     }

@@ -126,7 +126,8 @@ void gm_launch_rf(const gm_lanecfg &lc, const gm_dsetup *dS, const double *model
                   const int32_t *count, int64_t n_direct, int64_t grid_items,
                   float *alpha, float *beta, int64_t ab_stride, double *chi_out,
                   float *synth_out, float *rf_out, int mixed, hipStream_t st,
-                  int alpha_first = 1);
+                  int alpha_first = 1, double *model_dst = nullptr,
+                  float *pre_dst = nullptr);
 void gm_launch_synth(const gm_lanecfg &lc, const gm_dsetup *dS, const double *models,
                      const float *pre, int64_t model_stride, const float *obs, const int32_t *list,
                      const int32_t *count, int64_t n_direct, int64_t grid_items,
@@ -142,7 +143,7 @@ void gm_launch_decide(const gm_dsetup *dS, gm_state st, const int32_t *list,
                       const int32_t *count, int64_t grid_items, int32_t *next_list,
                       int32_t *next_count, int32_t *acc_list, int32_t *acc_count,
                       int32_t *reset_fb, int32_t *done_list, int32_t *done_count, int mixed,
-                      hipStream_t stream);
+                      hipStream_t stream, int defer_copy = 0);
 void gm_launch_init_state(const gm_dsetup *dS, gm_state st, const float *obs,
                           const double *init, int init_mode, int nlambda, int64_t slot0,
                           int64_t npix, int32_t *list0, hipStream_t stream);

@@ -545,7 +545,7 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
     int64_t n_direct, float *__restrict__ alpha, float *__restrict__ beta,
     int64_t aes, int64_t aps, int64_t bes, int64_t bps, double *__restrict__ chi_out,
     float *__restrict__ synth_out, float *__restrict__ rf_out, int rows, int padlo,
-    int alpha_first)
+    int alpha_first, double *__restrict__ model_dst, float *__restrict__ pre_dst)
 {
     /* XM: 0 the 9-parameter kernel, 1 with macroturbulence / stray light,
      * 2 the eleven-slot system (slot 9 mac, slot 10 filling factor) */
@@ -588,6 +588,15 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
             load_model(models, mfs, mps, pix, m);
             pp = make_pre(m);
         }
+        /* the accepted trial becomes the best state here rather than in
+         * k_decide: lanes 0-10 load its model and constants now and store
+         * them at the end of the pixel, behind the computation */
+        const bool copy_rec = model_dst && valid && k0 < GM_MODEL_REC / 2 + GM_PRE_REC / 4;
+        float4 rec = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (copy_rec)
+            rec = (k0 < GM_MODEL_REC / 2)
+                      ? reinterpret_cast<const float4 *>(models + pix * GM_MODEL_REC)[k0]
+                      : reinterpret_cast<const float4 *>(pre + pix * GM_PRE_REC)[k0 - GM_MODEL_REC / 2];
         /* observed samples of this lane's wavelengths, loaded before the
          * synthesis so that their latency is hidden behind it */
         float ob[NLPL][4];
@@ -845,6 +854,12 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
         }
         if (MIXED) chi = chi / (double)nfree;
         if (chi_out && valid && k0 == 0) chi_out[pix] = chi;
+        if (copy_rec) {
+            if (k0 < GM_MODEL_REC / 2)
+                reinterpret_cast<float4 *>(model_dst + pix * GM_MODEL_REC)[k0] = rec;
+            else
+                reinterpret_cast<float4 *>(pre_dst + pix * GM_PRE_REC)[k0 - GM_MODEL_REC / 2] = rec;
+        }
         __syncthreads();
     }
 }
@@ -1766,7 +1781,8 @@ __global__ __launch_bounds__(1024) void k_decide(const gm_dsetup *__restrict__ S
                                                  int32_t *__restrict__ acc_count,
                                                  int32_t *__restrict__ reset_fb,
                                                  int32_t *__restrict__ done_list,
-                                                 int32_t *__restrict__ done_count)
+                                                 int32_t *__restrict__ done_count,
+                                                 int defer_copy)
 {
     using T = typename std::conditional<MIXED, double, float>::type;
     __shared__ int wc[16], wa[16], wd[16];
@@ -1786,7 +1802,9 @@ __global__ __launch_bounds__(1024) void k_decide(const gm_dsetup *__restrict__ S
             const int it = st.iter[pix];
             const LmStep<T> d = lm_step<T>(S, chisqr, ochisqr, (T)st.flambda[pix],
                                            (T)st.factor[pix], it);
-            if (d.accept) {
+            /* with defer_copy the RF kernel copies the accepted trial of
+             * a continuing pixel; finished pixels are copied here */
+            if (d.accept && !(defer_copy && d.cont)) {
 #pragma unroll
                 for (int q = 0; q < GM_MODEL_REC / 2; ++q)
                     reinterpret_cast<double2 *>(st.best + pix * GM_MODEL_REC)[q] =
@@ -1795,8 +1813,8 @@ __global__ __launch_bounds__(1024) void k_decide(const gm_dsetup *__restrict__ S
                 for (int q = 0; q < GM_PRE_REC / 4; ++q)
                     reinterpret_cast<float4 *>(st.pre_best + pix * GM_PRE_REC)[q] =
                         reinterpret_cast<const float4 *>(st.pre_trial + pix * GM_PRE_REC)[q];
-                ochisqr = chisqr;
             }
+            if (d.accept) ochisqr = chisqr;
             if (S->logclambda) {
                 const T c0 = (T)st.chisqr0[pix];
                 st.factor[pix] = (double)(log10(chisqr) / log10(c0));
@@ -2099,7 +2117,8 @@ void launch_rf_t(const gm_lanecfg &lc, const gm_dsetup *dS, const double *models
                  const float *pre, int64_t mfs, int64_t mps, const float *obs, const int32_t *list,
                  const int32_t *count, int64_t n_direct, int64_t grid_items, float *alpha,
                  float *beta, int64_t aes, int64_t aps, int64_t bes, int64_t bps, double *chi,
-                 float *synth, float *rf, int mixed, hipStream_t s, int alpha_first)
+                 float *synth, float *rf, int mixed, hipStream_t s, int alpha_first,
+                 double *model_dst, float *pre_dst)
 {
     const int xm = lc.np == 11 ? 2 : (lc.ext ? 1 : 0);
     const int red4 = (LPP / 2) * (xm == 2 ? 20 : 15);
@@ -2110,7 +2129,7 @@ void launch_rf_t(const gm_lanecfg &lc, const gm_dsetup *dS, const double *models
 #define GM_RF_CASE(MX, X)                                                                        \
     hipLaunchKernelGGL((k_rf<LPP, NLPL, MX, X>), dim3(g), dim3(64), shm, s, dS, models, pre, mfs,  \
                        mps, obs, list, count, n_direct, alpha, beta, aes, aps, bes, bps, chi, synth,  \
-                       rf, lc.rows, lc.padlo, alpha_first)
+                       rf, lc.rows, lc.padlo, alpha_first, model_dst, pre_dst)
     if (xm == 2) {
         if (mixed) GM_RF_CASE(true, 2);
         else GM_RF_CASE(false, 2);
@@ -2161,7 +2180,8 @@ void gm_launch_rf(const gm_lanecfg &lc, const gm_dsetup *dS, const double *model
                   const float *pre, int64_t model_stride, const float *obs, const int32_t *list,
                   const int32_t *count, int64_t n_direct, int64_t grid_items, float *alpha,
                   float *beta, int64_t ab_stride, double *chi_out, float *synth_out,
-                  float *rf_out, int mixed, hipStream_t st, int alpha_first)
+                  float *rf_out, int mixed, hipStream_t st, int alpha_first,
+                  double *model_dst, float *pre_dst)
 {
     int64_t mfs, mps;
     split_strides(model_stride, mfs, mps);
@@ -2178,16 +2198,20 @@ void gm_launch_rf(const gm_lanecfg &lc, const gm_dsetup *dS, const double *model
     const int64_t bps = ab_stride > 0 ? 1 : (ab_stride < 0 ? -ab_stride : np);
     if (lc.lpp == 32 && lc.nlpl == 1)
         launch_rf_t<32, 1>(lc, dS, models, pre, mfs, mps, obs, list, count, n_direct, grid_items, alpha,
-                           beta, aes, aps, bes, bps, chi_out, synth_out, rf_out, mixed, st, alpha_first);
+                           beta, aes, aps, bes, bps, chi_out, synth_out, rf_out, mixed, st, alpha_first,
+                           model_dst, pre_dst);
     else if (lc.lpp == 64 && lc.nlpl == 1)
         launch_rf_t<64, 1>(lc, dS, models, pre, mfs, mps, obs, list, count, n_direct, grid_items, alpha,
-                           beta, aes, aps, bes, bps, chi_out, synth_out, rf_out, mixed, st, alpha_first);
+                           beta, aes, aps, bes, bps, chi_out, synth_out, rf_out, mixed, st, alpha_first,
+                           model_dst, pre_dst);
     else if (lc.lpp == 64 && lc.nlpl == 2)
         launch_rf_t<64, 2>(lc, dS, models, pre, mfs, mps, obs, list, count, n_direct, grid_items, alpha,
-                           beta, aes, aps, bes, bps, chi_out, synth_out, rf_out, mixed, st, alpha_first);
+                           beta, aes, aps, bes, bps, chi_out, synth_out, rf_out, mixed, st, alpha_first,
+                           model_dst, pre_dst);
     else
         launch_rf_t<64, 4>(lc, dS, models, pre, mfs, mps, obs, list, count, n_direct, grid_items, alpha,
-                           beta, aes, aps, bes, bps, chi_out, synth_out, rf_out, mixed, st, alpha_first);
+                           beta, aes, aps, bes, bps, chi_out, synth_out, rf_out, mixed, st, alpha_first,
+                           model_dst, pre_dst);
 }
 
 void gm_launch_synth(const gm_lanecfg &lc, const gm_dsetup *dS, const double *models,
@@ -2274,17 +2298,17 @@ void gm_launch_decide(const gm_dsetup *dS, gm_state st, const int32_t *list,
                       const int32_t *count, int64_t grid_items, int32_t *next_list,
                       int32_t *next_count, int32_t *acc_list, int32_t *acc_count,
                       int32_t *reset_fb, int32_t *done_list, int32_t *done_count, int mixed,
-                      hipStream_t s)
+                      hipStream_t s, int defer_copy)
 {
     const int g = grid_for(grid_items, 1024, 512);
     if (mixed)
         hipLaunchKernelGGL((k_decide<true>), dim3(g), dim3(1024), 0, s, dS, st, list, count,
                            next_list, next_count, acc_list, acc_count, reset_fb, done_list,
-                           done_count);
+                           done_count, defer_copy);
     else
         hipLaunchKernelGGL((k_decide<false>), dim3(g), dim3(1024), 0, s, dS, st, list, count,
                            next_list, next_count, acc_list, acc_count, reset_fb, done_list,
-                           done_count);
+                           done_count, defer_copy);
 }
 
 void gm_launch_init_state(const gm_dsetup *dS, gm_state st, const float *obs,

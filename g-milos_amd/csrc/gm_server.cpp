@@ -303,9 +303,10 @@ static void launch_round(Engine &e)
     gm_launch_synth(e.lc, e.dS, st.trial, st.pre_trial, -GM_MODEL_REC, e.obs, st.list[cur], &cnt[cur], 0, grid,
                     st.chi_trial, e.synth, e.mixed, s);
     gm_launch_decide(e.dS, st, st.list[cur], &cnt[cur], grid, st.list[nxt], &cnt[nxt], st.list[2],
-                     &cnt[2], &cnt[3], e.done, &cnt[5], e.mixed, s);
-    gm_launch_rf(e.lc, e.dS, st.best, st.pre_best, -GM_MODEL_REC, e.obs, st.list[2], &cnt[2], 0, grid, st.alpha,
-                 st.beta, -st.ab_rec, nullptr, nullptr, nullptr, e.mixed, s, 0);
+                     &cnt[2], &cnt[3], e.done, &cnt[5], e.mixed, s, 1);
+    gm_launch_rf(e.lc, e.dS, st.trial, st.pre_trial, -GM_MODEL_REC, e.obs, st.list[2], &cnt[2], 0, grid,
+                 st.alpha, st.beta, -st.ab_rec, nullptr, nullptr, nullptr, e.mixed, s, 0, st.best,
+                 st.pre_best);
     gm_launch_retire(st, e.done, &cnt[5], grid, e.synth, 4 * e.nl, e.slot_tile, e.tiles_dev,
                      e.completed_dev, s);
     (void)hipMemcpyAsync(e.completed_host[slot], e.completed_dev, sizeof(int32_t) * (e.max_tiles + 1),
