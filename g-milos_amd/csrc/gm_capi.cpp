@@ -428,7 +428,7 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
         kend();
         kbegin(0);
         gm_launch_synth(lc, c->dS, st.trial, st.pre_trial, -GM_MODEL_REC, obs, st.list[cur], &cnt[cur], 0,
-                        npix, st.chi_trial, out->synth, mixed, s);
+                        npix, st.chi_trial, nullptr, mixed, s);
         kend();
         kbegin(3);
         gm_launch_decide(c->dS, st, st.list[cur], &cnt[cur], npix, st.list[nxt], &cnt[nxt],
@@ -446,6 +446,15 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
         cur = nxt;
     }
     gm_launch_copy_count(&cnt[2], hist + 3 * miter - 1, s);
+    /* the synthetic output is the last evaluated model's spectrum (the
+     * reference's spectra buffer after its loop): one synthesis of every
+     * pixel's last trial, instead of writing every trial's spectrum */
+    if (out->synth && miter > 0) {
+        kbegin(0);
+        gm_launch_synth(lc, c->dS, st.trial, st.pre_trial, -GM_MODEL_REC, nullptr, nullptr, nullptr,
+                        npix, npix, nullptr, out->synth, mixed, s);
+        kend();
+    }
     kbegin(3);
     gm_launch_finalize(st, npix, *out, s);
     kend();
