@@ -409,7 +409,7 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
     kend();
     kbegin(1);
     gm_launch_rf(lc, c->dS, st.best, st.pre_best, -GM_MODEL_REC, obs, st.list[0], &cnt[0], 0, npix, st.alpha,
-                 st.beta, -st.ab_rec, st.ochisqr, out->synth, nullptr, mixed, s);
+                 st.beta, -st.ab_rec, st.ochisqr, miter > 0 ? nullptr : out->synth, nullptr, mixed, s);
     kend();
     kbegin(3);
     gm_launch_init_chi(st, 0, npix, s);
