@@ -409,7 +409,7 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
     kend();
     kbegin(1);
     gm_launch_rf(lc, c->dS, st.best, st.pre_best, -GM_MODEL_REC, obs, st.list[0], &cnt[0], 0, npix, st.alpha,
-                 st.beta, -st.ab_rec, st.ochisqr, miter > 0 ? nullptr : out->synth, nullptr, mixed, s);
+                 st.beta, -st.ab_rec, st.ochisqr, nullptr, nullptr, mixed, s);
     kend();
     kbegin(3);
     gm_launch_init_chi(st, 0, npix, s);
@@ -449,7 +449,7 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
     /* the synthetic output is the last evaluated model's spectrum (the
      * reference's spectra buffer after its loop): one synthesis of every
      * pixel's last trial, instead of writing every trial's spectrum */
-    if (out->synth && miter > 0) {
+    if (out->synth) {
         kbegin(0);
         gm_launch_synth(lc, c->dS, st.trial, st.pre_trial, -GM_MODEL_REC, nullptr, nullptr, nullptr,
                         npix, npix, nullptr, out->synth, mixed, s);
