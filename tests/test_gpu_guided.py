@@ -87,3 +87,43 @@ def test_learned_guess_cuts_iterations_at_equal_error(gm, orc, cuda, trained):
     chi = out["chisqr"].cpu().numpy()
     assert st["n_fallback"] >= st["n_replaced"]
     assert np.all(np.isfinite(chi))
+
+
+def test_untrained_network_falls_back_to_fixed_guess(gm, cuda):
+    """A deliberately poor learned guess (random weights): the pixels whose
+    chi^2 exceeds the threshold are re-inverted from the fixed guess and
+    keep the lower chi^2, so no pixel ends worse than either path; with a
+    threshold of zero every pixel falls back and the result is the plain
+    inversion's wherever the plain chi^2 is lower."""
+    import torch
+    n = 4096
+    truth = gm.draw_models(n, seed=901)
+    plain = gm.Context(gm.setup_single_line(psf_fwhm=60.0, precision=gm.GM_PREC_MIXED))
+    # two LM iterations from the untrained guess leave most pixels far from
+    # their optimum
+    learned = gm.Context(gm.setup_single_line(psf_fwhm=60.0, precision=gm.GM_PREC_MIXED,
+                                              init_mode=gm.GM_INIT_PER_PIXEL, miter=2))
+    obs = plain.synth(truth)
+    gm.add_noise(truth, obs, 1e-3, seed=902)
+    torch.cuda.synchronize()
+    ref = plain.invert(obs)
+    net = gm.MLP(30, seed=5)                         # never trained
+    thr = float(np.percentile(ref["chisqr"].cpu().numpy(), 99))
+    out, st = gm.invert_guided(learned, plain, net, obs, thr)
+    c_out = out["chisqr"].cpu().numpy()
+    c_ref = ref["chisqr"].cpu().numpy()
+    assert st["n_fallback"] > 0.2 * n                # the poor guess is caught
+    assert st["n_replaced"] > 0
+    refit = c_out > thr
+    # every pixel above the threshold after the fallback is one the plain
+    # path could not fit below it either
+    assert np.all(c_ref[refit] > thr * (1 - 1e-6))
+    out0, st0 = gm.invert_guided(learned, plain, net, obs, 0.0)
+    assert st0["n_fallback"] == n
+    c0 = out0["chisqr"].cpu().numpy()
+    assert np.all(c0 <= c_ref)
+    m0 = out0["model"].cpu().numpy()
+    mref = ref["model"].cpu().numpy()
+    same = c0 == c_ref
+    assert same.mean() > 0.5
+    np.testing.assert_array_equal(m0[same], mref[same])
