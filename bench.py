@@ -253,7 +253,7 @@ def guided_leg(gm, torch, dev, npix, n_train, steps, seed, oracle_sample):
     torch.cuda.synchronize()
 
     def timed(fn):
-        fn(obs[:65536].contiguous())                       # warm-up on a slice
+        fn(obs)                                            # warm-up: buffers sized for the cube
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         r = fn(obs)

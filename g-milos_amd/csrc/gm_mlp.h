@@ -32,4 +32,17 @@ void gm_mlp_adamw(float *w, const float *g, float *m, float *v, uint16_t *wb, in
                   float b1, float b2, float eps, float wd, float bc1, float bc2, hipStream_t s);
 void gm_mlp_to_bf16(const float *x, uint16_t *y, int64_t n, hipStream_t s);
 
+/* Fused inference (kin <= GM_MLP_HIDDEN, kin % 64 == 0, obs 16-byte
+ * aligned): obs [npix][4][nl]
+ * -> start models [npix][11] in one kernel.  wt holds every layer's BF16
+ * weights transposed to [out][in] at the same offsets as the [in][out]
+ * copy; w the FP32 parameters (biases read from it). */
+struct gm_mlp_offsets {
+    int64_t woff[GM_MLP_LAYERS], boff[GM_MLP_LAYERS];
+};
+void gm_mlp_infer_fused(const float *obs, int64_t npix, int nl, int kin, const uint16_t *wt,
+                        const float *w, const gm_mlp_offsets &of, const double *base, double *models,
+                        hipStream_t s);
+void gm_mlp_transpose(const uint16_t *w, int in, int out, uint16_t *wt, hipStream_t s);
+
 #endif

@@ -33,18 +33,26 @@ __device__ __forceinline__ uint16_t f2bf(float x)
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
+/* tanh(u) = 1 - 2 / (1 + exp(2u)): one exponential and one division
+ * (absolute error ~1e-7, far below the BF16 rounding of the activations);
+ * saturates to +-1 through exp's overflow to inf and underflow to 0. */
+__device__ __forceinline__ float tanh_fast(float u)
+{
+    return 1.f - __fdividef(2.f, 1.f + __expf(2.f * u));
+}
+
 /* GELU, tanh form (Hendrycks & Gimpel 2016) and its derivative. */
 __device__ __forceinline__ float gelu(float x)
 {
     const float k = 0.7978845608028654f;   /* sqrt(2/pi) */
-    const float t = tanhf(k * (x + 0.044715f * x * x * x));
+    const float t = tanh_fast(k * (x + 0.044715f * x * x * x));
     return 0.5f * x * (1.f + t);
 }
 __device__ __forceinline__ float gelu_grad(float x)
 {
     const float k = 0.7978845608028654f;
     const float u = k * (x + 0.044715f * x * x * x);
-    const float t = tanhf(u);
+    const float t = tanh_fast(u);
     const float du = k * (1.f + 3.f * 0.044715f * x * x);
     return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * du;
 }
@@ -280,32 +288,209 @@ __global__ void k_prep_target(const double *__restrict__ models, int64_t npix,
 /* Output decode into a start model (init fields untouched where the network
  * has no output), clamped to the LM's physical ranges.  FP32 arithmetic:
  * the result is a starting point for the LM. */
+__device__ __forceinline__ void decode_row(const float *__restrict__ o, const double *__restrict__ base,
+                                           double *__restrict__ m)
+{
+    const float rad = 57.29577951308232f;
+    const float B = o[0] * 866.f + 1500.f;
+    const float sg = fmaxf((o[2] + 1.f) * 0.5f, 0.f);
+    const float gm = atan2f(sg, o[1]) * rad;
+    float az = 0.5f * atan2f(o[4], o[3]) * rad;
+    if (az < 0.f) az += 180.f;
+    const float ic = fmaxf(o[10] * 0.2f + 1.f, 0.05f);
+    const float s0f = fminf(fmaxf(o[9] * 0.1f + 0.25f, 0.01f), 0.99f);
+    m[0] = fminf(fmaxf(o[8] * 5.485f + 10.5f, 1.f), 2500.f);
+    m[1] = fminf(fmaxf(B, 0.f), 5000.f);
+    m[2] = fminf(fmaxf(o[5] * 1.732f, -20.f), 20.f);
+    m[3] = fminf(fmaxf(o[6] * 0.01155f + 0.04f, 0.005f), 0.6f);
+    m[4] = fminf(fmaxf(o[7] * 0.1414f + 0.255f, 0.0001f), 10.f);
+    m[5] = fminf(fmaxf(gm, 0.f), 180.f);
+    m[6] = fminf(fmaxf(az, 0.f), 180.f);
+    m[7] = s0f * ic;
+    m[8] = (1.f - s0f) * ic;
+    m[9] = base[9];
+    m[10] = base[10];
+}
+
 __global__ void k_decode(const float *__restrict__ y, int64_t npix, const double *__restrict__ base,
                          double *__restrict__ models)
 {
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npix;
-         p += (int64_t)gridDim.x * blockDim.x) {
-        const float *o = y + p * GM_MLP_NOUT;
-        double *m = models + p * 11;
-        const float rad = 57.29577951308232f;
-        const float B = o[0] * 866.f + 1500.f;
-        const float sg = fmaxf((o[2] + 1.f) * 0.5f, 0.f);
-        const float gm = atan2f(sg, o[1]) * rad;
-        float az = 0.5f * atan2f(o[4], o[3]) * rad;
-        if (az < 0.f) az += 180.f;
-        const float ic = fmaxf(o[10] * 0.2f + 1.f, 0.05f);
-        const float s0f = fminf(fmaxf(o[9] * 0.1f + 0.25f, 0.01f), 0.99f);
-        m[0] = fminf(fmaxf(o[8] * 5.485f + 10.5f, 1.f), 2500.f);
-        m[1] = fminf(fmaxf(B, 0.f), 5000.f);
-        m[2] = fminf(fmaxf(o[5] * 1.732f, -20.f), 20.f);
-        m[3] = fminf(fmaxf(o[6] * 0.01155f + 0.04f, 0.005f), 0.6f);
-        m[4] = fminf(fmaxf(o[7] * 0.1414f + 0.255f, 0.0001f), 10.f);
-        m[5] = fminf(fmaxf(gm, 0.f), 180.f);
-        m[6] = fminf(fmaxf(az, 0.f), 180.f);
-        m[7] = s0f * ic;
-        m[8] = (1.f - s0f) * ic;
-        m[9] = base[9];
-        m[10] = base[10];
+         p += (int64_t)gridDim.x * blockDim.x)
+        decode_row(y + p * GM_MLP_NOUT, base, models + p * 11);
+}
+
+/* Fused inference for kin <= 256: input normalisation (as k_prep_input),
+ * the four layers and the decode in one kernel, so that no activation
+ * leaves the CU.  A workgroup takes 64 pixels; their activations stay in LDS
+ * (64 x 256 BF16, overwritten layer by layer) and each layer's weights,
+ * transposed to [out][in] so that a B fragment is one 16-byte read, are
+ * staged through LDS 64 k at a time and shared by the four waves.  Wave w
+ * computes output columns [w N/4, (w+1) N/4) of all 64 rows: 4 x N/64
+ * fragments of v_mfma_f32_16x16x32_bf16.  The accumulation order over k,
+ * the bias/GELU epilogue and the BF16 rounding are those of k_gemm_nn, so
+ * the outputs equal the layered path's.                                    */
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+constexpr int FX_ROWS = 64;
+constexpr int FX_AS = GM_MLP_HIDDEN + 8;   /* activation row stride (bf16) */
+constexpr int FX_WS = 64 + 8;              /* weight row stride (bf16)      */
+
+/* One layer of k_mlp_infer.  `pre` holds this layer's first 64-k weight
+ * chunk on entry (16-byte pieces, PER per thread); the last chunk's
+ * iteration loads the next layer's first chunk (wtx, row length KX, FNX x 64
+ * outputs) into it, so no layer starts by waiting for memory. */
+template <int FN, int FNX>
+__device__ __forceinline__ void fused_layer(uint16_t *__restrict__ sX, uint16_t *__restrict__ sW,
+                                            const uint16_t *__restrict__ wt, const float *__restrict__ bias,
+                                            int K, float *__restrict__ sY, u32x4_t (&pre)[8],
+                                            const uint16_t *__restrict__ wtx, int KX)
+{
+    constexpr int N = 64 * FN, PER = N * 8 / 256, PERX = 64 * FNX * 8 / 256;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wn = wave * (N / 4), kq8 = 8 * (lane >> 4), l16 = lane & 15;
+    float4_t acc[4][FN];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < K; k0 += 64) {
+        __syncthreads();                       /* previous reads of sW (and sX writes) done */
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int q = tid + 256 * e, n = q >> 3, c = q & 7;
+            *reinterpret_cast<u32x4_t *>(&sW[n * FX_WS + c * 8]) = pre[e];
+        }
+        __syncthreads();
+        if (k0 + 64 < K) {
+#pragma unroll
+            for (int e = 0; e < PER; ++e) {
+                const int q = tid + 256 * e, n = q >> 3, c = q & 7;
+                pre[e] = *reinterpret_cast<const u32x4_t *>(&wt[(size_t)n * K + k0 + 64 + c * 8]);
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < PERX; ++e) {
+                const int q = tid + 256 * e, n = q >> 3, c = q & 7;
+                pre[e] = *reinterpret_cast<const u32x4_t *>(&wtx[(size_t)n * KX + c * 8]);
+            }
+        }
+#pragma unroll
+        for (int kk = 0; kk < 64; kk += 32) {
+            bf16x8_t a[4], b[FN];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                a[i] = *reinterpret_cast<const bf16x8_t *>(&sX[(i * 16 + l16) * FX_AS + k0 + kk + kq8]);
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+                b[j] = *reinterpret_cast<const bf16x8_t *>(&sW[(wn + j * 16 + l16) * FX_WS + kk + kq8]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads();                           /* every wave has read this layer's input */
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = i * 16 + 4 * (lane >> 4) + r;
+                const int col = wn + j * 16 + l16;
+                const float v = acc[i][j][r] + bias[col];
+                if (sY) {
+                    if (col < 11) sY[row * 12 + col] = v;
+                } else {
+                    sX[row * FX_AS + col] = f2bf(gelu(v));
+                }
+            }
+}
+
+/* obs must be 16-byte aligned (4 nl floats per pixel keep every row so). */
+__global__ __launch_bounds__(256, 2) void k_mlp_infer(const float *__restrict__ obs, int64_t npix, int nl,
+                                                      int kin, const uint16_t *__restrict__ wt,
+                                                      const float *__restrict__ w, gm_mlp_offsets of,
+                                                      const double *__restrict__ base,
+                                                      double *__restrict__ models)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t sX[FX_ROWS * FX_AS];
+    __shared__ __attribute__((aligned(16))) uint16_t sW[GM_MLP_HIDDEN * FX_WS];
+    __shared__ float sIc[FX_ROWS];
+    __shared__ float sY[FX_ROWS * 12];
+    const int tid = threadIdx.x;
+    const int64_t p0 = (int64_t)blockIdx.x * FX_ROWS;
+    const int rows = (npix - p0 < FX_ROWS) ? (int)(npix - p0) : FX_ROWS;
+    const int nf = 4 * nl, n4 = rows * nf / 4;
+    /* first weight chunk and the whole input tile in flight together */
+    u32x4_t pre[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int q = tid + 256 * e, n = q >> 3, c = q & 7;
+        pre[e] = *reinterpret_cast<const u32x4_t *>(&wt[of.woff[0] + (size_t)n * kin + c * 8]);
+    }
+    const float4 *src = reinterpret_cast<const float4 *>(obs + p0 * nf);
+    float4 v[FX_ROWS * GM_MLP_HIDDEN / 4 / 256];
+#pragma unroll
+    for (int j = 0; j < FX_ROWS * GM_MLP_HIDDEN / 4 / 256; ++j) {
+        const int idx = tid + 256 * j;
+        if (idx < n4) v[j] = src[idx];
+    }
+    if (tid < FX_ROWS) {
+        float ic = 1.f;
+        if (tid < rows) {
+            const float *o = obs + (p0 + tid) * nf;
+            ic = fmaxf(fmaxf(o[0], o[nl - 1]), 1e-6f);
+        }
+        sIc[tid] = ic;
+    }
+    /* zero the pad columns and the rows past the end of the cube */
+    for (int e = tid; e < FX_ROWS * (kin - nf); e += 256) {
+        const int r = e / (kin - nf);
+        sX[r * FX_AS + nf + (e - r * (kin - nf))] = 0;
+    }
+    for (int e = tid; e < (FX_ROWS - rows) * nf; e += 256) {
+        const int r = rows + e / nf;
+        sX[r * FX_AS + (e - (r - rows) * nf)] = 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < FX_ROWS * GM_MLP_HIDDEN / 4 / 256; ++j) {
+        const int idx = tid + 256 * j;
+        if (idx < n4) {
+            const int r = (4 * idx) / nf, f = 4 * idx - r * nf;
+            const float ic = sIc[r];
+            const float x[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
+            uint16_t h[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                float y = x[c] / ic;
+                y = (f + c < nl) ? (y - 1.f) * 2.f : y * 10.f;
+                h[c] = f2bf(y);
+            }
+            *reinterpret_cast<uint2 *>(&sX[r * FX_AS + f]) = *reinterpret_cast<const uint2 *>(h);
+        }
+    }
+    fused_layer<4, 4>(sX, sW, wt + of.woff[0], w + of.boff[0], kin, nullptr, pre, wt + of.woff[1],
+                      GM_MLP_HIDDEN);
+    fused_layer<4, 4>(sX, sW, wt + of.woff[1], w + of.boff[1], GM_MLP_HIDDEN, nullptr, pre,
+                      wt + of.woff[2], GM_MLP_HIDDEN);
+    fused_layer<4, 1>(sX, sW, wt + of.woff[2], w + of.boff[2], GM_MLP_HIDDEN, nullptr, pre,
+                      wt + of.woff[3], GM_MLP_HIDDEN);
+    fused_layer<1, 0>(sX, sW, wt + of.woff[3], w + of.boff[3], GM_MLP_HIDDEN, sY, pre, nullptr, 0);
+    __syncthreads();
+    if (tid < rows) decode_row(&sY[tid * 12], base, models + (p0 + tid) * 11);
+}
+
+/* W [in][out] (row-major, BF16) -> WT [out][in] for one layer. */
+__global__ void k_transpose_w(const uint16_t *__restrict__ w, int in, int out, uint16_t *__restrict__ wt)
+{
+    const int64_t n = (int64_t)in * out;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int o = (int)(i / in), k = (int)(i % in);
+        wt[i] = w[(int64_t)k * out + o];
     }
 }
 
@@ -457,4 +642,17 @@ void gm_mlp_adamw(float *w, const float *g, float *m, float *v, uint16_t *wb, in
 void gm_mlp_to_bf16(const float *x, uint16_t *y, int64_t n, hipStream_t s)
 {
     hipLaunchKernelGGL(k_to_bf16, dim3(grid1d(n, 256)), dim3(256), 0, s, x, y, n);
+}
+
+void gm_mlp_infer_fused(const float *obs, int64_t npix, int nl, int kin, const uint16_t *wt,
+                        const float *w, const gm_mlp_offsets &of, const double *base, double *models,
+                        hipStream_t s)
+{
+    const int64_t tiles = (npix + FX_ROWS - 1) / FX_ROWS;
+    hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)tiles), dim3(256), 0, s, obs, npix, nl, kin, wt, w, of,
+                       base, models);
+}
+void gm_mlp_transpose(const uint16_t *w, int in, int out, uint16_t *wt, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_transpose_w, dim3(grid1d((int64_t)in * out, 256)), dim3(256), 0, s, w, in, out, wt);
 }

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <random>
 #include <vector>
@@ -35,6 +36,7 @@ struct gm_mlp {
     int64_t nparam = 0;
     float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;   /* FP32 flat */
     uint16_t *wb = nullptr;                                          /* BF16 copy */
+    uint16_t *wt = nullptr;            /* BF16 weights transposed per layer (inference) */
     int64_t step = 0;
     /* activations for a batch of `cap` rows */
     int64_t cap = 0;
@@ -121,9 +123,10 @@ extern "C" gm_status gm_mlp_create(int32_t nlambda, uint64_t seed, gm_mlp **out)
     n->m = dalloc<float>(off);
     n->v = dalloc<float>(off);
     n->wb = dalloc<uint16_t>(off);
+    n->wt = dalloc<uint16_t>(off);
     n->loss = dalloc<float>(1);
     n->base = dalloc<double>(GM_NMODEL);
-    if (!n->w || !n->g || !n->m || !n->v || !n->wb || !n->loss || !n->base) {
+    if (!n->w || !n->g || !n->m || !n->v || !n->wb || !n->wt || !n->loss || !n->base) {
         gm_mlp_destroy(n);
         return GM_ERR_NOMEM;
     }
@@ -141,7 +144,7 @@ extern "C" gm_status gm_mlp_destroy(gm_mlp *n)
     if (!n) return GM_ERR_INVALID;
     (void)hipDeviceSynchronize();
     free_batch(n);
-    void *ps[] = {n->w, n->g, n->m, n->v, n->wb, n->loss, n->base};
+    void *ps[] = {n->w, n->g, n->m, n->v, n->wb, n->wt, n->loss, n->base};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     delete n;
@@ -258,11 +261,22 @@ extern "C" gm_status gm_mlp_predict(gm_mlp *n, const float *obs, int64_t npix, c
     if (!n || !obs || !base || !models || npix < 0) return GM_ERR_INVALID;
     if (npix == 0) return GM_OK;
     hipStream_t s = (hipStream_t)stream;
+    if (hipMemcpyAsync(n->base, base, sizeof(double) * GM_NMODEL, hipMemcpyHostToDevice, s) != hipSuccess)
+        return GM_ERR_HIP;
+    if (n->kin <= GM_MLP_HIDDEN && ((uintptr_t)obs & 15) == 0) {
+        /* one fused kernel: no batch buffers, activations stay on chip */
+        gm_mlp_offsets of{};
+        for (int l = 0; l < GM_MLP_LAYERS; ++l) {
+            gm_mlp_transpose(n->wb + n->L[l].woff, n->L[l].in, n->L[l].out, n->wt + n->L[l].woff, s);
+            of.woff[l] = n->L[l].woff;
+            of.boff[l] = n->L[l].boff;
+        }
+        gm_mlp_infer_fused(obs, npix, n->nl, n->kin, n->wt, n->w, of, n->base, models, s);
+        return hipGetLastError() == hipSuccess ? GM_OK : GM_ERR_HIP;
+    }
     const int64_t chunk = 1 << 18;
     gm_status st = ensure_batch(n, npix < chunk ? npix : chunk);
     if (st != GM_OK) return st;
-    if (hipMemcpyAsync(n->base, base, sizeof(double) * GM_NMODEL, hipMemcpyHostToDevice, s) != hipSuccess)
-        return GM_ERR_HIP;
     for (int64_t p0 = 0; p0 < npix; p0 += chunk) {
         const int64_t rows = (npix - p0 < chunk) ? npix - p0 : chunk;
         const int64_t padded = ((rows + 127) / 128) * 128;
