@@ -361,7 +361,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--npix", type=int, default=1024 * 1024)
     ap.add_argument("--precision", choices=["mixed", "fp32"], default="mixed")
-    ap.add_argument("--solver", choices=["svd", "ldl", "jacobi"], default="svd")
+    ap.add_argument("--solver", choices=["svd", "ldl", "jacobi", "cordic"], default="svd")
     ap.add_argument("--miter", type=int, default=50)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -403,7 +403,7 @@ def main():
 
     prec = gm.GM_PREC_MIXED if args.precision == "mixed" else gm.GM_PREC_FP32
     solver = {"svd": gm.GM_SOLVE_SVD, "ldl": gm.GM_SOLVE_LDL,
-              "jacobi": gm.GM_SOLVE_JACOBI}[args.solver]
+              "jacobi": gm.GM_SOLVE_JACOBI, "cordic": gm.GM_SOLVE_CORDIC}[args.solver]
     setup = gm.setup_single_line(psf_fwhm=60.0, precision=prec, solver=solver,
                                  miter=args.miter)
     ctx = gm.Context(setup)

@@ -29,7 +29,7 @@ GM_MAX_COMP = 10
 GM_NMODEL = 11
 
 GM_PREC_FP32, GM_PREC_MIXED = 0, 1
-GM_SOLVE_SVD, GM_SOLVE_LDL, GM_SOLVE_JACOBI = 0, 1, 2
+GM_SOLVE_SVD, GM_SOLVE_LDL, GM_SOLVE_JACOBI, GM_SOLVE_CORDIC = 0, 1, 2, 3
 GM_INIT_CE, GM_INIT_MODEL, GM_INIT_PER_PIXEL = 0, 1, 2
 
 # Init_Model order (defines.h:94-106)

@@ -111,7 +111,8 @@ static gm_status build_dsetup(const gm_setup &s, gm_dsetup &d)
             if (!std::isfinite(s.slight[k])) return GM_ERR_INVALID;
     if (s.ce_line < 0 || s.ce_line >= s.nlines) return GM_ERR_INVALID;
     if (s.precision != GM_PREC_FP32 && s.precision != GM_PREC_MIXED) return GM_ERR_INVALID;
-    if (s.solver != GM_SOLVE_SVD && s.solver != GM_SOLVE_LDL && s.solver != GM_SOLVE_JACOBI)
+    if (s.solver != GM_SOLVE_SVD && s.solver != GM_SOLVE_LDL && s.solver != GM_SOLVE_JACOBI &&
+        s.solver != GM_SOLVE_CORDIC)
         return GM_ERR_INVALID;
     d.nlambda = n;
     d.nlines = s.nlines;
@@ -424,7 +425,8 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
         const int nxt = 1 - cur;
         kbegin(2);
         gm_launch_solve(c->dS, st, st.list[cur], &cnt[cur], npix, mixed, solver, &cnt[nxt],
-                        &cnt[2], hist + 3 * r, st.list[3], &cnt[3], s, lc.np);
+                        &cnt[2], hist + 3 * r, st.list[3], &cnt[3], s, lc.np,
+                        gm_cordic10(c->setup, lc.np));
         kend();
         kbegin(0);
         gm_launch_synth(lc, c->dS, st.trial, st.pre_trial, -GM_MODEL_REC, obs, st.list[cur], &cnt[cur], 0,
@@ -539,7 +541,8 @@ extern "C" gm_status gm_step_solve(gm_ctx *c, const float *alpha, const float *b
     if (npix == 0) return GM_OK;
     if (!alpha || !beta || !flambda || !delta) return GM_ERR_INVALID;
     gm_launch_step_solve(alpha, beta, flambda, npix, delta, c->setup.precision == GM_PREC_MIXED,
-                         c->setup.solver, (hipStream_t)stream, c->lc.np);
+                         c->setup.solver, (hipStream_t)stream, c->lc.np,
+                         gm_cordic10(c->setup, c->lc.np));
     return hip_ok(hipGetLastError());
 }
 

@@ -135,10 +135,18 @@ void gm_launch_synth(const gm_lanecfg &lc, const gm_dsetup *dS, const double *mo
 void gm_launch_solve(const gm_dsetup *dS, gm_state st, const int32_t *list,
                      const int32_t *count, int64_t grid_items, int mixed, int solver,
                      int32_t *reset0, int32_t *reset1, int32_t *hist,
-                     int32_t *fb_list, int32_t *fb_count, hipStream_t stream, int np = 9);
+                     int32_t *fb_list, int32_t *fb_count, hipStream_t stream, int np = 9,
+                     int cordic10 = 0);
 void gm_launch_step_solve(const float *alpha, const float *beta, const double *flambda,
                           int64_t n, double *delta, int mixed, int solver,
-                          hipStream_t stream, int np = 9);
+                          hipStream_t stream, int np = 9, int cordic10 = 0);
+/* eleven-slot contexts with mac fixed take the reference's 10-term CORDIC
+ * solve under GM_SOLVE_CORDIC (with mac free the reference solves the
+ * 11-term system with an FP64 SVD) */
+inline int gm_cordic10(const gm_setup &s, int np)
+{
+    return s.solver == GM_SOLVE_CORDIC && np == 11 && !s.fix[9];
+}
 void gm_launch_decide(const gm_dsetup *dS, gm_state st, const int32_t *list,
                       const int32_t *count, int64_t grid_items, int32_t *next_list,
                       int32_t *next_count, int32_t *acc_list, int32_t *acc_count,

@@ -1609,6 +1609,169 @@ __device__ __forceinline__ void solve_svd_n(T (&a)[N * (N + 1) / 2], const T (&b
     }
 }
 
+/* ---- the reference's 10-term step solve ---------------------------------
+ * mil_svdf (milosUtils.cu:328-365) over svdcordicf (svdcordic.cu:423-537):
+ * NUM_ITER_SVD_CORDIC = 18 steps (definesCuda.cuh:119) of a Brent-Luk
+ * parallel Jacobi eigen-decomposition of the damped 10 x 10 system in FP32,
+ * then x = V diag(1/w, |w| > 1e-12) V^T b.  A step takes the five diagonal
+ * 2 x 2 blocks of index pairs (2k, 2k+1) and their angles
+ * theta_k = atan(2 a12 / (a22 - a11)) / 2 (pi/4 when a11 = a22, clamped to
+ * +-pi/4), applies G_i^T B G_j (rows first, then columns) to every block
+ * (i, j) with G = [[c, s], [-s, c]], zeroes the off-diagonal entry of the
+ * diagonal blocks, rotates V's column pairs by G, and moves index i to
+ * CP[i] (the systolic exchange, gmilos.cu:108-128).  No value moves here:
+ * SIG.s[k][i] is the register slot that holds index i after k steps, known
+ * at compile time; CP is a 9-cycle with 0 fixed, so every nine steps each
+ * index is back in its own slot, and two passes of nine steps are the
+ * reference's 18.                                                          */
+namespace cordic {
+constexpr int CP[10] = {0, 2, 4, 1, 6, 3, 8, 5, 9, 7};
+struct Frames {
+    int s[9][10];
+};
+constexpr Frames make_frames()
+{
+    Frames f{};
+    int inv[10] = {};
+    for (int i = 0; i < 10; ++i) inv[CP[i]] = i;
+    for (int i = 0; i < 10; ++i) f.s[0][i] = i;
+    for (int k = 1; k < 9; ++k)
+        for (int i = 0; i < 10; ++i) f.s[k][i] = f.s[k - 1][inv[i]];
+    return f;
+}
+constexpr Frames SIG = make_frames();
+constexpr float QPI = 0.78539816339744830962f;    /* pi / 4 */
+
+/* (r0 r1) x (c0 c1) block of the symmetric matrix: rows rotated by
+ * (cr, sr), then columns by (cc, sc), in the reference's operation order. */
+__device__ __forceinline__ void rot_block(float &b00, float &b01, float &b10, float &b11, float cr,
+                                          float sr, float cc, float sc)
+{
+    const float o11 = b00 * cr - b10 * sr, o21 = b00 * sr + b10 * cr;
+    const float o12 = b01 * cr - b11 * sr, o22 = b01 * sr + b11 * cr;
+    b00 = o11 * cc - o12 * sc;
+    b01 = o11 * sc + o12 * cc;
+    b10 = o21 * cc - o22 * sc;
+    b11 = o21 * sc + o22 * cc;
+}
+
+template <int K>
+__device__ __forceinline__ void step(float (&m)[55], float (&v)[100])
+{
+    float c[5], s[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const int i = SIG.s[K][2 * k], j = SIG.s[K][2 * k + 1];
+        const float a11 = m[utn<10>(i, i)], a12 = m[utn<10>(i, j)], a22 = m[utn<10>(j, j)];
+        float th;
+        if (a11 == a22) {
+            th = QPI;
+        } else {
+            th = __fdividef(atanf(__fdividef(2.f * a12, a22 - a11)), 2.f);
+            th = th > QPI ? QPI : (th < -QPI ? -QPI : th);
+        }
+        __sincosf(th, &s[k], &c[k]);
+    }
+#pragma unroll
+    for (int bi = 0; bi < 5; ++bi)
+#pragma unroll
+        for (int bj = bi; bj < 5; ++bj) {
+            const int r0 = SIG.s[K][2 * bi], r1 = SIG.s[K][2 * bi + 1];
+            const int c0 = SIG.s[K][2 * bj], c1 = SIG.s[K][2 * bj + 1];
+            float b00 = m[utn<10>(r0, c0)], b01 = m[utn<10>(r0, c1)];
+            float b10 = m[utn<10>(r1, c0)], b11 = m[utn<10>(r1, c1)];
+            rot_block(b00, b01, b10, b11, c[bi], s[bi], c[bj], s[bj]);
+            m[utn<10>(r0, c0)] = b00;
+            m[utn<10>(r1, c1)] = b11;
+            if (bi == bj) {
+                m[utn<10>(r0, c1)] = 0.f;
+            } else {
+                m[utn<10>(r0, c1)] = b01;
+                m[utn<10>(r1, c0)] = b10;
+            }
+        }
+#pragma unroll
+    for (int f = 0; f < 10; ++f)
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            const int q0 = SIG.s[K][2 * k], q1 = SIG.s[K][2 * k + 1];
+            const float v0 = v[f * 10 + q0], v1 = v[f * 10 + q1];
+            v[f * 10 + q0] = v0 * c[k] - v1 * s[k];
+            v[f * 10 + q1] = v0 * s[k] + v1 * c[k];
+        }
+}
+
+template <int K>
+__device__ __forceinline__ void nine_steps(float (&m)[55], float (&v)[100])
+{
+    step<K>(m, v);
+    if constexpr (K < 8) nine_steps<K + 1>(m, v);
+}
+
+/* m: packed damped 10 x 10 upper triangle (destroyed); b, x: 10 */
+__device__ __forceinline__ void solve10(float (&m)[55], const float (&b)[10], float (&x)[10])
+{
+    float v[100];
+#pragma unroll
+    for (int e = 0; e < 100; ++e) v[e] = (e % 11 == 0) ? 1.f : 0.f;
+    for (int pass = 0; pass < 2; ++pass) nine_steps<0>(m, v);
+    float aux[10];
+#pragma unroll
+    for (int j = 0; j < 10; ++j) {
+        float sum = 0.f;
+#pragma unroll
+        for (int k = 0; k < 10; ++k) sum += b[k] * v[k * 10 + j];
+        const float w = m[utn<10>(j, j)];
+        aux[j] = sum * ((fabsf(w) > 1e-12f) ? 1.f / w : 0.f);
+    }
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        float sum = 0.f;
+#pragma unroll
+        for (int k = 0; k < 10; ++k) sum += v[i * 10 + k] * aux[k];
+        x[i] = sum;
+    }
+}
+
+/* The 9-slot system (packed 45) as the reference's 10 x 10 with slot 9 (the
+ * filling factor's response, zero while alfa is fixed) a zero row. */
+template <typename T>
+__device__ __forceinline__ void solve9(const T (&a)[45], const T (&b)[9], T (&x)[9])
+{
+    float m[55], bb[10], xx[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i)
+#pragma unroll
+        for (int j = i; j < 10; ++j) m[utn<10>(i, j)] = (j < 9) ? (float)a[ut(i, j)] : 0.f;
+#pragma unroll
+    for (int p = 0; p < 9; ++p) bb[p] = (float)b[p];
+    bb[9] = 0.f;
+    solve10(m, bb, xx);
+#pragma unroll
+    for (int p = 0; p < 9; ++p) x[p] = (T)xx[p];
+}
+
+/* The 11-slot system with mac fixed (slot 9 a zero row) as the reference's
+ * 10-term one: slots 0-8, then the filling factor in slot 9. */
+__device__ __forceinline__ void solve11_as10(const double (&a)[66], const double (&b)[11],
+                                             double (&x)[11])
+{
+    constexpr int MAP[10] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 10};
+    float m[55], bb[10], xx[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i)
+#pragma unroll
+        for (int j = i; j < 10; ++j) m[utn<10>(i, j)] = (float)a[utn<11>(MAP[i], MAP[j])];
+#pragma unroll
+    for (int p = 0; p < 10; ++p) bb[p] = (float)b[MAP[p]];
+    solve10(m, bb, xx);
+#pragma unroll
+    for (int p = 0; p < 9; ++p) x[p] = xx[p];
+    x[9] = 0.0;
+    x[10] = xx[9];
+}
+} // namespace cordic
+
 /* Step solve of the eleven-slot system for the active list, FP64 (the
  * reference's 11-term solve is FP64 svdcmp): LDL^T where it provably equals
  * the pseudo-inverse, the Jacobi SVD otherwise, then AplicaDelta + check. */
@@ -1617,7 +1780,7 @@ __global__ __launch_bounds__(64) void k_solve11(const gm_dsetup *__restrict__ S,
                                                 const int32_t *__restrict__ count,
                                                 int32_t *__restrict__ reset0,
                                                 int32_t *__restrict__ reset1,
-                                                int32_t *__restrict__ hist)
+                                                int32_t *__restrict__ hist, int cordic10)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (hist) {
@@ -1636,10 +1799,14 @@ __global__ __launch_bounds__(64) void k_solve11(const gm_dsetup *__restrict__ S,
         double a[66], b[11], x[11];
         load_damped_n<double, 11>(st.alpha + pix * st.ab_rec, st.beta + pix * st.ab_rec, 1,
                                   st.flambda[pix], a, b);
-        double a2[66];
+        if (cordic10) {
+            cordic::solve11_as10(a, b, x);
+        } else {
+            double a2[66];
 #pragma unroll
-        for (int e = 0; e < 66; ++e) a2[e] = a[e];
-        if (!solve_ldl_n<double, 11>(a, b, x)) solve_svd_n<double, 11, 14>(a2, b, x);
+            for (int e = 0; e < 66; ++e) a2[e] = a[e];
+            if (!solve_ldl_n<double, 11>(a, b, x)) solve_svd_n<double, 11, 14>(a2, b, x);
+        }
         double best[GM_NMODEL], m[GM_NMODEL];
 #pragma unroll
         for (int f = 0; f < GM_NMODEL; ++f) {
@@ -1657,16 +1824,21 @@ __global__ __launch_bounds__(64) void k_solve11(const gm_dsetup *__restrict__ S,
 __global__ __launch_bounds__(64) void k_step_solve11(const float *__restrict__ alpha,
                                                      const float *__restrict__ beta,
                                                      const double *__restrict__ flambda,
-                                                     int64_t n, double *__restrict__ delta)
+                                                     int64_t n, double *__restrict__ delta,
+                                                     int cordic10)
 {
     for (int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * 64) {
         double a[66], b[11], x[11];
         load_damped_n<double, 11>(alpha + i * 66, beta + i * 11, 1, flambda[i], a, b);
-        double a2[66];
+        if (cordic10) {
+            cordic::solve11_as10(a, b, x);
+        } else {
+            double a2[66];
 #pragma unroll
-        for (int e = 0; e < 66; ++e) a2[e] = a[e];
-        if (!solve_ldl_n<double, 11>(a, b, x)) solve_svd_n<double, 11, 14>(a2, b, x);
+            for (int e = 0; e < 66; ++e) a2[e] = a[e];
+            if (!solve_ldl_n<double, 11>(a, b, x)) solve_svd_n<double, 11, 14>(a2, b, x);
+        }
 #pragma unroll
         for (int p = 0; p < 11; ++p) delta[i * 11 + p] = x[p];
     }
@@ -1675,7 +1847,8 @@ __global__ __launch_bounds__(64) void k_step_solve11(const float *__restrict__ a
 /* Step solve for the active list.  SOLVER = GM_SOLVE_SVD: LDL^T where it
  * provably equals the SVD pseudo-inverse, otherwise the pixel goes to the
  * fallback list solved by k_solve_svd; GM_SOLVE_LDL: LDL^T only;
- * GM_SOLVE_JACOBI: always the Jacobi SVD. */
+ * GM_SOLVE_JACOBI: always the Jacobi SVD; GM_SOLVE_CORDIC: the reference's
+ * 18-step FP32 CORDIC solve (cordic::solve10). */
 template <bool MIXED, int SOLVER>
 __global__ __launch_bounds__(64) void k_solve(const gm_dsetup *__restrict__ S,
                                               gm_state st, const int32_t *__restrict__ list,
@@ -1706,6 +1879,11 @@ __global__ __launch_bounds__(64) void k_solve(const gm_dsetup *__restrict__ S,
         T a[45], b[9], x[9];
         load_damped<T>(st.alpha + pix * st.ab_rec, st.beta + pix * st.ab_rec, 1,
                        (T)st.flambda[pix], a, b);
+        if (SOLVER == GM_SOLVE_CORDIC) {
+            cordic::solve9<T>(a, b, x);
+            apply_to_trial<T>(S, st, pix, x);
+            continue;
+        }
         if (SOLVER == GM_SOLVE_JACOBI) {
             fb_list[atomicAdd(fb_count, 1)] = (int32_t)pix;
             continue;
@@ -1754,7 +1932,9 @@ __global__ __launch_bounds__(64) void k_step_solve(const float *__restrict__ alp
         /* caller layout: alpha [n][45], beta [n][9] */
         T a[45], b[9], x[9];
         load_damped<T>(alpha + i * 45, beta + i * 9, 1, (T)flambda[i], a, b);
-        if (SOLVER == GM_SOLVE_JACOBI) {
+        if (SOLVER == GM_SOLVE_CORDIC) {
+            cordic::solve9<T>(a, b, x);
+        } else if (SOLVER == GM_SOLVE_JACOBI) {
             solve_svd<T, (sizeof(T) == 8 ? 12 : 10)>(a, b, x);
         } else {
             T a2[45];
@@ -2238,13 +2418,14 @@ void gm_launch_synth(const gm_lanecfg &lc, const gm_dsetup *dS, const double *mo
 void gm_launch_solve(const gm_dsetup *dS, gm_state st, const int32_t *list,
                      const int32_t *count, int64_t grid_items, int mixed, int solver,
                      int32_t *reset0, int32_t *reset1, int32_t *hist, int32_t *fb_list,
-                     int32_t *fb_count, hipStream_t s, int np)
+                     int32_t *fb_count, hipStream_t s, int np, int cordic10)
 {
     const int g = grid_for(grid_items, 64, kMaxBlocks);
     if (np == 11) {
-        /* eleven slots: FP64 LDL^T / Jacobi in one kernel, no fallback list */
+        /* eleven slots: FP64 LDL^T / Jacobi in one kernel, no fallback list;
+         * with mac fixed the reference runs its 10-term CORDIC solve */
         hipLaunchKernelGGL(k_solve11, dim3(g), dim3(64), 0, s, dS, st, list, count, reset0, reset1,
-                           hist);
+                           hist, cordic10);
         return;
     }
 #define GM_SOLVE_CASE(MX, SV)                                                                    \
@@ -2253,14 +2434,16 @@ void gm_launch_solve(const gm_dsetup *dS, gm_state st, const int32_t *list,
     if (mixed) {
         if (solver == GM_SOLVE_SVD) GM_SOLVE_CASE(true, GM_SOLVE_SVD);
         else if (solver == GM_SOLVE_LDL) GM_SOLVE_CASE(true, GM_SOLVE_LDL);
+        else if (solver == GM_SOLVE_CORDIC) GM_SOLVE_CASE(true, GM_SOLVE_CORDIC);
         else GM_SOLVE_CASE(true, GM_SOLVE_JACOBI);
     } else {
         if (solver == GM_SOLVE_SVD) GM_SOLVE_CASE(false, GM_SOLVE_SVD);
         else if (solver == GM_SOLVE_LDL) GM_SOLVE_CASE(false, GM_SOLVE_LDL);
+        else if (solver == GM_SOLVE_CORDIC) GM_SOLVE_CASE(false, GM_SOLVE_CORDIC);
         else GM_SOLVE_CASE(false, GM_SOLVE_JACOBI);
     }
 #undef GM_SOLVE_CASE
-    if (solver != GM_SOLVE_LDL) {
+    if (solver == GM_SOLVE_SVD || solver == GM_SOLVE_JACOBI) {
         const int gf = grid_for(grid_items, 32, 1024);
         if (mixed)
             hipLaunchKernelGGL((k_solve_svd<true>), dim3(gf), dim3(64), 0, s, dS, st, fb_list, fb_count,
@@ -2273,11 +2456,12 @@ void gm_launch_solve(const gm_dsetup *dS, gm_state st, const int32_t *list,
 
 void gm_launch_step_solve(const float *alpha, const float *beta, const double *flambda,
                           int64_t n, double *delta, int mixed, int solver, hipStream_t s,
-                          int np)
+                          int np, int cordic10)
 {
     const int g = grid_for(n, 64, kMaxBlocks);
     if (np == 11) {
-        hipLaunchKernelGGL(k_step_solve11, dim3(g), dim3(64), 0, s, alpha, beta, flambda, n, delta);
+        hipLaunchKernelGGL(k_step_solve11, dim3(g), dim3(64), 0, s, alpha, beta, flambda, n, delta,
+                           cordic10);
         return;
     }
 #define GM_STEP_CASE(MX, SV)                                                                     \
@@ -2285,10 +2469,12 @@ void gm_launch_step_solve(const float *alpha, const float *beta, const double *f
     if (mixed) {
         if (solver == GM_SOLVE_SVD) GM_STEP_CASE(true, GM_SOLVE_SVD);
         else if (solver == GM_SOLVE_LDL) GM_STEP_CASE(true, GM_SOLVE_LDL);
+        else if (solver == GM_SOLVE_CORDIC) GM_STEP_CASE(true, GM_SOLVE_CORDIC);
         else GM_STEP_CASE(true, GM_SOLVE_JACOBI);
     } else {
         if (solver == GM_SOLVE_SVD) GM_STEP_CASE(false, GM_SOLVE_SVD);
         else if (solver == GM_SOLVE_LDL) GM_STEP_CASE(false, GM_SOLVE_LDL);
+        else if (solver == GM_SOLVE_CORDIC) GM_STEP_CASE(false, GM_SOLVE_CORDIC);
         else GM_STEP_CASE(false, GM_SOLVE_JACOBI);
     }
 #undef GM_STEP_CASE

@@ -66,7 +66,7 @@ struct Engine {
     gm_ctx *ctx = nullptr;
     const gm_dsetup *dS = nullptr;
     gm_lanecfg lc{};
-    int nl = 0, mixed = 0, solver = 0, init_mode = 0;
+    int nl = 0, mixed = 0, solver = 0, init_mode = 0, cordic10 = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[kDepth] = {nullptr, nullptr};
     int32_t *completed_host[kDepth] = {nullptr, nullptr};   /* pinned */
@@ -173,6 +173,7 @@ static gm_status build_engine(gm_server *srv, const gm_setup *setup, Engine **ou
     e->nl = ps.nlambda;
     e->mixed = ps.precision == GM_PREC_MIXED;
     e->solver = ps.solver;
+    e->cordic10 = gm_cordic10(ps, e->lc.np);
     e->init_mode = ps.init_mode;
     e->max_tiles = srv->cfg.max_tiles;
     const int64_t nl4 = 4 * (int64_t)e->nl;
@@ -299,7 +300,7 @@ static void launch_round(Engine &e)
     gm_launch_set_count(&cnt[5], 0, s);
     gm_launch_set_count(e.completed_dev, 0, s);
     gm_launch_solve(e.dS, st, st.list[cur], &cnt[cur], grid, e.mixed, e.solver, &cnt[nxt], &cnt[2],
-                    nullptr, st.list[3], &cnt[3], s, e.lc.np);
+                    nullptr, st.list[3], &cnt[3], s, e.lc.np, e.cordic10);
     gm_launch_synth(e.lc, e.dS, st.trial, st.pre_trial, -GM_MODEL_REC, e.obs, st.list[cur], &cnt[cur], 0, grid,
                     st.chi_trial, e.synth, e.mixed, s);
     gm_launch_decide(e.dS, st, st.list[cur], &cnt[cur], grid, st.list[nxt], &cnt[nxt], st.list[2],

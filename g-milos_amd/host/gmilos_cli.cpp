@@ -49,11 +49,12 @@ std::string base_of(const gmio_control &c)
 
 int main(int argc, char **argv)
 {
-    if (argc < 2) return fail("usage: gmilos <file.mtrol> [--mixed] [--ldl] [--exact-rf]");
+    if (argc < 2) return fail("usage: gmilos <file.mtrol> [--mixed] [--ldl] [--cordic] [--exact-rf]");
     int precision = GM_PREC_FP32, solver = GM_SOLVE_SVD, exact_rf = 0;
     for (int i = 2; i < argc; ++i) {
         if (!std::strcmp(argv[i], "--mixed")) precision = GM_PREC_MIXED;
         else if (!std::strcmp(argv[i], "--ldl")) solver = GM_SOLVE_LDL;
+        else if (!std::strcmp(argv[i], "--cordic")) solver = GM_SOLVE_CORDIC;   /* the reference's solve */
         else if (!std::strcmp(argv[i], "--exact-rf")) exact_rf = 1;
         else return fail("unknown option ", argv[i]);
     }

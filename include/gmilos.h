@@ -55,7 +55,13 @@ typedef enum {
                                 rule): LDL^T where trace(A^-1) < 1e12 proves
                                 every eigenvalue > 1e-12, Jacobi SVD otherwise */
     GM_SOLVE_LDL = 1,        /* LDL^T only (zero rows dropped)               */
-    GM_SOLVE_JACOBI = 2      /* always the Jacobi SVD                        */
+    GM_SOLVE_JACOBI = 2,     /* always the Jacobi SVD                        */
+    GM_SOLVE_CORDIC = 3      /* the reference's own 10-term solve, result for
+                                result: 18 steps of its FP32 CORDIC Jacobi
+                                (mil_svdf + svdcordicf, milosUtils.cu:328-365,
+                                svdcordic.cu:423-537), unconverged as there;
+                                the 11-term system (mac free) keeps the FP64
+                                SVD, as lm_mils_11 does                     */
 } gm_solver;
 
 typedef enum {

@@ -656,6 +656,98 @@ void gmo_svd_solve(const double *Ain, const double *b, int n, double *x,
     }
 }
 
+/* The reference's 10-term solve (mil_svdf, milosUtils.cu:328-365, over
+ * svdcordicf, svdcordic.cu:423-537; NUM_ITER_SVD_CORDIC = 18,
+ * definesCuda.cuh:119), restated in FP64.  A Brent-Luk parallel Jacobi
+ * eigen-decomposition: each step takes the five diagonal 2 x 2 blocks of
+ * index pairs (2k, 2k+1), their angles theta_k = atan(2 a12/(a22 - a11))/2
+ * (pi/4 when a11 == a22, clamped to +-pi/4; svdcordic.cu:83-114), rotates
+ * every block (i, j) as G_i^T B G_j -- rows first, then columns, with
+ * G = [[c, s], [-s, c]] (ROTACIONf, svdcordic.cu:33-50; PDf, PNDf) -- zeroes
+ * the diagonal blocks' off-diagonal entry, rotates V's column pairs by G
+ * (PVf), and then moves index i to CP[i] in both the matrix and V (the
+ * a_out positions of svdcordic.cu:441-500 and cordicPosFila/Col,
+ * gmilos.cu:108-128).  CP is a 9-cycle, so after 9k steps V A_k V^T = A.
+ * a_out / v_out (10 x 10, may be NULL) receive the final matrix and V. */
+static const int CORDIC_CP[10] = {0, 2, 4, 1, 6, 3, 8, 5, 9, 7};
+
+void gmo_cordic_eig10(const double *Ain, int steps, double *a_out, double *v_out)
+{
+    double A[10][10], V[10][10], An[10][10], Vn[10][10];
+    for (int i = 0; i < 10; i++)
+        for (int j = 0; j < 10; j++) {
+            A[i][j] = (i <= j) ? Ain[i * 10 + j] : Ain[j * 10 + i];
+            V[i][j] = (i == j) ? 1.0 : 0.0;
+        }
+    const double qpi = GMO_PI / 4;
+    for (int st = 0; st < steps; st++) {
+        double c[5], sn[5];
+        for (int k = 0; k < 5; k++) {
+            const double a11 = A[2 * k][2 * k], a12 = A[2 * k][2 * k + 1];
+            const double a22 = A[2 * k + 1][2 * k + 1];
+            double th = qpi;
+            if (a11 != a22) {
+                th = atan(2 * a12 / (a22 - a11)) / 2;
+                if (th > qpi) th = qpi;
+                if (th < -qpi) th = -qpi;
+            }
+            c[k] = cos(th);
+            sn[k] = sin(th);
+        }
+        for (int bi = 0; bi < 5; bi++)
+            for (int bj = bi; bj < 5; bj++) {
+                const int r = 2 * bi, q = 2 * bj;
+                /* rows of the block by G_bi^T */
+                const double o11 = A[r][q] * c[bi] - A[r + 1][q] * sn[bi];
+                const double o21 = A[r][q] * sn[bi] + A[r + 1][q] * c[bi];
+                const double o12 = A[r][q + 1] * c[bi] - A[r + 1][q + 1] * sn[bi];
+                const double o22 = A[r][q + 1] * sn[bi] + A[r + 1][q + 1] * c[bi];
+                /* then its columns by G_bj */
+                double m[2][2];
+                m[0][0] = o11 * c[bj] - o12 * sn[bj];
+                m[0][1] = o11 * sn[bj] + o12 * c[bj];
+                m[1][0] = o21 * c[bj] - o22 * sn[bj];
+                m[1][1] = o21 * sn[bj] + o22 * c[bj];
+                if (bi == bj) m[0][1] = m[1][0] = 0.0;
+                for (int u = 0; u < 2; u++)
+                    for (int w = 0; w < 2; w++) {
+                        const int P = CORDIC_CP[r + u], Q = CORDIC_CP[q + w];
+                        An[P][Q] = An[Q][P] = m[u][w];
+                    }
+            }
+        for (int f = 0; f < 10; f++)
+            for (int k = 0; k < 5; k++) {
+                const double v0 = V[f][2 * k], v1 = V[f][2 * k + 1];
+                Vn[CORDIC_CP[f]][CORDIC_CP[2 * k]] = v0 * c[k] - v1 * sn[k];
+                Vn[CORDIC_CP[f]][CORDIC_CP[2 * k + 1]] = v0 * sn[k] + v1 * c[k];
+            }
+        memcpy(A, An, sizeof A);
+        memcpy(V, Vn, sizeof V);
+    }
+    if (a_out) memcpy(a_out, A, sizeof A);
+    if (v_out) memcpy(v_out, V, sizeof V);
+}
+
+/* delta = V diag(1/w, |w| > 1e-12) V^T b with w = diag of the 18-step
+ * matrix, the reference's order of summation (milosUtils.cu:337-364). */
+void gmo_cordic_solve10(const double *A, const double *b, double *x, double *w_out)
+{
+    double a[100], v[100], aux[10];
+    gmo_cordic_eig10(A, 18, a, v);
+    for (int j = 0; j < 10; j++) {
+        double sum = 0;
+        for (int k = 0; k < 10; k++) sum += b[k] * v[k * 10 + j];
+        const double w = a[j * 10 + j];
+        if (w_out) w_out[j] = w;
+        aux[j] = sum * ((fabs(w) > 1e-12) ? (1.0 / w) : 0.0);
+    }
+    for (int i = 0; i < 10; i++) {
+        double sum = 0;
+        for (int k = 0; k < 10; k++) sum += v[i * 10 + k] * aux[k];
+        x[i] = sum;
+    }
+}
+
 /* Apply the step to the free parameters with the reference's clamps on the
  * step: |dB| <= 300 G, |dgamma|,|dphi| <= 30 deg (milosUtils.cu:93-159,
  * NTERMS=10 rule: slot 9 updates the filling factor when it is free).      */
@@ -864,7 +956,10 @@ int gmo_lm_pixel(const gmo_setup *s, const float *obs, const double *init,
     do {
         memcpy(covar, alpha, sizeof(double) * NT * NT);
         for (int j = 0; j < NT; j++) covar[j * (NT + 1)] = alpha[j * (NT + 1)] * (1.0 + flambda);
-        gmo_svd_solve(covar, beta, NT, delta, NULL);
+        if (s->solver == GMO_SOLVE_CORDIC && NT == 10)
+            gmo_cordic_solve10(covar, beta, delta, NULL);
+        else
+            gmo_svd_solve(covar, beta, NT, delta, NULL);
         gmo_apply_delta(s, best, delta, model);
         gmo_check(model);
         gmo_synth(s, model, synth);

@@ -60,7 +60,14 @@ typedef struct {
     double  slight[4 * GMO_MAXL];    /* stray light, I,Q,U,V blocks of n      */
     int     nterms;                  /* reference NTERMS: 10 (lm_mils, also 0)
                                         or 11 (lm_mils_11: mac and alfa free) */
+    int     solver;                  /* GMO_SOLVE_SVD: the pseudo-inverse from a
+                                        converged SVD; GMO_SOLVE_CORDIC: the
+                                        reference's 18-step CORDIC solve for
+                                        the 10-term system                  */
 } gmo_setup;
+
+#define GMO_SOLVE_SVD 0
+#define GMO_SOLVE_CORDIC 3
 
 /* ---- atomic / PSF set-up ---------------------------------------------- */
 int  gmo_cuanten(double sl, double ll, double jl, double su, double lu,
@@ -99,6 +106,11 @@ int  gmo_nterms(const gmo_setup *s);
 /* pseudo-inverse solve of a symmetric n x n system via an SVD (n <= 11) */
 void gmo_svd_solve(const double *A, const double *b, int n, double *x,
                    double *w_out);
+/* the reference's 10-term CORDIC solve: `steps` Brent-Luk steps of the
+ * symmetric 10 x 10 A (upper triangle read), final matrix and V (10 x 10,
+ * either may be NULL); the solve runs 18 steps */
+void gmo_cordic_eig10(const double *A, int steps, double *a_out, double *v_out);
+void gmo_cordic_solve10(const double *A, const double *b, double *x, double *w_out);
 void gmo_apply_delta(const gmo_setup *s, const double *model,
                      const double *delta, double *out);
 void gmo_check(double *model);

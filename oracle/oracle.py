@@ -43,7 +43,9 @@ class gmo_setup(ctypes.Structure):
                 ("init_model", ctypes.c_double * NMODEL), ("use_ce", ctypes.c_int),
                 ("ce_line", ctypes.c_int), ("rf_reference_rules", ctypes.c_int),
                 ("use_slight", ctypes.c_int), ("slight", ctypes.c_double * (4 * MAXL)),
-                ("nterms", ctypes.c_int)]
+                ("nterms", ctypes.c_int), ("solver", ctypes.c_int)]
+
+SOLVE_SVD, SOLVE_CORDIC = 0, 3
 
 
 _lib = None
@@ -78,6 +80,8 @@ def lib():
         L.gmo_chisqr.argtypes = [P(gmo_setup), vp, vp, i]
         L.gmo_chisqr.restype = d
         L.gmo_svd_solve.argtypes = [vp, vp, i, vp, vp]
+        L.gmo_cordic_eig10.argtypes = [vp, i, vp, vp]
+        L.gmo_cordic_solve10.argtypes = [vp, vp, vp, vp]
         L.gmo_apply_delta.argtypes = [P(gmo_setup), vp, vp, vp]
         L.gmo_check.argtypes = [vp]
         L.gmo_classical_estimates.argtypes = [P(gmo_setup), vp, vp, i]
@@ -169,7 +173,8 @@ def conv(x, h, ic=None):
 
 def make_setup(lines, offsets_mA, psf=None, init=None, free=None, miter=50, noise=1e-6,
                weights=(1.0, 1.0, 1.0, 1.0), ilambda=0.1, toplim=1e-12, mu=1.0, use_ce=1,
-               ce_line=0, rf_reference_rules=1, logclambda=0, init_defaults=None):
+               ce_line=0, rf_reference_rules=1, logclambda=0, init_defaults=None,
+               solver=SOLVE_SVD):
     """Oracle set-up from the same ingredients as the product's make_setup:
     grid offsets in mA from lines[0]'s centre (gmilos.cu:171-196), PSF as a
     Gaussian FWHM on the grid (gmilos.cu:244-245) or a (dl, value) table
@@ -233,6 +238,7 @@ def make_setup(lines, offsets_mA, psf=None, init=None, free=None, miter=50, nois
     s.ce_line = int(ce_line)
     s.rf_reference_rules = int(rf_reference_rules)
     s.nterms = 11 if (s.fix[9] and s.fix[10]) else 10
+    s.solver = int(solver)
     return s
 
 
@@ -285,6 +291,26 @@ def chisqr(s, obs, synth_, nfree=None):
     synth_ = np.ascontiguousarray(synth_, dtype=np.float64)
     nfree = nfree if nfree is not None else 4 * s.nlambda - nterms(s)
     return lib().gmo_chisqr(ctypes.byref(s), _p(obs), _p(synth_), nfree)
+
+
+def cordic_eig10(A, steps):
+    """The reference's Brent-Luk CORDIC steps on a symmetric 10 x 10 A:
+    (final matrix, V)."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    a = np.zeros((10, 10))
+    v = np.zeros((10, 10))
+    lib().gmo_cordic_eig10(_p(A), int(steps), _p(a), _p(v))
+    return a, v
+
+
+def cordic_solve10(A, b):
+    """The reference's 10-term step solve (18 CORDIC steps): (x, w)."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(10)
+    w = np.zeros(10)
+    lib().gmo_cordic_solve10(_p(A), _p(b), _p(x), _p(w))
+    return x, w
 
 
 def svd_solve(A, b):

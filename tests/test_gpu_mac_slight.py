@@ -207,6 +207,38 @@ def test_free_filling_factor_normal_equations_match_oracle(gm, orc, cuda, free):
         assert c[i] == pytest.approx(orc.chisqr(o, obs[i], sp[i]), rel=2e-5)
 
 
+def test_ten_term_cordic_step_solve_matches_oracle(gm, orc, cuda):
+    """Filling factor free, mac fixed, solver 3: the 11-slot system (slot 9,
+    mac, a zero row) solved as the reference's 10-term one -- slots 0-8, then
+    alfa in slot 9 -- by the CORDIC solve; mac's step is zero."""
+    L = stray_profile()
+    p, _ = ext_pair(gm, orc, None, slight=L, free=FREE10, solver=gm.GM_SOLVE_CORDIC)
+    ctx = gm.Context(p)
+    rng = np.random.default_rng(46)
+    n = 128
+    packed = np.zeros((n, 66), dtype=np.float32)
+    lam = 10.0 ** rng.uniform(-4, 2, n)
+    for i in range(n):
+        J = rng.normal(size=(120, 11)) * rng.uniform(0.01, 10, 11)
+        M = J.T @ J
+        M[9, :] = M[:, 9] = 0.0
+        for q in range(11):
+            for r in range(q, 11):
+                packed[i, gm.packed_index(q, r, 11)] = M[q, r]
+    b = (rng.normal(size=(n, 11)) * 10).astype(np.float32)
+    b[:, 9] = 0.0
+    d = ctx.step_solve(to_dev(packed, cuda), to_dev(b, cuda), to_dev(lam, cuda)).cpu().numpy()
+    keep = [0, 1, 2, 3, 4, 5, 6, 7, 8, 10]
+    for i in range(n):
+        A = gm.unpack_alpha(packed[i]).astype(np.float64)
+        A[np.diag_indices(11)] *= 1.0 + lam[i]
+        x10, _ = orc.cordic_solve10(A[np.ix_(keep, keep)], b[i, keep].astype(np.float64))
+        x = np.zeros(11)
+        x[keep] = x10
+        assert d[i, 9] == 0.0
+        assert np.abs(d[i] - x).max() <= 2e-2 * np.abs(x).max() + 1e-12, (i, d[i], x)
+
+
 def test_eleven_slot_step_solve_matches_oracle(gm, orc, cuda):
     """FP64 LDL^T / Jacobi pseudo-inverse of 11x11 systems: well-conditioned
     damped ones, zero rows, and exactly singular integer ones (A = J^T J / 64
@@ -248,12 +280,14 @@ def test_eleven_slot_step_solve_matches_oracle(gm, orc, cuda):
         assert np.abs(d[i] - x).max() <= 1e-6 * np.abs(x).max() + 1e-12, (i, d[i], x)
 
 
-@pytest.mark.parametrize("free", [FREE10, FREE11])
-def test_free_filling_factor_inversion_matches_oracle(gm, orc, cuda, free):
+@pytest.mark.parametrize("free,solver", [(FREE10, 0), (FREE11, 0), (FREE10, 3), (FREE11, 3)])
+def test_free_filling_factor_inversion_matches_oracle(gm, orc, cuda, free, solver):
     """Inversions with the filling factor (and mac) free against the oracle's
-    10-/11-term LM: medians of |d| for B, gamma, phi, vlos and alfa."""
+    10-/11-term LM: medians of |d| for B, gamma, phi, vlos and alfa.  Under
+    solver 3 the 10-term run takes the reference's CORDIC solve (the 11-term
+    one keeps its FP64 SVD, as lm_mils_11)."""
     L = stray_profile()
-    p, o = ext_pair(gm, orc, GAUSS60, mac=1.0, alfa=0.9, slight=L, free=free)
+    p, o = ext_pair(gm, orc, GAUSS60, mac=1.0, alfa=0.9, slight=L, free=free, solver=solver)
     truth = ff_models(gm, 384, 45)
     if "mac" not in free:
         truth[:, 9] = 1.0

@@ -32,6 +32,8 @@ def pair(gm, orc, kind="single", psf=None, **kw):
     pkw = dict(kw)
     for k in ("precision", "solver", "psf_tail_tol", "init_mode"):
         okw.pop(k, None)
+    if pkw.get("solver") == gm.GM_SOLVE_CORDIC:
+        okw["solver"] = orc.SOLVE_CORDIC
     if pkw.get("init_mode", gm.GM_INIT_CE) != gm.GM_INIT_CE:
         okw["use_ce"] = 0
     if kind == "single":
@@ -143,7 +145,8 @@ def _random_systems(n, seed, degenerate=False):
     return A, b, lam
 
 
-@pytest.mark.parametrize("precision,solver", [(1, 0), (1, 1), (1, 2), (0, 0), (0, 1), (0, 2)])
+@pytest.mark.parametrize("precision,solver", [(1, 0), (1, 1), (1, 2), (0, 0), (0, 1), (0, 2),
+                                              (1, 3), (0, 3)])
 def test_step_solve_matches_oracle(gm, orc, cuda, precision, solver):
     p, _ = pair(gm, orc, "single", None, precision=precision, solver=solver)
     ctx = gm.Context(p)
@@ -155,11 +158,21 @@ def test_step_solve_matches_oracle(gm, orc, cuda, precision, solver):
     b32 = b.astype(np.float32)
     d = ctx.step_solve(to_dev(packed, cuda), to_dev(b32, cuda), to_dev(lam, cuda))
     d = d.cpu().numpy()
-    rtol = 1e-6 if precision == 1 else 2e-2
+    # the CORDIC solve is FP32 in both precision modes, as the reference's
+    rtol = 1e-6 if (precision == 1 and solver != gm.GM_SOLVE_CORDIC) else 2e-2
     for i in range(256):
         Af = gm.unpack_alpha(packed[i]).astype(np.float64)
         Af[np.diag_indices(9)] *= (1.0 + lam[i])
-        x, _ = orc.svd_solve(Af, b32[i].astype(np.float64))
+        if solver == gm.GM_SOLVE_CORDIC:
+            # the reference's 10 x 10 system: the tenth slot (alfa, fixed) is
+            # a zero row and gets a zero step
+            A10 = np.zeros((10, 10))
+            A10[:9, :9] = Af
+            x, _ = orc.cordic_solve10(A10, np.append(b32[i].astype(np.float64), 0.0))
+            assert x[9] == 0.0
+            x = x[:9]
+        else:
+            x, _ = orc.svd_solve(Af, b32[i].astype(np.float64))
         assert np.abs(d[i] - x).max() <= rtol * np.abs(x).max() + 1e-12, (i, d[i], x)
 
 
@@ -237,9 +250,10 @@ def _assert_medians(d, label=""):
     assert med[1] <= 1.0 and med[2] <= 1e-3 and med[5] <= 0.1 and med[6] <= 0.1, med
 
 
-@pytest.mark.parametrize("precision,solver", [(1, 0), (0, 0), (1, 1), (1, 2)])
+@pytest.mark.parametrize("precision,solver", [(1, 0), (0, 0), (1, 1), (1, 2), (1, 3), (0, 3)])
 def test_inversion_config0_matches_oracle(gm, orc, cuda, precision, solver):
-    """64 x 64 pixels, Fe I 6173, 30 wavelengths, no PSF, 9 free, 50 it."""
+    """64 x 64 pixels, Fe I 6173, 30 wavelengths, no PSF, 9 free, 50 it;
+    solver 3 (the reference's CORDIC solve) against the oracle's CORDIC LM."""
     got, ref, truth, obs, ctx = _inversion_errors(gm, orc, cuda, "single", None, 4096, 100,
                                                   precision=precision, solver=solver)
     d = param_errors(got["model"], ref["model"])
@@ -276,14 +290,15 @@ def test_inversion_two_lines_matches_oracle(gm, orc, cuda):
     _assert_medians(param_errors(got["model"], ref["model"]), "cfg2")
 
 
-def test_inversion_recovers_reference_example(gm, orc, cuda):
+@pytest.mark.parametrize("solver", [0, 3])
+def test_inversion_recovers_reference_example(gm, orc, cuda, solver):
     per = np.loadtxt(os.path.join(GOLDEN, "synthesis_gpu.per"))
     obs = np.ascontiguousarray(per[:, 2:6].T[None].astype(np.float32))
     guess = read_mod(os.path.join(GOLDEN, "guess.mod"))
     truth = model_row(read_mod(os.path.join(GOLDEN, "init.mod")))
     dl, val = crisp_table()
     p = gm.make_setup([gm.FE6173], gm.GRID_6173, psf=("table", dl, val), init=guess,
-                      miter=100)
+                      miter=100, solver=solver)
     out = gm.Context(p).invert(to_dev(obs, cuda))
     np.testing.assert_allclose(out["model"].cpu().numpy()[0, :9], truth[:9], rtol=5e-3, atol=2e-3)
 
