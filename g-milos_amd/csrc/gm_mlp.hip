@@ -23,38 +23,34 @@ namespace {
 typedef short short4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
+/* FP32 -> BF16, round to nearest even (v_cvt_pk_bf16_f32 on gfx950) */
 __device__ __forceinline__ uint16_t f2bf(float x)
 {
-    /* round to nearest even */
-    uint32_t u = __float_as_uint(x);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16) | ((u & 0xffffu) ? 0x40 : 0);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
+    const __bf16 b = (__bf16)x;
+    return __builtin_bit_cast(uint16_t, b);
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
-/* tanh(u) = 1 - 2 / (1 + exp(2u)): one exponential and one division
- * (absolute error ~1e-7, far below the BF16 rounding of the activations);
- * saturates to +-1 through exp's overflow to inf and underflow to 0. */
-__device__ __forceinline__ float tanh_fast(float u)
+/* GELU, tanh form (Hendrycks & Gimpel 2016), and its derivative, through
+ * 0.5 x (1 + tanh u) = x s with s = 1 / (1 + exp(-2u)),
+ * u = sqrt(2/pi) (x + 0.044715 x^3): one exp2 and one reciprocal
+ * (v_exp_f32, v_rcp_f32, about 1 ulp each, far below the BF16 rounding of
+ * the activations).  For large |x|, exp2 overflows to inf or underflows to
+ * 0 and s saturates to 0 or 1. */
+__device__ __forceinline__ float gelu_s(float x)
 {
-    return 1.f - __fdividef(2.f, 1.f + __expf(2.f * u));
+    constexpr float k = 0.7978845608028654f;          /* sqrt(2/pi) */
+    constexpr float l2e = 1.4426950408889634f;        /* log2(e)    */
+    const float m = x * fmaf(x * x, -2.f * k * 0.044715f * l2e, -2.f * k * l2e);   /* -2u log2 e */
+    return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(m));
 }
-
-/* GELU, tanh form (Hendrycks & Gimpel 2016) and its derivative. */
-__device__ __forceinline__ float gelu(float x)
-{
-    const float k = 0.7978845608028654f;   /* sqrt(2/pi) */
-    const float t = tanh_fast(k * (x + 0.044715f * x * x * x));
-    return 0.5f * x * (1.f + t);
-}
+__device__ __forceinline__ float gelu(float x) { return x * gelu_s(x); }
 __device__ __forceinline__ float gelu_grad(float x)
 {
-    const float k = 0.7978845608028654f;
-    const float u = k * (x + 0.044715f * x * x * x);
-    const float t = tanh_fast(u);
-    const float du = k * (1.f + 3.f * 0.044715f * x * x);
-    return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * du;
+    constexpr float k = 0.7978845608028654f;
+    const float s = gelu_s(x);
+    const float du = k * fmaf(3.f * 0.044715f, x * x, 1.f);
+    return fmaf(2.f * x * s * (1.f - s), du, s);
 }
 
 /* C[M x N] (+)= op(A)[M x K] . op(B)[K x N].
@@ -231,8 +227,9 @@ __global__ __launch_bounds__(256) void k_gemm_nn(const uint16_t *__restrict__ A,
 }
 
 /* Input: obs float [npix][4][nl] -> bf16 [npix][kin], normalised by the
- * continuum Ic = max(I[0], I[nl-1]); I -> 2 (I/Ic - 1), Q, U, V -> 10 x/Ic.
- * One thread per 8 consecutive features (one 16-byte store). */
+ * continuum Ic = max(I[0], I[nl-1]) (as x * (1/Ic)); I -> 2 (I/Ic - 1),
+ * Q, U, V -> 10 x/Ic.  One thread per 8 consecutive features (one 16-byte
+ * store). */
 __global__ void k_prep_input(const float *__restrict__ obs, int64_t npix, int nl, int kin,
                              uint16_t *__restrict__ x)
 {
@@ -243,14 +240,14 @@ __global__ void k_prep_input(const float *__restrict__ obs, int64_t npix, int nl
         const int64_t p = i / per;
         const int f0 = (int)(i % per) * 8;
         const float *o = obs + p * 4 * nl;
-        const float ic = fmaxf(fmaxf(o[0], o[nl - 1]), 1e-6f);
+        const float rc = 1.f / fmaxf(fmaxf(o[0], o[nl - 1]), 1e-6f);
         uint16_t h[8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int f = f0 + e;
             float v = 0.f;
             if (f < 4 * nl) {
-                v = o[f] / ic;
+                v = o[f] * rc;
                 v = (f < nl) ? (v - 1.f) * 2.f : v * 10.f;
             }
             h[e] = f2bf(v);
@@ -417,7 +414,7 @@ __global__ __launch_bounds__(256, 2) void k_mlp_infer(const float *__restrict__ 
 {
     __shared__ __attribute__((aligned(16))) uint16_t sX[FX_ROWS * FX_AS];
     __shared__ __attribute__((aligned(16))) uint16_t sW[GM_MLP_HIDDEN * FX_WS];
-    __shared__ float sIc[FX_ROWS];
+    __shared__ float sIc[FX_ROWS];          /* 1 / Ic per row */
     __shared__ float sY[FX_ROWS * 12];
     const int tid = threadIdx.x;
     const int64_t p0 = (int64_t)blockIdx.x * FX_ROWS;
@@ -438,12 +435,12 @@ __global__ __launch_bounds__(256, 2) void k_mlp_infer(const float *__restrict__ 
         if (idx < n4) v[j] = src[idx];
     }
     if (tid < FX_ROWS) {
-        float ic = 1.f;
+        float rc = 1.f;
         if (tid < rows) {
             const float *o = obs + (p0 + tid) * nf;
-            ic = fmaxf(fmaxf(o[0], o[nl - 1]), 1e-6f);
+            rc = 1.f / fmaxf(fmaxf(o[0], o[nl - 1]), 1e-6f);
         }
-        sIc[tid] = ic;
+        sIc[tid] = rc;
     }
     /* zero the pad columns and the rows past the end of the cube */
     for (int e = tid; e < FX_ROWS * (kin - nf); e += 256) {
@@ -460,12 +457,12 @@ __global__ __launch_bounds__(256, 2) void k_mlp_infer(const float *__restrict__ 
         const int idx = tid + 256 * j;
         if (idx < n4) {
             const int r = (4 * idx) / nf, f = 4 * idx - r * nf;
-            const float ic = sIc[r];
+            const float rc = sIc[r];
             const float x[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
             uint16_t h[4];
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                float y = x[c] / ic;
+                float y = x[c] * rc;
                 y = (f + c < nl) ? (y - 1.f) * 2.f : y * 10.f;
                 h[c] = f2bf(y);
             }
