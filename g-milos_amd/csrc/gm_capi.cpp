@@ -48,7 +48,12 @@ struct gm_ctx {
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     int timing = 0;
     std::vector<Events> kev;         /* per-kernel events when timing is on   */
+    std::vector<int> kclass;         /* their classes: 0 synth 1 rf 2 solve 3 other */
     gm_stats stats;
+    /* gm_invert returns once its work is enqueued; the round counters and
+     * event times of the last call are read back on the first gm_last_stats */
+    int stats_pending = 0;
+    int pending_miter = 0;
     int64_t slight_map_npix = 0;     /* pixels covered by the stray-light map */
 };
 
@@ -296,9 +301,48 @@ extern "C" gm_status gm_ctx_set_timing(gm_ctx *c, int32_t enable)
     return GM_OK;
 }
 
+/* Waits for the last gm_invert and turns its round counters and event
+ * times into gm_stats. */
+static gm_status finish_stats(gm_ctx *c)
+{
+    if (!c->stats_pending) return GM_OK;
+    GM_HIP(hipEventSynchronize(c->ev_stop));
+    c->stats_pending = 0;
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->ev_start, c->ev_stop);
+    c->stats.ms_total = ms;
+    const int64_t npix = c->stats.npix;
+    c->stats.synth_evals = npix;
+    c->stats.rf_evals = npix;
+    int rounds = 0;
+    for (int r = 0; r < c->pending_miter; ++r) {
+        const int32_t *hc = c->h_counts + 3 * r;
+        if (hc[0] > 0) rounds = r + 1;
+        c->stats.solves += hc[0];
+        c->stats.svd_solves += hc[1];
+        c->stats.synth_evals += hc[0];
+        c->stats.rf_evals += hc[2];
+    }
+    c->stats.lm_rounds = rounds;
+    if (c->timing) {
+        for (size_t i = 0; i < c->kclass.size(); ++i) {
+            float t = 0;
+            (void)hipEventElapsedTime(&t, c->kev[i].a, c->kev[i].b);
+            switch (c->kclass[i]) {
+            case 0: c->stats.ms_synth += t; break;
+            case 1: c->stats.ms_rf += t; break;
+            case 2: c->stats.ms_solve += t; break;
+            default: c->stats.ms_other += t; break;
+            }
+        }
+    }
+    return GM_OK;
+}
+
 extern "C" gm_status gm_last_stats(const gm_ctx *c, gm_stats *s)
 {
     if (!c || !s) return GM_ERR_INVALID;
+    GM_TRY(finish_stats(const_cast<gm_ctx *>(c)));
     *s = c->stats;
     return GM_OK;
 }
@@ -372,6 +416,7 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
     if (!c || !out || npix < 0) return GM_ERR_INVALID;
     if (npix == 0) {
         std::memset(&c->stats, 0, sizeof c->stats);
+        c->stats_pending = 0;
         return GM_OK;
     }
     if (!obs || npix > (int64_t)0x7fffffff) return GM_ERR_INVALID;
@@ -388,12 +433,13 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
     gm_state st = c->st;
     const gm_lanecfg &lc = c->lc;
     int32_t *cnt = st.counts;
-    std::vector<int> kclass;   /* 0 synth, 1 rf, 2 solve, 3 other */
+    c->kclass.clear();
+    c->stats_pending = 0;
     size_t kidx = 0;
     auto kbegin = [&](int cls) {
         if (!c->timing) return;
         hipEventRecord(kernel_events(c, kidx).a, s);
-        kclass.push_back(cls);
+        c->kclass.push_back(cls);
     };
     auto kend = [&]() {
         if (!c->timing) return;
@@ -460,37 +506,11 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
     kbegin(3);
     gm_launch_finalize(st, npix, *out, s);
     kend();
+    GM_HIP(hipMemcpyAsync(c->h_counts, hist, sizeof(int32_t) * 3 * miter, hipMemcpyDeviceToHost, s));
     GM_HIP(hipEventRecord(c->ev_stop, s));
     GM_HIP(hipGetLastError());
-    GM_HIP(hipEventSynchronize(c->ev_stop));
-    float ms = 0;
-    hipEventElapsedTime(&ms, c->ev_start, c->ev_stop);
-    c->stats.ms_total = ms;
-    GM_HIP(hipMemcpy(c->h_counts, hist, sizeof(int32_t) * 3 * miter, hipMemcpyDeviceToHost));
-    c->stats.synth_evals = npix;
-    c->stats.rf_evals = npix;
-    int rounds = 0;
-    for (int r = 0; r < miter; ++r) {
-        const int32_t *hc = c->h_counts + 3 * r;
-        if (hc[0] > 0) rounds = r + 1;
-        c->stats.solves += hc[0];
-        c->stats.svd_solves += hc[1];
-        c->stats.synth_evals += hc[0];
-        c->stats.rf_evals += hc[2];
-    }
-    c->stats.lm_rounds = rounds;
-    if (c->timing) {
-        for (size_t i = 0; i < kidx; ++i) {
-            float t = 0;
-            hipEventElapsedTime(&t, c->kev[i].a, c->kev[i].b);
-            switch (kclass[i]) {
-            case 0: c->stats.ms_synth += t; break;
-            case 1: c->stats.ms_rf += t; break;
-            case 2: c->stats.ms_solve += t; break;
-            default: c->stats.ms_other += t; break;
-            }
-        }
-    }
+    c->stats_pending = 1;
+    c->pending_miter = miter;
     return GM_OK;
 }
 

@@ -168,12 +168,19 @@ const char *gm_status_string(gm_status st);
 gm_status gm_ctx_set_stray_map(gm_ctx *ctx, const float *map, int64_t npix);
 /* Enables per-kernel-class event timing in gm_stats (adds syncs). */
 gm_status gm_ctx_set_timing(gm_ctx *ctx, int32_t enable);
+/* Statistics of the context's last gm_invert; waits for that call to
+ * finish on its stream. */
 gm_status gm_last_stats(const gm_ctx *ctx, gm_stats *stats);
 
 /* ---- inversion ---------------------------------------------------------- */
 /* Inverts npix pixels: the work of lm_mils (milosUtils.cu:574-710) launched
  * by gmilos.cu:1233-1240.  init is [npix][11] when init_mode is
- * GM_INIT_PER_PIXEL, otherwise ignored (may be NULL).                      */
+ * GM_INIT_PER_PIXEL, otherwise ignored (may be NULL).  Returns once the work
+ * is enqueued on `stream` (no host wait, except when the context's state
+ * must grow for a larger npix): the outputs are complete when the stream
+ * reaches this point, so copies or kernels queued behind it on the same
+ * stream see them, and host work or other streams overlap it.  Calls on one
+ * context must use one stream at a time.                                   */
 gm_status gm_invert(gm_ctx *ctx, const float *obs, int64_t npix,
                     const double *init, const gm_out *out, void *stream);
 
