@@ -2234,6 +2234,13 @@ __global__ void k_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t v)
  * iteration and acceptance counts and the last evaluated spectrum from the
  * state slot to the owning tile's output, one wave per pixel; the last pixel
  * of a tile appends the tile to the completed list (completed[0] = count). */
+/* Copies retired pixels into their tiles' outputs (one wave per pixel) and
+ * counts them off their tiles: one lane per pixel, and one atomic per wave
+ * when the wave's 64 consecutive retired pixels share a tile (the common
+ * case: a tile is a contiguous slot range and pixels retire in slot order),
+ * so a large tile finishing in one round does not serialise on one counter.
+ * The host reads the completed list after the launch, so no fence orders
+ * the copies before the counts. */
 __global__ __launch_bounds__(256) void k_retire(gm_state st, const int32_t *__restrict__ done,
                                                 const int32_t *__restrict__ done_count,
                                                 const float *__restrict__ synth, int nl4,
@@ -2244,11 +2251,11 @@ __global__ __launch_bounds__(256) void k_retire(gm_state st, const int32_t *__re
     const int64_t cnt = *done_count;
     const int64_t cap = st.cap;
     const int lane = threadIdx.x & 63;
-    for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < cnt;
-         i += ((int64_t)gridDim.x * blockDim.x) >> 6) {
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t i = wave; i < cnt; i += nwaves) {
         const int64_t slot = done[i];
-        const int t = slot_tile[slot];
-        gm_tile_dev &g = tiles[t];
+        const gm_tile_dev &g = tiles[slot_tile[slot]];
         const int64_t idx = slot - g.slot0;
         if (g.synth)
             for (int k = lane; k < nl4; k += 64) g.synth[idx * nl4 + k] = synth[slot * nl4 + k];
@@ -2258,9 +2265,18 @@ __global__ __launch_bounds__(256) void k_retire(gm_state st, const int32_t *__re
             if (g.iters) g.iters[idx] = st.iter[slot];
             if (g.naccs) g.naccs[idx] = st.nacc[slot];
         }
-        __threadfence();
-        if (lane == 0 && atomicSub(&g.remaining, 1) == 1)
-            completed[1 + atomicAdd(&completed[0], 1)] = t;
+    }
+    for (int64_t b0 = wave * 64; b0 < cnt; b0 += nwaves * 64) {
+        const int n = (cnt - b0 < 64) ? (int)(cnt - b0) : 64;
+        const int t = (lane < n) ? slot_tile[done[b0 + lane]] : -1;
+        const int t0 = __shfl(t, 0, 64);
+        if (__all(lane >= n || t == t0)) {
+            if (lane == 0 && atomicSub(&tiles[t0].remaining, n) == n)
+                completed[1 + atomicAdd(&completed[0], 1)] = t0;
+        } else if (lane < n) {
+            if (atomicSub(&tiles[t].remaining, 1) == 1)
+                completed[1 + atomicAdd(&completed[0], 1)] = t;
+        }
     }
 }
 
