@@ -55,9 +55,13 @@ def _ext_setup(gm):
                                 init=dict(mac=0.8, alfa=0.9), slight=stray)
 
 
-@pytest.mark.parametrize("kind", ["plain", "ext"])
+@pytest.mark.parametrize("kind", ["plain", "ext", "cordic", "ext_cordic"])
 def test_server_tile_matches_invert(gm, cuda, kind):
-    setup = _setups(gm)[0] if kind == "plain" else _ext_setup(gm)
+    """Plain and eleven-slot set-ups, each also with the reference's CORDIC
+    step solve (the ext set-up is then a 10-term system, mac fixed)."""
+    setup = _setups(gm)[0] if kind in ("plain", "cordic") else _ext_setup(gm)
+    if kind.endswith("cordic"):
+        setup.solver = gm.GM_SOLVE_CORDIC
     ctx = gm.Context(setup)
     obs = _obs(gm, ctx, 3000, seed=41)
     ref = ctx.invert(obs)
