@@ -6,9 +6,11 @@
  * __constant__ symbols, the cube is split over streams and threads, and one
  * kernel runs the whole per-pixel LM with device-side child launches.  Here a
  * context holds the set-up in device memory, the LM state lives in HBM as
- * structure-of-arrays, and each LM round is four stream-ordered launches over
- * compacted pixel lists: step solve -> synthesis + chi^2 -> accept/reject ->
- * response functions + normal equations for accepted pixels.
+ * per-pixel records (models, synthesis constants, normal equations) beside
+ * per-pixel scalar arrays, and each LM round is four stream-ordered launches
+ * over compacted pixel lists: step solve -> synthesis + chi^2 ->
+ * accept/reject -> response functions + normal equations for accepted
+ * pixels.  gm_invert only enqueues; gm_last_stats reads the round counters.
  */
 #include <hip/hip_runtime.h>
 
@@ -347,13 +349,16 @@ extern "C" gm_status gm_last_stats(const gm_ctx *c, gm_stats *s)
     return GM_OK;
 }
 
-/* Carves the SoA state out of one allocation sized for `cap` pixels. */
+/* Accessors for the serving loop and the learned-guess driver. */
 const gm_dsetup *gm_ctx_device_setup(const gm_ctx *c) { return c->dS; }
 const gm_dsetup &gm_ctx_host_setup(const gm_ctx *c) { return c->host_ds; }
 const gm_lanecfg &gm_ctx_lanes(const gm_ctx *c) { return c->lc; }
 const gm_setup &gm_ctx_public_setup(const gm_ctx *c) { return c->setup; }
 int gm_ctx_device(const gm_ctx *c) { return c->device; }
 
+/* Carves the LM state (records and scalar arrays) out of one allocation
+ * sized for `cap` pixels; grows it, after a device synchronisation, when a
+ * call needs more. */
 static gm_status ensure_state(gm_ctx *c, int64_t npix)
 {
     if (npix <= c->st.cap) return GM_OK;
