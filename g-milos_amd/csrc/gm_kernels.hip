@@ -1622,8 +1622,7 @@ __device__ __forceinline__ void solve_svd_n(T (&a)[N * (N + 1) / 2], const T (&b
  * CP[i] (the systolic exchange, gmilos.cu:108-128).  No value moves here:
  * SIG.s[k][i] is the register slot that holds index i after k steps, known
  * at compile time; CP is a 9-cycle with 0 fixed, so every nine steps each
- * index is back in its own slot, and two passes of nine steps are the
- * reference's 18.                                                          */
+ * index is back in its own slot after the reference's 18 steps.          */
 namespace cordic {
 constexpr int CP[10] = {0, 2, 4, 1, 6, 3, 8, 5, 9, 7};
 struct Frames {
@@ -1655,22 +1654,31 @@ __device__ __forceinline__ void rot_block(float &b00, float &b01, float &b10, fl
     b11 = o21 * sc + o22 * cc;
 }
 
-template <int K>
-__device__ __forceinline__ void step(float (&m)[55], float (&v)[100])
+/* Step K18 (0-17) of the sweep pair: the angles of the five diagonal blocks
+ * (kept for the back-substitution), the two-sided block rotations, and the
+ * same rotations applied to z = V^T b.  V itself is never formed: with V the
+ * product of the step rotations G_0 ... G_17 (in register-slot terms), V^T b
+ * is the rotations applied to b in order and V u is their transposes applied
+ * in reverse order (rev). */
+template <int K18>
+__device__ __forceinline__ void fwd(float (&m)[55], float (&z)[10], volatile float *th)
 {
+    constexpr int K = K18 % 9;
     float c[5], s[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
         const int i = SIG.s[K][2 * k], j = SIG.s[K][2 * k + 1];
         const float a11 = m[utn<10>(i, i)], a12 = m[utn<10>(i, j)], a22 = m[utn<10>(j, j)];
-        float th;
+        float t;
         if (a11 == a22) {
-            th = QPI;
+            t = QPI;
         } else {
-            th = __fdividef(atanf(__fdividef(2.f * a12, a22 - a11)), 2.f);
-            th = th > QPI ? QPI : (th < -QPI ? -QPI : th);
+            /* __fdividef of the reference: a reciprocal and a multiply */
+            t = 0.5f * atanf(2.f * a12 * __builtin_amdgcn_rcpf(a22 - a11));
+            t = t > QPI ? QPI : (t < -QPI ? -QPI : t);
         }
-        __sincosf(th, &s[k], &c[k]);
+        th[(K18 * 5 + k) * 64] = t;
+        __sincosf(t, &s[k], &c[k]);
     }
 #pragma unroll
     for (int bi = 0; bi < 5; ++bi)
@@ -1691,52 +1699,57 @@ __device__ __forceinline__ void step(float (&m)[55], float (&v)[100])
             }
         }
 #pragma unroll
-    for (int f = 0; f < 10; ++f)
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            const int q0 = SIG.s[K][2 * k], q1 = SIG.s[K][2 * k + 1];
-            const float v0 = v[f * 10 + q0], v1 = v[f * 10 + q1];
-            v[f * 10 + q0] = v0 * c[k] - v1 * s[k];
-            v[f * 10 + q1] = v0 * s[k] + v1 * c[k];
-        }
+    for (int k = 0; k < 5; ++k) {
+        const int q0 = SIG.s[K][2 * k], q1 = SIG.s[K][2 * k + 1];
+        const float z0 = z[q0], z1 = z[q1];
+        z[q0] = z0 * c[k] - z1 * s[k];
+        z[q1] = z0 * s[k] + z1 * c[k];
+    }
+    if constexpr (K18 < 17) fwd<K18 + 1>(m, z, th);
 }
 
-template <int K>
-__device__ __forceinline__ void nine_steps(float (&m)[55], float (&v)[100])
+template <int K18>
+__device__ __forceinline__ void rev(float (&u)[10], volatile float *th)
 {
-    step<K>(m, v);
-    if constexpr (K < 8) nine_steps<K + 1>(m, v);
+    constexpr int K = K18 % 9;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        float s, c;
+        __sincosf(th[(K18 * 5 + k) * 64], &s, &c);
+        const int q0 = SIG.s[K][2 * k], q1 = SIG.s[K][2 * k + 1];
+        const float u0 = u[q0], u1 = u[q1];
+        u[q0] = u0 * c + u1 * s;
+        u[q1] = u1 * c - u0 * s;
+    }
+    if constexpr (K18 > 0) rev<K18 - 1>(u, th);
 }
 
-/* m: packed damped 10 x 10 upper triangle (destroyed); b, x: 10 */
-__device__ __forceinline__ void solve10(float (&m)[55], const float (&b)[10], float (&x)[10])
+/* m: packed damped 10 x 10 upper triangle (destroyed); b, x: 10; th: this
+ * lane's column of a [90][64] LDS array for the step angles (kept out of
+ * registers, which the two matrices fill).  After the 18 steps every index
+ * is back in its own register slot (CP^18 = I), so the diagonal of m is w in
+ * index order. */
+constexpr int TH_FLOATS = 90 * 64;
+__device__ __forceinline__ void solve10(float (&m)[55], const float (&b)[10], float (&x)[10],
+                                        volatile float *th)
 {
-    float v[100];
+    float z[10];
 #pragma unroll
-    for (int e = 0; e < 100; ++e) v[e] = (e % 11 == 0) ? 1.f : 0.f;
-    for (int pass = 0; pass < 2; ++pass) nine_steps<0>(m, v);
-    float aux[10];
+    for (int i = 0; i < 10; ++i) z[i] = b[i];
+    fwd<0>(m, z, th);
 #pragma unroll
     for (int j = 0; j < 10; ++j) {
-        float sum = 0.f;
-#pragma unroll
-        for (int k = 0; k < 10; ++k) sum += b[k] * v[k * 10 + j];
         const float w = m[utn<10>(j, j)];
-        aux[j] = sum * ((fabsf(w) > 1e-12f) ? 1.f / w : 0.f);
+        x[j] = z[j] * ((fabsf(w) > 1e-12f) ? 1.f / w : 0.f);
     }
-#pragma unroll
-    for (int i = 0; i < 10; ++i) {
-        float sum = 0.f;
-#pragma unroll
-        for (int k = 0; k < 10; ++k) sum += v[i * 10 + k] * aux[k];
-        x[i] = sum;
-    }
+    rev<17>(x, th);
 }
 
 /* The 9-slot system (packed 45) as the reference's 10 x 10 with slot 9 (the
  * filling factor's response, zero while alfa is fixed) a zero row. */
 template <typename T>
-__device__ __forceinline__ void solve9(const T (&a)[45], const T (&b)[9], T (&x)[9])
+__device__ __forceinline__ void solve9(const T (&a)[45], const T (&b)[9], T (&x)[9],
+                                       volatile float *th)
 {
     float m[55], bb[10], xx[10];
 #pragma unroll
@@ -1746,7 +1759,7 @@ __device__ __forceinline__ void solve9(const T (&a)[45], const T (&b)[9], T (&x)
 #pragma unroll
     for (int p = 0; p < 9; ++p) bb[p] = (float)b[p];
     bb[9] = 0.f;
-    solve10(m, bb, xx);
+    solve10(m, bb, xx, th);
 #pragma unroll
     for (int p = 0; p < 9; ++p) x[p] = (T)xx[p];
 }
@@ -1754,7 +1767,7 @@ __device__ __forceinline__ void solve9(const T (&a)[45], const T (&b)[9], T (&x)
 /* The 11-slot system with mac fixed (slot 9 a zero row) as the reference's
  * 10-term one: slots 0-8, then the filling factor in slot 9. */
 __device__ __forceinline__ void solve11_as10(const double (&a)[66], const double (&b)[11],
-                                             double (&x)[11])
+                                             double (&x)[11], volatile float *th)
 {
     constexpr int MAP[10] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 10};
     float m[55], bb[10], xx[10];
@@ -1764,7 +1777,7 @@ __device__ __forceinline__ void solve11_as10(const double (&a)[66], const double
         for (int j = i; j < 10; ++j) m[utn<10>(i, j)] = (float)a[utn<11>(MAP[i], MAP[j])];
 #pragma unroll
     for (int p = 0; p < 10; ++p) bb[p] = (float)b[MAP[p]];
-    solve10(m, bb, xx);
+    solve10(m, bb, xx, th);
 #pragma unroll
     for (int p = 0; p < 9; ++p) x[p] = xx[p];
     x[9] = 0.0;
@@ -1800,7 +1813,8 @@ __global__ __launch_bounds__(64) void k_solve11(const gm_dsetup *__restrict__ S,
         load_damped_n<double, 11>(st.alpha + pix * st.ab_rec, st.beta + pix * st.ab_rec, 1,
                                   st.flambda[pix], a, b);
         if (cordic10) {
-            cordic::solve11_as10(a, b, x);
+            __shared__ float th[cordic::TH_FLOATS];
+            cordic::solve11_as10(a, b, x, th + threadIdx.x);
         } else {
             double a2[66];
 #pragma unroll
@@ -1832,7 +1846,8 @@ __global__ __launch_bounds__(64) void k_step_solve11(const float *__restrict__ a
         double a[66], b[11], x[11];
         load_damped_n<double, 11>(alpha + i * 66, beta + i * 11, 1, flambda[i], a, b);
         if (cordic10) {
-            cordic::solve11_as10(a, b, x);
+            __shared__ float th[cordic::TH_FLOATS];
+            cordic::solve11_as10(a, b, x, th + threadIdx.x);
         } else {
             double a2[66];
 #pragma unroll
@@ -1879,8 +1894,9 @@ __global__ __launch_bounds__(64) void k_solve(const gm_dsetup *__restrict__ S,
         T a[45], b[9], x[9];
         load_damped<T>(st.alpha + pix * st.ab_rec, st.beta + pix * st.ab_rec, 1,
                        (T)st.flambda[pix], a, b);
-        if (SOLVER == GM_SOLVE_CORDIC) {
-            cordic::solve9<T>(a, b, x);
+        if constexpr (SOLVER == GM_SOLVE_CORDIC) {
+            __shared__ float th[cordic::TH_FLOATS];
+            cordic::solve9<T>(a, b, x, th + threadIdx.x);
             apply_to_trial<T>(S, st, pix, x);
             continue;
         }
@@ -1932,8 +1948,9 @@ __global__ __launch_bounds__(64) void k_step_solve(const float *__restrict__ alp
         /* caller layout: alpha [n][45], beta [n][9] */
         T a[45], b[9], x[9];
         load_damped<T>(alpha + i * 45, beta + i * 9, 1, (T)flambda[i], a, b);
-        if (SOLVER == GM_SOLVE_CORDIC) {
-            cordic::solve9<T>(a, b, x);
+        if constexpr (SOLVER == GM_SOLVE_CORDIC) {
+            __shared__ float th[cordic::TH_FLOATS];
+            cordic::solve9<T>(a, b, x, th + threadIdx.x);
         } else if (SOLVER == GM_SOLVE_JACOBI) {
             solve_svd<T, (sizeof(T) == 8 ? 12 : 10)>(a, b, x);
         } else {
