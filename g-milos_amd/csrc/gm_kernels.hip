@@ -537,7 +537,7 @@ __device__ __forceinline__ LmStep<T> lm_step(const gm_dsetup *__restrict__ S, T 
 /* ------------------------------------------------------------------------ */
 /* Synthesis + response functions + PSF + normal equations + chi^2 for a list
  * of pixels; wavelength-parallel mapping.                                   */
-template <int LPP, int NLPL, bool MIXED, int XM>
+template <int LPP, int NLPL, bool MIXED, int XM, bool REC>
 __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
     const gm_dsetup *__restrict__ S, const double *__restrict__ models,
     const float *__restrict__ pre, int64_t mfs, int64_t mps, const float *__restrict__ obs,
@@ -556,12 +556,32 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
     constexpr int NRED = NA + NP + 1;            /* alpha, beta, chi^2         */
     constexpr int PPW = 64 / LPP;
     constexpr int RSTR = P11 ? 80 : 60;        /* reduction row stride (floats) */
+    /* REC: the LM loop's calls — models and constants in per-pixel records,
+     * normal equations into per-pixel records, a device pixel list,
+     * observations, no response output.  Fixed at compile time, the strides
+     * and null tests need no scalar registers: with them at run time the
+     * kernel spills about 130 SGPRs to VGPR lanes and reads them back with
+     * one v_readlane (a VALU issue) each per pixel pair. */
+    if constexpr (REC) {
+        mfs = 1;
+        mps = GM_MODEL_REC;
+        aes = 1;
+        bes = 1;
+        aps = P11 ? GM_AB_REC11 : GM_AB_REC9;
+        bps = aps;
+    }
+    const bool has_pre = REC || pre != nullptr;
+    const bool has_list = REC || list != nullptr;
+    const bool has_obs = REC || obs != nullptr;
+    const bool has_alpha = REC || alpha != nullptr;
+    const bool has_beta = REC || beta != nullptr;
+    const bool has_rf = !REC && rf_out != nullptr;
     extern __shared__ float4 lds4[];
     const int lane = threadIdx.x;
     const int h = lane / LPP;
     const int k0 = lane % LPP;
     const int n = S->nlambda;
-    const int64_t cnt = count ? (int64_t)(*count) : n_direct;
+    const int64_t cnt = (REC || count) ? (int64_t)(*count) : n_direct;
     const int region4 = max(rows * 9, (LPP / 2) * (RSTR / 4));
     float4 *conv = lds4 + h * region4;
     float *red = reinterpret_cast<float *>(conv);
@@ -579,9 +599,9 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
          base += (int64_t)gridDim.x * PPW) {
         const int64_t item = base + h;
         const bool valid = item < cnt;
-        const int64_t pix = list ? (int64_t)list[valid ? item : base] : (valid ? item : base);
+        const int64_t pix = has_list ? (int64_t)list[valid ? item : base] : (valid ? item : base);
         PixPre pp;
-        if (pre) {
+        if (has_pre) {
             pp = load_pre(pre, pix);
         } else {
             double m[GM_NMODEL];
@@ -605,7 +625,7 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
             const int k = k0 + s * LPP;
 #pragma unroll
             for (int st = 0; st < 4; ++st)
-                ob[s][st] = (obs && k < n) ? obs[pix * 4 * n + st * n + k] : 0.f;
+                ob[s][st] = (has_obs && k < n) ? obs[pix * 4 * n + st * n + k] : 0.f;
         }
 
         /* zero padding rows (the reduction buffer aliases them) */
@@ -758,7 +778,7 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
                 float r[4];
 #pragma unroll
                 for (int st = 0; st < 4; ++st) {
-                    r[st] = obs ? y[st * 9] - ob[s][st] : 0.f;
+                    r[st] = has_obs ? y[st * 9] - ob[s][st] : 0.f;
 #pragma unroll
                     for (int p = 0; p < 9; ++p) {
                         const int js = staged_slot(p);
@@ -771,7 +791,7 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
                     }
                 }
 #pragma unroll
-                for (int st = 0; st < 4 && obs; ++st) {
+                for (int st = 0; st < 4 && has_obs; ++st) {
                     const float ws = S->wsig[st];
                     if (MIXED)
                         chi += (double)ws * ((double)r[st] * (double)r[st]);
@@ -792,7 +812,7 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
 #pragma unroll
                     for (int st = 0; st < 4; ++st) so[st * n] = y[st * 9];
                 }
-                if (rf_out && valid) {
+                if (has_rf && valid) {
                     if constexpr (P11) {
                         /* reference layout [4][NTERMS][n]: NTERMS 10 keeps
                          * the filling factor in slot 9, 11 has mac in 9 and
@@ -840,9 +860,9 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
             for (int l = 0; l < HL; ++l) sum += red[l * RSTR + e];
             if (valid) {
                 if (e < NA) {
-                    if (alpha) alpha[e * aes + pix * aps] = sum;
+                    if (has_alpha) alpha[e * aes + pix * aps] = sum;
                 } else {
-                    if (beta) beta[(e - NA) * bes + pix * bps] = sum;
+                    if (has_beta) beta[(e - NA) * bes + pix * bps] = sum;
                 }
             }
         }
@@ -865,7 +885,7 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
 }
 
 /* Synthesis + PSF + chi^2 only (rejected steps need nothing more). */
-template <int LPP, int NLPL, bool MIXED, bool EXT>
+template <int LPP, int NLPL, bool MIXED, bool EXT, bool REC>
 __global__ __launch_bounds__(64) void k_synth(
     const gm_dsetup *__restrict__ S, const double *__restrict__ models,
     const float *__restrict__ pre, int64_t mfs, int64_t mps, const float *__restrict__ obs,
@@ -874,12 +894,23 @@ __global__ __launch_bounds__(64) void k_synth(
     int rows, int padlo)
 {
     constexpr int PPW = 64 / LPP;
+    /* REC: the LM loop's call (per-pixel model records with constants, a
+     * device pixel list, observations, a chi^2 output), fixed at compile
+     * time as in k_rf */
+    if constexpr (REC) {
+        mfs = 1;
+        mps = GM_MODEL_REC;
+    }
+    const bool has_pre = REC || pre != nullptr;
+    const bool has_list = REC || list != nullptr;
+    const bool has_obs = REC || obs != nullptr;
+    const bool has_chi = REC || chi_out != nullptr;
     extern __shared__ float4 lds4[];
     const int lane = threadIdx.x;
     const int h = lane / LPP;
     const int k0 = lane % LPP;
     const int n = S->nlambda;
-    const int64_t cnt = count ? (int64_t)(*count) : n_direct;
+    const int64_t cnt = (REC || count) ? (int64_t)(*count) : n_direct;
     float4 *conv = lds4 + h * rows;
     /* EXT: macroturbulence kernel buffers after the PPW row blocks */
     float *gk = reinterpret_cast<float *>(lds4 + PPW * rows) + h * 2 * (LPP * NLPL);
@@ -895,9 +926,9 @@ __global__ __launch_bounds__(64) void k_synth(
          base += (int64_t)gridDim.x * PPW) {
         const int64_t item = base + h;
         const bool valid = item < cnt;
-        const int64_t pix = list ? (int64_t)list[valid ? item : base] : (valid ? item : base);
+        const int64_t pix = has_list ? (int64_t)list[valid ? item : base] : (valid ? item : base);
         PixPre pp;
-        if (pre) {
+        if (has_pre) {
             pp = load_pre(pre, pix);
         } else {
             double m[GM_NMODEL];
@@ -912,7 +943,7 @@ __global__ __launch_bounds__(64) void k_synth(
             const int k = k0 + s * LPP;
 #pragma unroll
             for (int st = 0; st < 4; ++st)
-                ob[s][st] = (obs && k < n) ? obs[pix * 4 * n + st * n + k] : 0.f;
+                ob[s][st] = (has_obs && k < n) ? obs[pix * 4 * n + st * n + k] : 0.f;
         }
 #pragma unroll
         for (int s = 0; s < NLPL; ++s) {
@@ -971,7 +1002,7 @@ __global__ __launch_bounds__(64) void k_synth(
                 y.z = alv * y.z + (1.f - alv) * sl[2 * n + k];
                 y.w = alv * y.w + (1.f - alv) * sl[3 * n + k];
             }
-            if (k < n && obs) {
+            if (k < n && has_obs) {
                 const float r0 = y.x - ob[s][0], r1 = y.y - ob[s][1], r2 = y.z - ob[s][2],
                             r3 = y.w - ob[s][3];
                 if (MIXED) {
@@ -993,7 +1024,7 @@ __global__ __launch_bounds__(64) void k_synth(
             chif = group_sum<float, LPP>(chif);
             chi = (double)(chif / (float)nfree);
         }
-        if (chi_out && valid && k0 == 0) chi_out[pix] = chi;
+        if (has_chi && valid && k0 == 0) chi_out[pix] = chi;
         __syncthreads();
     }
 }
@@ -2346,10 +2377,23 @@ void launch_rf_t(const gm_lanecfg &lc, const gm_dsetup *dS, const double *models
     const size_t shm = (size_t)lc.ppw * region4 * sizeof(float4) +
                        (size_t)lc.ppw * (xm == 2 ? 4 : (xm == 1 ? 2 : 0)) * LPP * NLPL * sizeof(float);
     const int g = grid_for(grid_items, lc.ppw, kMaxBlocks);
-#define GM_RF_CASE(MX, X)                                                                        \
-    hipLaunchKernelGGL((k_rf<LPP, NLPL, MX, X>), dim3(g), dim3(64), shm, s, dS, models, pre, mfs,  \
-                       mps, obs, list, count, n_direct, alpha, beta, aes, aps, bes, bps, chi, synth,  \
-                       rf, lc.rows, lc.padlo, alpha_first, model_dst, pre_dst)
+    /* the LM loop's calls take the specialised kernel (fixed record strides) */
+    const int64_t abrec = xm == 2 ? GM_AB_REC11 : GM_AB_REC9;
+    const bool rec = mfs == 1 && mps == GM_MODEL_REC && aes == 1 && bes == 1 && aps == abrec &&
+                     bps == abrec && pre && list && count && obs && alpha && beta && !rf;
+#define GM_RF_CASE(MX, X)                                                                         \
+    do {                                                                                          \
+        if (rec)                                                                                  \
+            hipLaunchKernelGGL((k_rf<LPP, NLPL, MX, X, true>), dim3(g), dim3(64), shm, s, dS,     \
+                               models, pre, mfs, mps, obs, list, count, n_direct, alpha, beta,    \
+                               aes, aps, bes, bps, chi, synth, rf, lc.rows, lc.padlo, alpha_first, \
+                               model_dst, pre_dst);                                               \
+        else                                                                                      \
+            hipLaunchKernelGGL((k_rf<LPP, NLPL, MX, X, false>), dim3(g), dim3(64), shm, s, dS,    \
+                               models, pre, mfs, mps, obs, list, count, n_direct, alpha, beta,    \
+                               aes, aps, bes, bps, chi, synth, rf, lc.rows, lc.padlo, alpha_first, \
+                               model_dst, pre_dst);                                               \
+    } while (0)
     if (xm == 2) {
         if (mixed) GM_RF_CASE(true, 2);
         else GM_RF_CASE(false, 2);
@@ -2372,9 +2416,18 @@ void launch_synth_t(const gm_lanecfg &lc, const gm_dsetup *dS, const double *mod
     const size_t shm = (size_t)lc.ppw * lc.rows * sizeof(float4) +
                        (lc.ext ? (size_t)lc.ppw * 2 * LPP * NLPL * sizeof(float) : 0);
     const int g = grid_for(grid_items, lc.ppw, kMaxBlocks);
+    const bool rec = mfs == 1 && mps == GM_MODEL_REC && pre && list && count && obs && chi;
 #define GM_SYNTH_CASE(MX, EX)                                                                     \
-    hipLaunchKernelGGL((k_synth<LPP, NLPL, MX, EX>), dim3(g), dim3(64), shm, s, dS, models, pre,  \
-                       mfs, mps, obs, list, count, n_direct, chi, synth, lc.rows, lc.padlo)
+    do {                                                                                          \
+        if (rec)                                                                                  \
+            hipLaunchKernelGGL((k_synth<LPP, NLPL, MX, EX, true>), dim3(g), dim3(64), shm, s, dS, \
+                               models, pre, mfs, mps, obs, list, count, n_direct, chi, synth,     \
+                               lc.rows, lc.padlo);                                                \
+        else                                                                                      \
+            hipLaunchKernelGGL((k_synth<LPP, NLPL, MX, EX, false>), dim3(g), dim3(64), shm, s,    \
+                               dS, models, pre, mfs, mps, obs, list, count, n_direct, chi, synth, \
+                               lc.rows, lc.padlo);                                                \
+    } while (0)
     if (lc.ext) {
         if (mixed) GM_SYNTH_CASE(true, true);
         else GM_SYNTH_CASE(false, true);
