@@ -705,9 +705,21 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
                     }
                 }
             } else if (use_psf) {
+                /* the first tap initialises the sums (dlo <= dhi whenever
+                 * the PSF is on) */
+                {
+                    const float tap = S->psf_tap[0];
+                    const float4 *row = conv + (padlo + k + dlo) * 9;
 #pragma unroll
-                for (int j = 0; j < 32; ++j) z[j] = 0.f;
-                for (int d = dlo; d <= dhi; ++d) {
+                    for (int j = 0; j < 8; ++j) {
+                        const float4 x = row[j];
+                        z[4 * j] = tap * x.x;
+                        z[4 * j + 1] = tap * x.y;
+                        z[4 * j + 2] = tap * x.z;
+                        z[4 * j + 3] = tap * x.w;
+                    }
+                }
+                for (int d = dlo + 1; d <= dhi; ++d) {
                     const float tap = S->psf_tap[d - dlo];
                     const float4 *row = conv + (padlo + k + d) * 9;
 #pragma unroll
@@ -981,8 +993,13 @@ __global__ __launch_bounds__(64) void k_synth(
                     y.w += tap * x.w;
                 }
             } else if (use_psf) {
-                y = make_float4(0.f, 0.f, 0.f, 0.f);
-                for (int d = dlo; d <= dhi; ++d) {
+                /* the first tap initialises the sums (dlo <= dhi) */
+                {
+                    const float tap = S->psf_tap[0];
+                    const float4 x = conv[padlo + k + dlo];
+                    y = make_float4(tap * x.x, tap * x.y, tap * x.z, tap * x.w);
+                }
+                for (int d = dlo + 1; d <= dhi; ++d) {
                     const float tap = S->psf_tap[d - dlo];
                     const float4 x = conv[padlo + k + d];
                     y.x += tap * x.x;
