@@ -24,9 +24,10 @@ METRIC = ("inverted pixels/s on 1 MI355X (1024², 30λ, 9 par, 50 it, PSF); "
 PUBLISHED_PX_S = 7400.0           # Tesla V100, 30 lambda, 9 par, <= 50 it, PSF
 PEAK_FP32_TFLOPS = 157.3          # MI355X FP32 (MFMA / packed VALU) peak
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per pixel of one k_rf launch (FETCH_SIZE + WRITE_SIZE, scaled by a
-# copy kernel of known size): profiles/k_rf_k_synth_hbm_counters_262144px.md
-K_RF_HBM_BYTES_PER_PIXEL = 905.0
+# HBM bytes per pixel evaluation of the LM loop's k_rf (FETCH_SIZE +
+# WRITE_SIZE summed over an inversion's launches, scaled by a copy kernel of
+# known size): profiles/k_rf_k_synth_hbm_counters_262144px.md
+K_RF_HBM_BYTES_PER_PIXEL = 1331.0
 
 
 def load_product():
@@ -162,10 +163,12 @@ def cpu_baseline(gm, setup_desc, obs_host, budget_s):
 
 
 def pmc_probe(gm, torch, dev, npix, seed):
-    """Counter calibration: a device-to-device copy of 256 MiB (its
-    FETCH_SIZE / WRITE_SIZE scale the counters to bytes), then one launch of
-    k_rf (normal equations at given models, all pixels) and one of k_synth
-    over npix pixels of the 30-lambda PSF set-up."""
+    """Counter calibration and probe: a device-to-device copy of 256 MiB (its
+    FETCH_SIZE / WRITE_SIZE scale the counters to bytes), then one inversion
+    of npix pixels of the 30-lambda PSF set-up (mixed precision, <= 50 LM
+    iterations), whose k_rf and k_synth launches are the ones the benchmark
+    times.  Prints the pixel evaluations of each kernel, by which the summed
+    counters of its launches are divided."""
     setup = gm.setup_single_line(psf_fwhm=60.0)
     ctx = gm.Context(setup)
     n_copy = 64 << 20
@@ -174,17 +177,16 @@ def pmc_probe(gm, torch, dev, npix, seed):
     truth = gm.draw_models(npix, seed=seed)
     obs = ctx.synth(truth)
     gm.add_noise(truth, obs, 1e-3, seed=seed + 1)
-    models = gm.draw_models(npix, seed=seed + 2)
     torch.cuda.synchronize()
     b.copy_(a)
     torch.cuda.synchronize()
-    ctx.normal_equations(models, obs)
-    torch.cuda.synchronize()
-    ctx.synth(models)
+    ctx.invert(obs)
+    st = ctx.stats()
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes": 4 * n_copy, "npix": npix,
-                      "k_rf_algorithmic_bytes": npix * (11 * 8 + 4 * 30 * 4 + 54 * 4 + 8),
-                      "k_synth_algorithmic_bytes": npix * (11 * 8 + 4 * 30 * 4)}), flush=True)
+                      "rf_evals": int(st["rf_evals"]), "synth_evals": int(st["synth_evals"]),
+                      "k_rf_algorithmic_bytes_per_eval": 20 * 4 + 4 * 30 * 4 + 54 * 4 + 8,
+                      "k_synth_algorithmic_bytes_per_eval": 20 * 4 + 4 * 30 * 4 + 8}), flush=True)
 
 
 # Serving admission budget: 4 GB of in-flight LM state (about 1-2 M pixels)
