@@ -1240,7 +1240,8 @@ __device__ __forceinline__ void jacobi_step_rows(T (&a)[45], T (&vr)[5][9])
 }
 
 template <typename T, int MAXSWEEP>
-__device__ __forceinline__ void solve_svd_pair(T (&a)[45], const T (&b)[9], int half, T (&x)[9])
+__device__ __forceinline__ void solve_svd_pair(T (&a)[45], const float *__restrict__ beta,
+                                               int half, T (&x)[9])
 {
     T vr[5][9];
 #pragma unroll
@@ -1271,6 +1272,11 @@ __device__ __forceinline__ void solve_svd_pair(T (&a)[45], const T (&b)[9], int 
             jacobi_step_rows<T, 8>(a, vr);
         }
     }
+    /* the right-hand side is read only now: held through the sweeps it
+     * pushes the FP64 version past 256 VGPRs into AGPR copies */
+    T b[9];
+#pragma unroll
+    for (int p = 0; p < 9; ++p) b[p] = (T)beta[p];
     /* V^T b: this lane's rows, then the partner's */
     T aux[9];
 #pragma unroll
@@ -1980,10 +1986,14 @@ __global__ __launch_bounds__(64) void k_solve_svd(const gm_dsetup *__restrict__ 
     for (int64_t i = ((int64_t)blockIdx.x * 64 + threadIdx.x) >> 1; i < cnt;
          i += ((int64_t)gridDim.x * 64) >> 1) {
         const int64_t pix = fb_list[i];
-        T a[45], b[9], x[9];
-        load_damped<T>(st.alpha + pix * st.ab_rec, st.beta + pix * st.ab_rec, 1,
-                       (T)st.flambda[pix], a, b);
-        solve_svd_pair<T, (sizeof(T) == 8 ? 12 : 10)>(a, b, half, x);
+        T a[45], x[9];
+        const float *al = st.alpha + pix * st.ab_rec;
+        const T damp = T(1) + (T)st.flambda[pix];
+#pragma unroll
+        for (int e = 0; e < 45; ++e) a[e] = (T)al[e];
+#pragma unroll
+        for (int p = 0; p < 9; ++p) a[ut(p, p)] = a[ut(p, p)] * damp;
+        solve_svd_pair<T, (sizeof(T) == 8 ? 12 : 10)>(a, st.beta + pix * st.ab_rec, half, x);
         if (half == 0) apply_to_trial<T>(S, st, pix, x);
     }
 }
