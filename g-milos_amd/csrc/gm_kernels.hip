@@ -1182,12 +1182,14 @@ __device__ __forceinline__ void solve_svd(T (&a)[45], const T (&b)[9], T (&x)[9]
     }
 }
 
-/* The same Jacobi SVD with V split over two lanes: both lanes rotate the
- * (identical) matrix, lane `half` keeps rows 5*half .. 5*half+4 of V (row 9
- * a zero dummy), so a lane holds 45 + 45 values instead of 45 + 81 and the
- * FP64 version stays in registers.  x is complete in the half-0 lane.      */
-template <typename T, int K>
-__device__ __forceinline__ void jacobi_step_rows(T (&a)[45], T (&vr)[5][9])
+/* The same Jacobi SVD with V split over a group of G lanes: every lane
+ * rotates the (identical) matrix, lane g keeps rows R*g .. R*g+R-1 of V
+ * (R = ceil(9/G), rows past 8 are zero dummies), so a lane holds 45 + 9R
+ * values instead of 45 + 81.  With G = 4 the FP64 version fits the 256
+ * architectural VGPRs and each lane updates 3 rows of V per rotation
+ * instead of 5.  x is complete in every lane of the group.                 */
+template <typename T, int K, int RR>
+__device__ __forceinline__ void jacobi_step_rows(T (&a)[45], T (&vr)[RR][9])
 {
     constexpr int P[4] = {(K + 1) % 9, (K + 2) % 9, (K + 3) % 9, (K + 4) % 9};
     constexpr int Q[4] = {(K + 8) % 9, (K + 7) % 9, (K + 6) % 9, (K + 5) % 9};
@@ -1230,7 +1232,7 @@ __device__ __forceinline__ void jacobi_step_rows(T (&a)[45], T (&vr)[5][9])
         a[ut(P[i], Q[i])] = T(0);
     }
 #pragma unroll
-    for (int r = 0; r < 5; ++r)
+    for (int r = 0; r < RR; ++r)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const T vp = vr[r][P[i]], vq = vr[r][Q[i]];
@@ -1239,15 +1241,16 @@ __device__ __forceinline__ void jacobi_step_rows(T (&a)[45], T (&vr)[5][9])
         }
 }
 
-template <typename T, int MAXSWEEP>
-__device__ __forceinline__ void solve_svd_pair(T (&a)[45], const float *__restrict__ beta,
-                                               int half, T (&x)[9])
+template <typename T, int MAXSWEEP, int G>
+__device__ __forceinline__ void solve_svd_group(T (&a)[45], const float *__restrict__ beta,
+                                                int g, T (&x)[9])
 {
-    T vr[5][9];
+    constexpr int RR = (9 + G - 1) / G;
+    T vr[RR][9];
 #pragma unroll
-    for (int r = 0; r < 5; ++r)
+    for (int r = 0; r < RR; ++r)
 #pragma unroll
-        for (int j = 0; j < 9; ++j) vr[r][j] = (half * 5 + r == j) ? T(1) : T(0);
+        for (int j = 0; j < 9; ++j) vr[r][j] = (g * RR + r == j) ? T(1) : T(0);
     const T eps2 = (sizeof(T) == 8) ? T(1e-30) : T(1e-14);
     bool done = false;
     for (int sweep = 0; sweep < MAXSWEEP; ++sweep) {
@@ -1261,51 +1264,52 @@ __device__ __forceinline__ void solve_svd_pair(T (&a)[45], const float *__restri
         done = done || (off <= eps2 * dg);
         if (__all(done)) break;
         if (!done) {
-            jacobi_step_rows<T, 0>(a, vr);
-            jacobi_step_rows<T, 1>(a, vr);
-            jacobi_step_rows<T, 2>(a, vr);
-            jacobi_step_rows<T, 3>(a, vr);
-            jacobi_step_rows<T, 4>(a, vr);
-            jacobi_step_rows<T, 5>(a, vr);
-            jacobi_step_rows<T, 6>(a, vr);
-            jacobi_step_rows<T, 7>(a, vr);
-            jacobi_step_rows<T, 8>(a, vr);
+            jacobi_step_rows<T, 0, RR>(a, vr);
+            jacobi_step_rows<T, 1, RR>(a, vr);
+            jacobi_step_rows<T, 2, RR>(a, vr);
+            jacobi_step_rows<T, 3, RR>(a, vr);
+            jacobi_step_rows<T, 4, RR>(a, vr);
+            jacobi_step_rows<T, 5, RR>(a, vr);
+            jacobi_step_rows<T, 6, RR>(a, vr);
+            jacobi_step_rows<T, 7, RR>(a, vr);
+            jacobi_step_rows<T, 8, RR>(a, vr);
         }
     }
     /* the right-hand side is read only now: held through the sweeps it
-     * pushes the FP64 version past 256 VGPRs into AGPR copies */
+     * would add 9 values to the register pressure of the loop */
     T b[9];
 #pragma unroll
     for (int p = 0; p < 9; ++p) b[p] = (T)beta[p];
-    /* V^T b: this lane's rows, then the partner's */
+    /* V^T b: this lane's rows, summed over the group */
     T aux[9];
 #pragma unroll
     for (int j = 0; j < 9; ++j) {
         T sum = 0;
 #pragma unroll
-        for (int r = 0; r < 5; ++r) {
-            const int row = half * 5 + r;
+        for (int r = 0; r < RR; ++r) {
+            const int row = g * RR + r;
             T br = b[0];
 #pragma unroll
             for (int k = 1; k < 9; ++k) br = (row == k) ? b[k] : br;
             sum += (row < 9) ? vr[r][j] * br : T(0);
         }
-        sum += __shfl_xor(sum, 1, 64);
+#pragma unroll
+        for (int m = 1; m < G; m <<= 1) sum += __shfl_xor(sum, m, 64);
         const T w = fabs(a[ut(j, j)]);
         aux[j] = (w > T(1e-12)) ? sum / w : T(0);
     }
-    T xr[5];
+    T xr[RR];
 #pragma unroll
-    for (int r = 0; r < 5; ++r) {
+    for (int r = 0; r < RR; ++r) {
         T sum = 0;
 #pragma unroll
         for (int k = 0; k < 9; ++k) sum += vr[r][k] * aux[k];
         xr[r] = sum;
     }
+    /* x_i lives in lane i / RR of the group, row i % RR */
+    const int base = (int)(threadIdx.x & 63) & ~(G - 1);
 #pragma unroll
-    for (int r = 0; r < 5; ++r) x[r] = xr[r];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) x[5 + r] = __shfl_xor(xr[r], 1, 64);
+    for (int i = 0; i < 9; ++i) x[i] = __shfl(xr[i % RR], base + i / RR, 64);
 }
 
 /* LDL^T solve of the damped matrix (symmetric positive semi-definite by
@@ -1982,9 +1986,10 @@ __global__ __launch_bounds__(64) void k_solve_svd(const gm_dsetup *__restrict__ 
     const int64_t cnt = *fb_count;
     if (hist && blockIdx.x == 0 && threadIdx.x == 0) hist[1] = (int32_t)cnt;
     const int64_t cap = st.cap;
-    const int half = threadIdx.x & 1;
-    for (int64_t i = ((int64_t)blockIdx.x * 64 + threadIdx.x) >> 1; i < cnt;
-         i += ((int64_t)gridDim.x * 64) >> 1) {
+    constexpr int G = 4;                  /* lanes per pixel */
+    const int g = threadIdx.x & (G - 1);
+    for (int64_t i = ((int64_t)blockIdx.x * 64 + threadIdx.x) / G; i < cnt;
+         i += ((int64_t)gridDim.x * 64) / G) {
         const int64_t pix = fb_list[i];
         T a[45], x[9];
         const float *al = st.alpha + pix * st.ab_rec;
@@ -1993,8 +1998,8 @@ __global__ __launch_bounds__(64) void k_solve_svd(const gm_dsetup *__restrict__ 
         for (int e = 0; e < 45; ++e) a[e] = (T)al[e];
 #pragma unroll
         for (int p = 0; p < 9; ++p) a[ut(p, p)] = a[ut(p, p)] * damp;
-        solve_svd_pair<T, (sizeof(T) == 8 ? 12 : 10)>(a, st.beta + pix * st.ab_rec, half, x);
-        if (half == 0) apply_to_trial<T>(S, st, pix, x);
+        solve_svd_group<T, (sizeof(T) == 8 ? 12 : 10), G>(a, st.beta + pix * st.ab_rec, g, x);
+        if (g == 0) apply_to_trial<T>(S, st, pix, x);
     }
 }
 
@@ -2568,7 +2573,7 @@ void gm_launch_solve(const gm_dsetup *dS, gm_state st, const int32_t *list,
     }
 #undef GM_SOLVE_CASE
     if (solver == GM_SOLVE_SVD || solver == GM_SOLVE_JACOBI) {
-        const int gf = grid_for(grid_items, 32, 1024);
+        const int gf = grid_for(grid_items, 16, 1024);   /* 16 pixels of 4 lanes */
         if (mixed)
             hipLaunchKernelGGL((k_solve_svd<true>), dim3(gf), dim3(64), 0, s, dS, st, fb_list, fb_count,
                                hist);
