@@ -1975,7 +1975,16 @@ __global__ __launch_bounds__(64) void k_solve(const gm_dsetup *__restrict__ S,
     }
 }
 
-/* Jacobi SVD for the pixels the LDL^T test could not certify. */
+/* Jacobi SVD for the pixels the LDL^T test could not certify, a group of
+ * kSvdLanes lanes per pixel.  Few pixels fall back per round, so the launch
+ * lasts as long as one group's instruction stream: every lane repeats the
+ * rotations of the matrix, but each keeps fewer rows of V the more lanes
+ * share a pixel.  Per 1M-pixel inversion (solve, both tiers): 8.3 ms with
+ * 2 lanes, 8.1 with 4, 7.9 with 8, 7.75 with 16. */
+#ifndef GM_SVD_LANES
+#define GM_SVD_LANES 16
+#endif
+constexpr int kSvdLanes = GM_SVD_LANES;
 template <bool MIXED>
 __global__ __launch_bounds__(64) void k_solve_svd(const gm_dsetup *__restrict__ S, gm_state st,
                                                   const int32_t *__restrict__ fb_list,
@@ -1986,7 +1995,7 @@ __global__ __launch_bounds__(64) void k_solve_svd(const gm_dsetup *__restrict__ 
     const int64_t cnt = *fb_count;
     if (hist && blockIdx.x == 0 && threadIdx.x == 0) hist[1] = (int32_t)cnt;
     const int64_t cap = st.cap;
-    constexpr int G = 4;                  /* lanes per pixel */
+    constexpr int G = kSvdLanes;
     const int g = threadIdx.x & (G - 1);
     for (int64_t i = ((int64_t)blockIdx.x * 64 + threadIdx.x) / G; i < cnt;
          i += ((int64_t)gridDim.x * 64) / G) {
@@ -2573,7 +2582,7 @@ void gm_launch_solve(const gm_dsetup *dS, gm_state st, const int32_t *list,
     }
 #undef GM_SOLVE_CASE
     if (solver == GM_SOLVE_SVD || solver == GM_SOLVE_JACOBI) {
-        const int gf = grid_for(grid_items, 16, 1024);   /* 16 pixels of 4 lanes */
+        const int gf = grid_for(grid_items, 64 / kSvdLanes, 1024);
         if (mixed)
             hipLaunchKernelGGL((k_solve_svd<true>), dim3(gf), dim3(64), 0, s, dS, st, fb_list, fb_count,
                                hist);
