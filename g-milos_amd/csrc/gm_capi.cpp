@@ -244,6 +244,15 @@ extern "C" gm_status gm_ctx_create(const gm_setup *setup, gm_ctx **out)
     if (s != GM_OK) { delete c; return s; }
     c->lc = gm_lane_config(c->host_ds);
     if (hipGetDevice(&c->device) != hipSuccess) { delete c; return GM_ERR_HIP; }
+    /* a wide PSF table applied with every tap at 129-256 wavelengths stages
+     * more rows per pixel than a workgroup's LDS holds */
+    int lds_max = 0;
+    if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) !=
+        hipSuccess) {
+        delete c;
+        return GM_ERR_HIP;
+    }
+    if (gm_lds_bytes(c->lc) > (size_t)lds_max) { delete c; return GM_ERR_UNSUPPORTED; }
     if (hipMalloc(&c->dS, sizeof(gm_dsetup)) != hipSuccess) { delete c; return GM_ERR_NOMEM; }
     if (hipMemcpy(c->dS, &c->host_ds, sizeof(gm_dsetup), hipMemcpyHostToDevice) != hipSuccess) {
         hipFree(c->dS);

@@ -116,6 +116,8 @@ struct gm_lanecfg {
 };
 
 gm_lanecfg gm_lane_config(const gm_dsetup &h);
+/* Dynamic LDS per workgroup of the wavelength-parallel kernels. */
+size_t gm_lds_bytes(const gm_lanecfg &lc);
 
 /* Launchers (gm_kernels.hip).  `list` may be NULL (identity) with the item
  * count given by *count (device) or by n_direct when count is NULL.  `pre`,

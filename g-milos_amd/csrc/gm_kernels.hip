@@ -2408,6 +2408,24 @@ inline int grid_for(int64_t items, int per_block, int cap_blocks)
  * one workgroup per pixel pair (262 144). */
 constexpr int kMaxBlocks = 32768;
 
+/* Dynamic LDS of k_rf and k_synth: the staged rows of each pixel of the
+ * workgroup (or the normal-equation reduction, whichever is larger), plus
+ * the macroturbulence kernel rows of the eleven-slot variants.            */
+size_t rf_lds_bytes(const gm_lanecfg &lc)
+{
+    const int xm = lc.np == 11 ? 2 : (lc.ext ? 1 : 0);
+    const int red4 = (lc.lpp / 2) * (xm == 2 ? 20 : 15);
+    const int region4 = (lc.rows * 9 > red4) ? lc.rows * 9 : red4;
+    return (size_t)lc.ppw * region4 * sizeof(float4) +
+           (size_t)lc.ppw * (xm == 2 ? 4 : (xm == 1 ? 2 : 0)) * lc.lpp * lc.nlpl * sizeof(float);
+}
+
+size_t synth_lds_bytes(const gm_lanecfg &lc)
+{
+    return (size_t)lc.ppw * lc.rows * sizeof(float4) +
+           (lc.ext ? (size_t)lc.ppw * 2 * lc.lpp * lc.nlpl * sizeof(float) : 0);
+}
+
 template <int LPP, int NLPL>
 void launch_rf_t(const gm_lanecfg &lc, const gm_dsetup *dS, const double *models,
                  const float *pre, int64_t mfs, int64_t mps, const float *obs, const int32_t *list,
@@ -2417,10 +2435,7 @@ void launch_rf_t(const gm_lanecfg &lc, const gm_dsetup *dS, const double *models
                  double *model_dst, float *pre_dst)
 {
     const int xm = lc.np == 11 ? 2 : (lc.ext ? 1 : 0);
-    const int red4 = (LPP / 2) * (xm == 2 ? 20 : 15);
-    const int region4 = (lc.rows * 9 > red4) ? lc.rows * 9 : red4;
-    const size_t shm = (size_t)lc.ppw * region4 * sizeof(float4) +
-                       (size_t)lc.ppw * (xm == 2 ? 4 : (xm == 1 ? 2 : 0)) * LPP * NLPL * sizeof(float);
+    const size_t shm = rf_lds_bytes(lc);
     const int g = grid_for(grid_items, lc.ppw, kMaxBlocks);
     /* the LM loop's calls take the specialised kernel (fixed record strides) */
     const int64_t abrec = xm == 2 ? GM_AB_REC11 : GM_AB_REC9;
@@ -2458,8 +2473,7 @@ void launch_synth_t(const gm_lanecfg &lc, const gm_dsetup *dS, const double *mod
                     const int32_t *count, int64_t n_direct, int64_t grid_items, double *chi,
                     float *synth, int mixed, hipStream_t s)
 {
-    const size_t shm = (size_t)lc.ppw * lc.rows * sizeof(float4) +
-                       (lc.ext ? (size_t)lc.ppw * 2 * LPP * NLPL * sizeof(float) : 0);
+    const size_t shm = synth_lds_bytes(lc);
     const int g = grid_for(grid_items, lc.ppw, kMaxBlocks);
     const bool rec = mfs == 1 && mps == GM_MODEL_REC && pre && list && count && obs && chi;
 #define GM_SYNTH_CASE(MX, EX)                                                                     \
@@ -2484,6 +2498,12 @@ void launch_synth_t(const gm_lanecfg &lc, const gm_dsetup *dS, const double *mod
 }
 
 } // namespace
+
+size_t gm_lds_bytes(const gm_lanecfg &lc)
+{
+    const size_t a = rf_lds_bytes(lc), b = synth_lds_bytes(lc);
+    return a > b ? a : b;
+}
 
 /* Model addressing: the LM state is SoA [11][cap] (field stride cap, pixel
  * stride 1); caller arrays are AoS [npix][11] (field stride 1, pixel 11).   */

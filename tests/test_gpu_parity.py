@@ -4,13 +4,20 @@ the FP64 oracle on the same seeded inputs.
 Tolerances (the HIP path computes synthesis, response functions, PSF and
 normal equations in FP32):
   synthesis          |d| <= 5e-6 * Ic + 2e-5 * |x|
-  response functions |d| <= 5e-4 * max|RF vector| + 2e-7 Ic / dp, dp = 1% of the
-                     parameter range (per Stokes and parameter)
+  response functions |d| <= 1.25e-4 * max|RF vector| + 5e-8 Ic / dp, dp = 1% of
+                     the parameter range (per Stokes and parameter): about
+                     10x the worst case measured over every set-up here and
+                     every wavelength count from 31 to 256
   normal equations   |d alpha| <= 1e-3 * max|alpha|, |d beta| <= 2e-3 * max|beta|
   step solve         FP64: rel 1e-6; FP32: rel 2e-2 (of |x|)
   estimates (FP64)   rel 1e-6 (FP32 intermediates as the reference)
-  inversion          medians of |d| vs the oracle's inversion: B <= 1 G,
-                     gamma, phi <= 0.1 deg, vlos <= 1 m/s
+  inversion          per-parameter |d| against the oracle's inversion of the
+                     same pixels: medians, 99th percentiles, and the fraction
+                     of pixels inside the bar |dB| <= 1 G, |dvlos| <= 1 m/s,
+                     |dgamma|, |dphi| <= 0.1 deg (>= 99 %); the FP32 LM can
+                     end in another local minimum than the FP64 one on a few
+                     near-degenerate pixels, so the tails are bounded by the
+                     fraction, the bulk by medians near the achieved ones
 """
 import os
 import sys
@@ -97,7 +104,8 @@ def test_response_functions_match_oracle(gm, orc, cuda, kind, psfname, rules):
     scale = np.abs(rf_ref).max(axis=-1, keepdims=True)
     ic = (models[:, 7] + models[:, 8])[:, None, None, None]
     step = np.array(DP_TYP + (1.0,))[None, None, :, None]
-    tol = np.broadcast_to(5e-4 * scale + 2e-7 * ic / step, rf.shape)
+    tol = np.broadcast_to(1.25e-4 * scale + 5e-8 * ic / step, rf.shape)
+    print("RF worst |d|/tol %.3g" % (np.abs(rf - rf_ref) / tol).max(), flush=True)
     bad = np.abs(rf - rf_ref) > tol
     if bad.any():
         for i in np.argwhere(bad)[:8]:
@@ -243,11 +251,104 @@ def _inversion_errors(gm, orc, cuda, kind, psf, npix, seed, **kw):
     return got, ref, truth, obs, ctx
 
 
+# |d| bars of the north star: B [G], vlos [km/s], gamma, phi [deg]
+BAR = {1: 1.0, 2: 1e-3, 5: 0.1, 6: 0.1}
+# medians near the achieved ones (~3e-4 G, ~2e-4 m/s, ~2e-5 deg, ~3e-5 deg)
+MED = {1: 0.01, 2: 1e-5, 5: 1e-3, 6: 1e-3}
+# 99th percentiles on well-posed pixels, ~10x the achieved ones (0.02 G,
+# 7e-3 m/s, 3e-3 deg, 1.2e-2 deg)
+P99 = {1: 0.25, 2: 1e-4, 5: 0.03, 6: 0.1}
+NAMES = {1: "B", 2: "vlos", 5: "gamma", 6: "phi"}
+
+
+def within_bar(d):
+    ok = np.ones(len(d), dtype=bool)
+    for f, b in BAR.items():
+        ok &= d[:, f] <= b
+    return ok
+
+
+def well_posed(truth):
+    """Pixels whose B, gamma, phi the spectrum determines: a field above
+    100 G that is not within 10 deg of the line of sight (there the azimuth
+    is undetermined and the LM path decides which of a flat valley's points
+    it returns)."""
+    return (truth[:, 1] > 100.0) & (truth[:, 5] > 10.0) & (truth[:, 5] < 170.0)
+
+
+def inversion_stats(d, truth=None):
+    """Median, 99th percentile and within-bar fraction of |d| [npix, 11];
+    with the true models, the same over the well-posed pixels."""
+    st = {"median": np.median(d, axis=0), "p99": np.percentile(d, 99, axis=0),
+          "within": float(within_bar(d).mean()), "n": len(d)}
+    if truth is not None:
+        wp = well_posed(truth)
+        st["n_wp"] = int(wp.sum())
+        st["within_wp"] = float(within_bar(d[wp]).mean())
+        st["p99_wp"] = np.percentile(d[wp], 99, axis=0)
+    return st
+
+
+def fp64_chisqr(orc, o, obs, models):
+    """chi^2 of each model against its observation, synthesised in FP64."""
+    sp = orc.synth(o, models)
+    return np.array([orc.chisqr(o, obs[i], sp[i]) for i in range(len(models))])
+
+
+def assert_inversion(d, label="", truth=None, fit=None, med=MED, p99=P99, frac=0.97,
+                     frac_wp=0.99):
+    """Bounds the per-pixel differences to the oracle's inversion of the
+    same pixels.
+
+    * medians of |dB|, |dvlos|, |dgamma|, |dphi| below `med`;
+    * with the true models: on the well-posed pixels at least `frac_wp`
+      inside the bar and the 99th percentiles below `p99`; over all pixels at
+      least `frac` inside the bar.  The FP64 oracle itself, given the same
+      observations perturbed by 1e-6 relative (FP32 rounding), leaves 98.5 -
+      99.3 % of all pixels and 99.86 - 100 % of the well-posed ones inside
+      the bar: near-longitudinal and weak fields, and pixels still moving at
+      the iteration cap, are where LM paths part;
+    * fit = (chi2 of the GPU models, chi2 of the oracle models), both in
+      FP64: where the two part, the GPU's fit is as good -- the median
+      relative difference below 1e-5 and at most 1 % of the pixels worse by
+      more than 1e-3."""
+    st = inversion_stats(d, truth)
+    msg = "%s n %d within-bar %.4f " % (label, st["n"], st["within"])
+    if truth is not None:
+        msg += "(well-posed %d: %.4f, p99 %s) " % (
+            st["n_wp"], st["within_wp"],
+            " ".join("%s %.3g" % (NAMES[f], st["p99_wp"][f]) for f in BAR))
+    msg += " ".join("%s med %.3g p99 %.3g" % (NAMES[f], st["median"][f], st["p99"][f])
+                    for f in BAR)
+    if fit is not None:
+        rel = (fit[0] - fit[1]) / fit[1]
+        msg += " | chi2 fp64 rel med %.2g worse>1e-3 %.4f" % (np.median(np.abs(rel)),
+                                                              (rel > 1e-3).mean())
+    print(msg, flush=True)
+    for f in BAR:
+        assert st["median"][f] <= med[f], (label, NAMES[f], "median", st["median"][f])
+    if truth is not None:
+        assert st["within"] >= frac, (label, "within-bar fraction", st["within"])
+        assert st["within_wp"] >= frac_wp, (label, "well-posed within-bar", st["within_wp"])
+        for f in BAR:
+            assert st["p99_wp"][f] <= p99[f], (label, NAMES[f], "p99", st["p99_wp"][f])
+    if fit is not None:
+        assert np.median(np.abs(rel)) <= 1e-5, (label, "chi2 median", np.median(np.abs(rel)))
+        assert (rel > 1e-3).mean() <= 0.01, (label, "chi2 worse fraction", (rel > 1e-3).mean())
+    return st
+
+
+def check_inversion(orc, o, obs, got, ref, truth, label, **kw):
+    """assert_inversion with the FP64 fit quality of both results."""
+    fit = (fp64_chisqr(orc, o, obs, got["model"]), fp64_chisqr(orc, o, obs, ref["model"]))
+    return assert_inversion(param_errors(got["model"], ref["model"]), label, truth=truth,
+                            fit=fit, **kw)
+
+
 def _assert_medians(d, label=""):
-    med = np.median(d, axis=0)
-    print(label, "median |d|: B %.3g G, vlos %.3g m/s, gamma %.3g deg, phi %.3g deg" %
-          (med[1], med[2] * 1e3, med[5], med[6]))
-    assert med[1] <= 1.0 and med[2] <= 1e-3 and med[5] <= 0.1 and med[6] <= 0.1, med
+    """Medians inside the north-star bar (set-ups whose measured spread is
+    not yet recorded)."""
+    return assert_inversion(d, label, med=BAR)
 
 
 @pytest.mark.parametrize("precision,solver", [(1, 0), (0, 0), (1, 1), (1, 2), (1, 3), (0, 3)])
@@ -256,8 +357,8 @@ def test_inversion_config0_matches_oracle(gm, orc, cuda, precision, solver):
     solver 3 (the reference's CORDIC solve) against the oracle's CORDIC LM."""
     got, ref, truth, obs, ctx = _inversion_errors(gm, orc, cuda, "single", None, 4096, 100,
                                                   precision=precision, solver=solver)
-    d = param_errors(got["model"], ref["model"])
-    _assert_medians(d, f"prec {precision} solver {solver}")
+    o = pair(gm, orc, "single", None, precision=precision, solver=solver)[1]
+    check_inversion(orc, o, obs, got, ref, truth, f"prec {precision} solver {solver}")
     assert np.all(np.isfinite(got["model"])) and np.all(got["iters"] >= 1)
     assert np.all(got["iters"] <= 50)
     rel_c2 = np.abs(got["chisqr"] - ref["chisqr"]) / ref["chisqr"]
@@ -266,11 +367,20 @@ def test_inversion_config0_matches_oracle(gm, orc, cuda, precision, solver):
     assert st["npix"] == 4096 and st["lm_rounds"] >= 1
 
 
-def test_inversion_config1_psf_matches_oracle(gm, orc, cuda):
-    """Config-1 line, grid and PSF (Gaussian FWHM 60 mA) on 2048 pixels."""
+@pytest.mark.parametrize("precision,tail", [(1, 1e-9), (0, 1e-9), (0, 0.0)])
+def test_inversion_config1_psf_matches_oracle(gm, orc, cuda, precision, tail):
+    """Config-1 line, grid and PSF (Gaussian FWHM 60 mA) on 2048 pixels, in
+    mixed precision and in config 1's own plain FP32; psf_tail_tol 0 applies
+    all 30 taps as the reference does (1e-9 keeps 9)."""
     got, ref, truth, obs, ctx = _inversion_errors(gm, orc, cuda, "single", ("gauss", 60.0),
-                                                  2048, 200)
-    _assert_medians(param_errors(got["model"], ref["model"]), "cfg1")
+                                                  2048, 200, precision=precision,
+                                                  psf_tail_tol=tail)
+    assert ctx.stats()["psf_taps"] == (30 if tail == 0.0 else 9)
+    o = pair(gm, orc, "single", ("gauss", 60.0))[1]
+    check_inversion(orc, o, obs, got, ref, truth, f"cfg1 prec {precision} "
+                    f"taps {ctx.stats()['psf_taps']}")
+    rel_c2 = np.abs(got["chisqr"] - ref["chisqr"]) / ref["chisqr"]
+    assert np.median(rel_c2) < 1e-4
     # synthetic output is the last evaluated model's spectrum: the best model
     # or a rejected trial a vanishing step away (lambda > 1e7 ends most runs)
     assert np.all(np.isfinite(got["synth"]))
@@ -287,7 +397,9 @@ def test_inversion_two_lines_matches_oracle(gm, orc, cuda):
     high-noise pixels, on 512 pixels."""
     got, ref, truth, obs, ctx = _inversion_errors(gm, orc, cuda, "two", ("gauss", 60.0), 512,
                                                   300)
-    _assert_medians(param_errors(got["model"], ref["model"]), "cfg2")
+    o = pair(gm, orc, "two", ("gauss", 60.0))[1]
+    # 10 % weak-field pixels (B < 100 G: gamma and phi barely determined)
+    check_inversion(orc, o, obs, got, ref, truth, "cfg2", frac=0.93)
 
 
 @pytest.mark.parametrize("solver", [0, 3])
@@ -339,4 +451,9 @@ def test_fixed_parameters_stay_fixed(gm, orc, cuda):
             continue    # vlos is set by the classical estimates
         np.testing.assert_array_equal(m[:, f], p.init_model[f])
     ref = orc.invert(o, obs)
-    _assert_medians(param_errors(m, ref["model"]), "3 free")
+    # the fixed parameters sit at the guess, not at the truth, so no model
+    # fits and the chi^2 valley is flat and multimodal in B, gamma, phi: the
+    # two paths may stop at different points of it, but the fits are equal
+    fit = (fp64_chisqr(orc, o, obs, m), fp64_chisqr(orc, o, obs, ref["model"]))
+    assert_inversion(param_errors(m, ref["model"]), "3 free", fit=fit,
+                     med={1: 0.05, 2: 1e-5, 5: 0.01, 6: 0.05})
