@@ -36,7 +36,8 @@ GM_INIT_CE, GM_INIT_MODEL, GM_INIT_PER_PIXEL = 0, 1, 2
 MODEL_FIELDS = ("eta0", "B", "vlos", "dopp", "aa", "gm", "az", "S0", "S1", "mac", "alfa")
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "HIP runtime error", 3: "out of device memory",
-          4: "unsupported configuration", 5: "not found"}
+          4: "unsupported configuration", 5: "not found", 6: "cancelled"}
+GM_OK, GM_ERR_NOT_FOUND, GM_CANCELLED = 0, 5, 6
 
 
 class GmError(RuntimeError):
@@ -90,7 +91,8 @@ class gm_stats(ctypes.Structure):
 
 class gm_server_config(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("pool_pixels", ctypes.c_int64),
-                ("max_tiles", ctypes.c_int32), ("device", ctypes.c_int32)]
+                ("max_tiles", ctypes.c_int32), ("device", ctypes.c_int32),
+                ("aging_ms", ctypes.c_int32), ("scan", ctypes.c_int32)]
 
 
 class gm_tile_result(ctypes.Structure):
@@ -130,6 +132,7 @@ EXPORTED = ("gm_ctx_create", "gm_ctx_destroy", "gm_status_string", "gm_ctx_set_t
             "gm_mlp_create", "gm_mlp_destroy", "gm_mlp_num_params", "gm_mlp_get_params",
             "gm_mlp_set_params", "gm_mlp_train_step", "gm_mlp_gradients", "gm_mlp_predict",
             "gm_server_create", "gm_server_add_setup", "gm_server_submit", "gm_server_poll",
+            "gm_server_cancel",
             "gm_server_get_stats", "gm_server_clock_ms", "gm_server_destroy",
             "gm_prior_default", "gm_draw_models", "gm_add_noise", "gm_mlp_fit",
             "gm_invert_guided", "gm_ctx_set_stray_map")
@@ -186,6 +189,7 @@ def load_library(path=LIB_PATH):
         "gm_server_add_setup": ([vp, P(gm_setup), P(i32)], i32),
         "gm_server_submit": ([vp, i32, vp, i64, vp, P(gm_out), P(ctypes.c_uint64)], i32),
         "gm_server_poll": ([vp, P(gm_tile_result), i32, P(i32), i32], i32),
+        "gm_server_cancel": ([vp, ctypes.c_uint64], i32),
         "gm_server_get_stats": ([vp, P(gm_server_stats)], i32),
         "gm_server_clock_ms": ([vp], ctypes.c_double),
         "gm_server_destroy": ([vp], i32),
@@ -680,9 +684,11 @@ class Server:
     tile's device buffers alive until its result is polled; this wrapper holds
     references to the tensors for that purpose."""
 
-    def __init__(self, hbm_budget_bytes=0, pool_pixels=0, max_tiles=0, device=0):
+    def __init__(self, hbm_budget_bytes=0, pool_pixels=0, max_tiles=0, device=0, aging_ms=0,
+                 scan=0):
         self.lib = load_library()
-        cfg = gm_server_config(int(hbm_budget_bytes), int(pool_pixels), int(max_tiles), int(device))
+        cfg = gm_server_config(int(hbm_budget_bytes), int(pool_pixels), int(max_tiles), int(device),
+                               int(aging_ms), int(scan))
         h = ctypes.c_void_p()
         _check(self.lib.gm_server_create(ctypes.byref(cfg), ctypes.byref(h)), "gm_server_create")
         self.h = h
@@ -736,6 +742,16 @@ class Server:
                         "latency_ms": r.t_done_ms - r.t_submit_ms,
                         "queue_ms": r.t_admit_ms - r.t_submit_ms, "out": kept[2]})
         return res
+
+    def cancel(self, ticket):
+        """True when the tile was queued or in flight (its result then comes
+        back through poll with status GM_CANCELLED), False when it is unknown
+        or already complete."""
+        code = self.lib.gm_server_cancel(self.h, ctypes.c_uint64(ticket))
+        if code == GM_ERR_NOT_FOUND:
+            return False
+        _check(code, "gm_server_cancel")
+        return True
 
     def clock_ms(self):
         return self.lib.gm_server_clock_ms(self.h)

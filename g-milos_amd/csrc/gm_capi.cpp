@@ -76,6 +76,7 @@ extern "C" const char *gm_status_string(gm_status st)
     case GM_ERR_NOMEM: return "out of device memory";
     case GM_ERR_UNSUPPORTED: return "unsupported configuration";
     case GM_ERR_NOT_FOUND: return "not found";
+    case GM_CANCELLED: return "cancelled";
     }
     return "unknown status";
 }

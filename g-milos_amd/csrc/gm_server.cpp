@@ -8,11 +8,18 @@
  *
  *   retire   observe finished rounds (event query), hand completed tiles to
  *            the results queue, free their slot ranges and HBM footprint;
- *   admit    take queued tiles in arrival order while the summed footprint of
- *            in-flight tiles stays within the budget and the engine has a
- *            free contiguous slot range: copy the observations into the pool,
- *            set the start state, evaluate the first normal equations and
- *            append the slots to the engine's active list;
+ *   cancel   drop cancelled tiles from the queue; make cancelled tiles in
+ *            flight leave the LM loop at the next round;
+ *   admit    examine queued tiles in arrival order and admit each one that
+ *            fits -- the summed footprint of in-flight tiles stays within the
+ *            budget and its engine has a free contiguous slot range -- so a
+ *            tile that does not fit yet (a large one, or one whose engine is
+ *            full) does not hold back later tiles of other sizes and setups;
+ *            once the oldest waiting tile has waited aging_ms, nothing passes
+ *            it and it enters as soon as room frees.  Admission copies the
+ *            observations into the pool, sets the start state, evaluates the
+ *            first normal equations and appends the slots to the engine's
+ *            active list;
  *   round    one LM round over the engine's whole active list (solve, synth
  *            + chi^2, accept/reject, normal equations of accepted pixels),
  *            then copy pixels that left the loop into their tiles' outputs;
@@ -53,6 +60,7 @@ struct Pending {
 
 struct TileSlot {
     bool used = false;
+    bool cancelled = false;
     uint64_t ticket = 0;
     int64_t slot0 = 0, npix = 0, bytes = 0;
     double t_submit = 0, t_admit = 0;
@@ -127,6 +135,7 @@ struct gm_server {
     std::condition_variable cv_work, cv_done;
     std::deque<Pending> pending;
     std::deque<gm_tile_result> results;
+    std::vector<uint64_t> cancel_req;   /* in-flight tickets to cancel */
     uint64_t next_ticket = 1;
     bool stop = false;
     gm_status error = GM_OK;
@@ -372,7 +381,8 @@ static void worker_main(gm_server *srv)
                     const int tid = c[1 + k];
                     if (tid < 0 || tid >= e.max_tiles || !e.tiles[tid].used) continue;
                     TileSlot &ts = e.tiles[tid];
-                    gm_tile_result r{ts.ticket, (int32_t)ei, GM_OK, ts.npix, ts.t_submit, ts.t_admit, t};
+                    gm_tile_result r{ts.ticket, (int32_t)ei, ts.cancelled ? GM_CANCELLED : GM_OK,
+                                     ts.npix, ts.t_submit, ts.t_admit, t};
                     srv->results.push_back(r);
                     srv->inflight_bytes -= ts.bytes;
                     srv->inflight_tiles -= 1;
@@ -382,6 +392,7 @@ static void worker_main(gm_server *srv)
                     e.inflight_px -= ts.npix;
                     e.ntiles -= 1;
                     ts.used = false;
+                    ts.cancelled = false;
                     e.free_tiles.push_back(tid);
                 }
                 if (n) srv->cv_done.notify_all();
@@ -390,7 +401,31 @@ static void worker_main(gm_server *srv)
                 progressed = true;
             }
         }
-        /* admit, in arrival order; the head of the queue blocks the rest */
+        /* cancel: in-flight tiles leave the LM loop at their next decision
+         * (iteration count past the cap); stream order puts this after the
+         * rounds already queued and before any reuse of the slots */
+        {
+            std::vector<std::pair<Engine *, TileSlot>> todo;
+            {
+                std::lock_guard<std::mutex> lk(srv->mu);
+                for (uint64_t tk : srv->cancel_req)
+                    for (Engine *ep : engines)
+                        for (TileSlot &ts : ep->tiles)
+                            if (ts.used && ts.ticket == tk && !ts.cancelled) {
+                                ts.cancelled = true;
+                                todo.push_back({ep, ts});
+                            }
+                srv->cancel_req.clear();
+            }
+            for (auto &c : todo) {
+                gm_launch_fill_i32(c.first->st.iter + c.second.slot0, c.second.npix,
+                                   0x3fffffff, c.first->stream);
+                progressed = true;
+            }
+        }
+        /* admit: every queued tile that fits, in arrival order, within the
+         * first `scan` of the queue; a tile that does not fit is passed by
+         * later ones until it has waited aging_ms */
         for (;;) {
             Pending pd;
             int t = -1;
@@ -399,26 +434,40 @@ static void worker_main(gm_server *srv)
             {
                 std::lock_guard<std::mutex> lk(srv->mu);
                 if (srv->pending.empty()) break;
-                pd = srv->pending.front();
+                const double tnow = now_ms(srv);
+                const int scan = srv->cfg.scan;
+                size_t pick = srv->pending.size();
+                for (size_t q = 0; q < srv->pending.size() && (int)q < scan; ++q) {
+                    const Pending &c = srv->pending[q];
+                    Engine &e = *srv->engines[c.setup];
+                    const int64_t bytes = c.npix * e.bytes_per_px;
+                    const bool fits = (srv->inflight_bytes == 0 ||
+                                       srv->inflight_bytes + bytes <= srv->cfg.hbm_budget_bytes) &&
+                                      !e.free_tiles.empty() && alloc_range(e, c.npix, slot0);
+                    if (fits) { pick = q; break; }
+                    /* strict order, or the oldest tile has waited long enough:
+                     * nothing passes it */
+                    if (srv->cfg.aging_ms < 0 || tnow - c.t_submit >= srv->cfg.aging_ms) break;
+                }
+                if (pick == srv->pending.size()) break;
+                pd = srv->pending[pick];
                 Engine &e = *srv->engines[pd.setup];
                 const int64_t bytes = pd.npix * e.bytes_per_px;
-                if (srv->inflight_bytes > 0 && srv->inflight_bytes + bytes > srv->cfg.hbm_budget_bytes) break;
-                if (e.free_tiles.empty()) break;
-                if (!alloc_range(e, pd.npix, slot0)) break;
                 t = e.free_tiles.back();
                 e.free_tiles.pop_back();
                 TileSlot &ts = e.tiles[t];
                 ts.used = true;
+                ts.cancelled = false;
                 ts.ticket = pd.ticket;
                 ts.slot0 = slot0;
                 ts.npix = pd.npix;
                 ts.bytes = bytes;
                 ts.t_submit = pd.t_submit;
-                ts.t_admit = now_ms(srv);
+                ts.t_admit = tnow;
                 srv->inflight_bytes += bytes;
                 srv->peak_bytes = std::max(srv->peak_bytes, srv->inflight_bytes);
                 srv->inflight_tiles += 1;
-                srv->pending.pop_front();
+                srv->pending.erase(srv->pending.begin() + (long)pick);
                 ep = &e;
             }
             admit(*ep, pd, t, slot0);
@@ -446,18 +495,11 @@ static void worker_main(gm_server *srv)
             std::this_thread::sleep_for(std::chrono::microseconds(20));
             continue;
         }
+        /* idle: a queued tile always fits an idle engine (submit bounds its
+         * size by the pool), so the next pass admits whatever arrived */
         std::unique_lock<std::mutex> lk(srv->mu);
         if (srv->stop && srv->pending.empty() && srv->inflight_tiles == 0) return;
-        if (!srv->pending.empty() && srv->inflight_tiles == 0 && !srv->stop) {
-            /* head tile cannot be admitted into an idle engine: too large */
-            const Pending pd = srv->pending.front();
-            srv->pending.pop_front();
-            gm_tile_result r{pd.ticket, pd.setup, (int32_t)GM_ERR_INVALID, pd.npix, pd.t_submit,
-                             now_ms(srv), now_ms(srv)};
-            srv->results.push_back(r);
-            srv->cv_done.notify_all();
-            continue;
-        }
+        if (!srv->pending.empty() || !srv->cancel_req.empty()) continue;
         srv->cv_work.wait_for(lk, std::chrono::milliseconds(5));
     }
 }
@@ -470,6 +512,8 @@ extern "C" gm_status gm_server_create(const gm_server_config *cfg, gm_server **o
     if (cfg) s->cfg = *cfg;
     if (s->cfg.hbm_budget_bytes <= 0) s->cfg.hbm_budget_bytes = (int64_t)8 << 30;
     if (s->cfg.max_tiles <= 0) s->cfg.max_tiles = 1024;
+    if (s->cfg.aging_ms == 0) s->cfg.aging_ms = 100;
+    if (s->cfg.scan <= 0) s->cfg.scan = 64;
     if (s->cfg.pool_pixels < 0 || s->cfg.max_tiles > (1 << 20)) { delete s; return GM_ERR_INVALID; }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || s->cfg.device < 0 || s->cfg.device >= ndev) {
@@ -511,6 +555,33 @@ extern "C" gm_status gm_server_submit(gm_server *s, int32_t setup_id, const floa
     if (ticket) *ticket = pd.ticket;
     s->cv_work.notify_one();
     return GM_OK;
+}
+
+extern "C" gm_status gm_server_cancel(gm_server *s, uint64_t ticket)
+{
+    if (!s) return GM_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (auto it = s->pending.begin(); it != s->pending.end(); ++it)
+        if (it->ticket == ticket) {
+            const double t = now_ms(s);
+            gm_tile_result r{it->ticket, it->setup, (int32_t)GM_CANCELLED, it->npix, it->t_submit,
+                             t, t};
+            s->pending.erase(it);
+            s->results.push_back(r);
+            s->cv_done.notify_all();
+            return GM_OK;
+        }
+    for (Engine *e : s->engines)
+        for (const TileSlot &ts : e->tiles)
+            if (ts.used && ts.ticket == ticket) {
+                if (ts.cancelled) return GM_ERR_NOT_FOUND;
+                for (uint64_t q : s->cancel_req)
+                    if (q == ticket) return GM_ERR_NOT_FOUND;
+                s->cancel_req.push_back(ticket);
+                s->cv_work.notify_one();
+                return GM_OK;
+            }
+    return GM_ERR_NOT_FOUND;
 }
 
 extern "C" gm_status gm_server_poll(gm_server *s, gm_tile_result *res, int32_t max, int32_t *n,

@@ -40,7 +40,8 @@ typedef enum {
     GM_ERR_HIP = 2,          /* a HIP runtime call failed                   */
     GM_ERR_NOMEM = 3,        /* device allocation failed                    */
     GM_ERR_UNSUPPORTED = 4,  /* valid for the reference, not built here     */
-    GM_ERR_NOT_FOUND = 5     /* requested item absent (e.g. LINES index)     */
+    GM_ERR_NOT_FOUND = 5,    /* requested item absent (e.g. LINES index)     */
+    GM_CANCELLED = 6         /* serving-loop tile cancelled by the caller    */
 } gm_status;
 
 typedef enum {
@@ -230,6 +231,13 @@ typedef struct {
     int64_t pool_pixels;             /* state slots per setup (0: budget-sized) */
     int32_t max_tiles;               /* in-flight tiles per setup (0: 1024)     */
     int32_t device;                  /* HIP device ordinal                      */
+    int32_t aging_ms;                /* admission: a queued tile that does not
+                                        fit yet is passed by later tiles that
+                                        do, until it has waited this long;
+                                        then nothing passes it (0: 100 ms,
+                                        <0: strict arrival order)              */
+    int32_t scan;                    /* queued tiles examined per admission
+                                        pass (0: 64)                           */
 } gm_server_config;
 typedef struct {
     uint64_t ticket;
@@ -261,6 +269,13 @@ gm_status gm_server_submit(gm_server *srv, int32_t setup_id, const float *obs, i
  * nothing is outstanding). */
 gm_status gm_server_poll(gm_server *srv, gm_tile_result *res, int32_t max, int32_t *n,
                          int32_t wait_ms);
+/* Cancels a submitted tile.  A queued tile leaves the queue; a tile in
+ * flight leaves the LM loop at the next round (its outputs then hold the
+ * models reached so far, iterations past the cap).  Either way its result
+ * arrives through gm_server_poll with status GM_CANCELLED, and its buffers
+ * stay in use until then.  GM_ERR_NOT_FOUND: unknown ticket, or the tile
+ * completed (or was cancelled) already.  Thread-safe.                      */
+gm_status gm_server_cancel(gm_server *srv, uint64_t ticket);
 gm_status gm_server_get_stats(gm_server *srv, gm_server_stats *stats);
 double    gm_server_clock_ms(const gm_server *srv);
 /* Stops the worker after the outstanding tiles complete, frees the pools. */

@@ -327,8 +327,8 @@ def assert_inversion(d, label="", truth=None, fit=None, med=MED, p99=P99, frac=0
     print(msg, flush=True)
     for f in BAR:
         assert st["median"][f] <= med[f], (label, NAMES[f], "median", st["median"][f])
+    assert st["within"] >= frac, (label, "within-bar fraction", st["within"])
     if truth is not None:
-        assert st["within"] >= frac, (label, "within-bar fraction", st["within"])
         assert st["within_wp"] >= frac_wp, (label, "well-posed within-bar", st["within_wp"])
         for f in BAR:
             assert st["p99_wp"][f] <= p99[f], (label, NAMES[f], "p99", st["p99_wp"][f])
@@ -346,9 +346,9 @@ def check_inversion(orc, o, obs, got, ref, truth, label, **kw):
 
 
 def _assert_medians(d, label=""):
-    """Medians inside the north-star bar (set-ups whose measured spread is
-    not yet recorded)."""
-    return assert_inversion(d, label, med=BAR)
+    """Medians near the achieved ones and at least 97 % of all pixels inside
+    the bar (tests without the true models at hand)."""
+    return assert_inversion(d, label)
 
 
 @pytest.mark.parametrize("precision,solver", [(1, 0), (0, 0), (1, 1), (1, 2), (1, 3), (0, 3)])
@@ -456,4 +456,4 @@ def test_fixed_parameters_stay_fixed(gm, orc, cuda):
     # two paths may stop at different points of it, but the fits are equal
     fit = (fp64_chisqr(orc, o, obs, m), fp64_chisqr(orc, o, obs, ref["model"]))
     assert_inversion(param_errors(m, ref["model"]), "3 free", fit=fit,
-                     med={1: 0.05, 2: 1e-5, 5: 0.01, 6: 0.05})
+                     med={1: 0.05, 2: 1e-5, 5: 0.01, 6: 0.05}, frac=0.75)

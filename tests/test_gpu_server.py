@@ -132,3 +132,88 @@ def test_server_budget_admission(gm, cuda):
     with pytest.raises(gm.GmError):
         srv.submit(sid, _obs(gm, ctx, (1 << 15) + 1, seed=9))
     srv.close()
+
+
+def test_server_back_to_back_single_tiles_on_idle_server(gm, cuda):
+    """Each tile submitted to an idle server completes with status 0: a tile
+    never fails admission for arriving while the worker is between passes."""
+    setup = _setups(gm)[1]
+    ctx = gm.Context(setup)
+    obs = _obs(gm, ctx, 4096, seed=55)
+    ref = ctx.invert(obs[:64].contiguous())
+    srv = gm.Server(pool_pixels=1 << 14)
+    sid = srv.add_setup(setup)
+    for k in range(150):
+        n = 1 + (k * 37) % 64
+        t, _ = srv.submit(sid, obs[:n].contiguous())
+        r = _wait_all(srv, [t])[t]
+        assert r["status"] == 0, (k, n, r["status"])
+        if n == 64:
+            _same(r["out"], ref, "single tile")
+    srv.close()
+
+
+def _bytes_per_slot(nlambda):
+    """Pool footprint of one state slot (gm_server.cpp's build_engine)."""
+    return 8 * (2 * 12 + 5) + 4 * (80 + 2 * 20) + 4 * (2 + 4 + 3) + 4 * 2 * 4 * nlambda
+
+
+@pytest.mark.parametrize("aging_ms", [10000, -1])
+def test_server_admission_passes_tiles_that_do_not_fit(gm, cuda, aging_ms):
+    """Budget for 10 000 slots: a 6 000-pixel tile in flight, a second one
+    that does not fit behind it, then a 2 000-pixel tile that does.  With
+    aging the small tile enters at once, ahead of the waiting one; in strict
+    arrival order it waits behind it."""
+    setup = _setups(gm)[1]
+    ctx = gm.Context(setup)
+    budget = 10000 * _bytes_per_slot(30)
+    srv = gm.Server(hbm_budget_bytes=budget, pool_pixels=1 << 15, aging_ms=aging_ms)
+    sid = srv.add_setup(setup)
+    obs = [_obs(gm, ctx, n, seed=300 + n) for n in (6000, 6000, 2000)]
+    tickets = [srv.submit(sid, o)[0] for o in obs]
+    got = _wait_all(srv, tickets)
+    admit = [got[t]["t_submit_ms"] + got[t]["queue_ms"] for t in tickets]
+    print("aging", aging_ms, "admitted at", admit, flush=True)
+    assert all(got[t]["status"] == 0 for t in tickets)
+    assert srv.stats()["peak_inflight_bytes"] <= budget
+    if aging_ms > 0:
+        assert admit[2] < admit[1]
+    else:
+        assert admit[2] >= admit[1]
+    for o, t in zip(obs, tickets):
+        _same(got[t]["out"], ctx.invert(o), "admission order")
+    srv.close()
+
+
+def test_server_cancel(gm, cuda):
+    """A queued tile and a tile in flight are cancelled: both come back with
+    GM_CANCELLED and the other tiles are unaffected (equal to gm_invert); a
+    completed or unknown ticket cannot be cancelled."""
+    setup = _setups(gm)[2]
+    ctx = gm.Context(setup)
+    budget = 12000 * _bytes_per_slot(112)
+    srv = gm.Server(hbm_budget_bytes=budget, pool_pixels=1 << 15, aging_ms=-1)
+    sid = srv.add_setup(setup)
+    obs = [_obs(gm, ctx, n, seed=400 + n) for n in (8000, 9000, 1500, 700)]
+    tickets = [srv.submit(sid, o)[0] for o in obs]
+    # tile 1 cannot enter while tile 0 is in flight (strict order holds
+    # tiles 2 and 3 behind it): cancel it in the queue and tile 0 in flight
+    assert srv.cancel(tickets[1])
+    assert srv.cancel(tickets[0])
+    assert not srv.cancel(tickets[0])            # already being cancelled
+    assert not srv.cancel(987654321)
+    got = _wait_all(srv, tickets)
+    assert got[tickets[0]]["status"] == gm.GM_CANCELLED
+    assert got[tickets[1]]["status"] == gm.GM_CANCELLED
+    for i in (2, 3):
+        assert got[tickets[i]]["status"] == 0
+        _same(got[tickets[i]]["out"], ctx.invert(obs[i]), f"tile {i} beside cancelled ones")
+    assert not srv.cancel(tickets[2])            # completed
+    st = srv.stats()
+    assert st["pending_tiles"] == 0 and st["inflight_tiles"] == 0
+    # the pool is whole again: a tile of the full budget runs
+    t, _ = srv.submit(sid, obs[1])
+    r = _wait_all(srv, [t])[t]
+    assert r["status"] == 0
+    _same(r["out"], ctx.invert(obs[1]), "after cancellations")
+    srv.close()
