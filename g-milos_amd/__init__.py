@@ -27,6 +27,7 @@ GM_MAX_LAMBDA = 256
 GM_MAX_LINES = 4
 GM_MAX_COMP = 10
 GM_NMODEL = 11
+GM_NOALLOC_MAX_PIXELS = 65536
 
 GM_PREC_FP32, GM_PREC_MIXED = 0, 1
 GM_SOLVE_SVD, GM_SOLVE_LDL, GM_SOLVE_JACOBI, GM_SOLVE_CORDIC = 0, 1, 2, 3
@@ -126,7 +127,7 @@ class gm_guided_stats(ctypes.Structure):
 
 
 EXPORTED = ("gm_ctx_create", "gm_ctx_destroy", "gm_status_string", "gm_ctx_set_timing",
-            "gm_last_stats", "gm_invert", "gm_synth", "gm_synth_rf", "gm_normal_equations",
+            "gm_last_stats", "gm_invert", "gm_invert_noalloc", "gm_synth", "gm_synth_rf", "gm_normal_equations",
             "gm_step_solve", "gm_classical_estimates", "gm_zeeman_from_quantum",
             "gm_parse_lines_file", "gm_psf_gaussian", "gm_psf_from_table", "gm_setup_defaults",
             "gm_mlp_create", "gm_mlp_destroy", "gm_mlp_num_params", "gm_mlp_get_params",
@@ -164,6 +165,7 @@ def load_library(path=LIB_PATH):
         "gm_ctx_set_stray_map": ([vp, vp, i64], i32),
         "gm_last_stats": ([vp, P(gm_stats)], i32),
         "gm_invert": ([vp, vp, i64, vp, P(gm_out), vp], i32),
+        "gm_invert_noalloc": ([vp, vp, i64, vp, P(gm_out), vp], i32),
         "gm_synth": ([vp, vp, i64, vp, vp], i32),
         "gm_synth_rf": ([vp, vp, i64, vp, vp, vp], i32),
         "gm_normal_equations": ([vp, vp, vp, i64, vp, vp, vp, vp], i32),
@@ -457,7 +459,12 @@ class Context:
         _check(self.lib.gm_last_stats(self.h, ctypes.byref(st)), "gm_last_stats")
         return st.as_dict()
 
-    def invert(self, obs, init=None, want_synth=True, out=None):
+    def invert_noalloc(self, obs, out, init=None):
+        """gm_invert_noalloc into caller-allocated outputs (a dict as
+        returned by invert); npix at most GM_NOALLOC_MAX_PIXELS."""
+        return self.invert(obs, init=init, out=out, noalloc=True)
+
+    def invert(self, obs, init=None, want_synth=True, out=None, noalloc=False):
         """obs: float32 [npix, 4, nlambda] CUDA tensor.  Returns a dict of
         tensors: model [npix, 11] f64, chisqr, iters, naccs, synth."""
         import torch
@@ -481,8 +488,9 @@ class Context:
         if init is not None:
             assert init.dtype == torch.float64 and init.shape == (npix, GM_NMODEL)
             ip = ctypes.c_void_p(init.data_ptr())
-        _check(self.lib.gm_invert(self.h, ctypes.c_void_p(obs.data_ptr()), npix, ip,
-                                  ctypes.byref(o), self._stream()), "gm_invert")
+        fn = self.lib.gm_invert_noalloc if noalloc else self.lib.gm_invert
+        _check(fn(self.h, ctypes.c_void_p(obs.data_ptr()), npix, ip, ctypes.byref(o),
+                  self._stream()), "gm_invert_noalloc" if noalloc else "gm_invert")
         return out
 
     def synth(self, models):

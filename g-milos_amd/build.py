@@ -12,7 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "lib", "libgmilos.so")
 SOURCES = ["csrc/gm_kernels.hip", "csrc/gm_capi.cpp", "csrc/gm_setup.cpp", "csrc/gm_mlp.hip",
-           "csrc/gm_mlp_host.cpp", "csrc/gm_server.cpp", "csrc/gm_guided.hip"]
+           "csrc/gm_mlp_host.cpp", "csrc/gm_server.cpp", "csrc/gm_guided.hip",
+           "csrc/gm_noalloc.hip"]
 IO_SOURCES = ["host/gm_io.cpp", "host/gm_fits.cpp"]
 IO_LIB = os.path.join(HERE, "lib", "libgmilos_io.so")
 CLI_SOURCES = ["host/gmilos_cli.cpp"]

@@ -24,7 +24,7 @@
 
 namespace {
 
-constexpr int kMaxRounds = 1024;
+constexpr int kMaxRounds = GM_MAX_ROUNDS;
 
 struct Events {
     hipEvent_t a = nullptr, b = nullptr;
@@ -254,6 +254,10 @@ extern "C" gm_status gm_ctx_create(const gm_setup *setup, gm_ctx **out)
         return GM_ERR_HIP;
     }
     if (gm_lds_bytes(c->lc) > (size_t)lds_max) { delete c; return GM_ERR_UNSUPPORTED; }
+    /* device address of this device's gm_invert_noalloc state (once per
+     * device) */
+    gm_status ss = gm_scratch_init(c->device);
+    if (ss != GM_OK) { delete c; return ss; }
     if (hipMalloc(&c->dS, sizeof(gm_dsetup)) != hipSuccess) { delete c; return GM_ERR_NOMEM; }
     if (hipMemcpy(c->dS, &c->host_ds, sizeof(gm_dsetup), hipMemcpyHostToDevice) != hipSuccess) {
         hipFree(c->dS);
@@ -366,36 +370,18 @@ const gm_lanecfg &gm_ctx_lanes(const gm_ctx *c) { return c->lc; }
 const gm_setup &gm_ctx_public_setup(const gm_ctx *c) { return c->setup; }
 int gm_ctx_device(const gm_ctx *c) { return c->device; }
 
-/* Carves the LM state (records and scalar arrays) out of one allocation
- * sized for `cap` pixels; grows it, after a device synchronisation, when a
- * call needs more. */
-static gm_status ensure_state(gm_ctx *c, int64_t npix)
+void gm_carve_state(void *base, int64_t cap, int np, gm_state &st)
 {
-    if (npix <= c->st.cap) return GM_OK;
-    int64_t cap = ((npix + 255) / 256) * 256;
-    const size_t per = sizeof(double) * (2 * GM_MODEL_REC + 5) +
-                       sizeof(float) * (GM_AB_REC11 + 2 * GM_PRE_REC) +
-                       sizeof(int32_t) * (2 + 4);
-    const size_t bytes = per * (size_t)cap + 256 * 22 + 64 + sizeof(int32_t) * 3 * (kMaxRounds + 2) + 256;
-    if (c->ws) {
-        hipDeviceSynchronize();
-        hipFree(c->ws);
-        c->ws = nullptr;
-        c->st.cap = 0;
-    }
-    GM_HIP(hipMalloc(&c->ws, bytes));
-    c->ws_bytes = bytes;
-    char *p = (char *)c->ws;
+    char *p = (char *)base;
     auto take = [&](size_t b) {
         char *r = p;
         p += (b + 255) & ~(size_t)255;
         return (void *)r;
     };
-    gm_state &st = c->st;
     st.cap = cap;
     st.best = (double *)take(sizeof(double) * GM_MODEL_REC * cap);
     st.trial = (double *)take(sizeof(double) * GM_MODEL_REC * cap);
-    const bool p11 = c->lc.np == 11;
+    const bool p11 = np == 11;
     st.ab_rec = p11 ? GM_AB_REC11 : GM_AB_REC9;
     st.alpha = (float *)take(sizeof(float) * GM_AB_REC11 * cap);
     st.beta = st.alpha + (p11 ? GM_AB_BETA11 : GM_AB_BETA9);
@@ -411,6 +397,24 @@ static gm_status ensure_state(gm_ctx *c, int64_t npix)
     for (int i = 0; i < 4; ++i) st.list[i] = (int32_t *)take(sizeof(int32_t) * cap);
     st.counts = (int32_t *)take(sizeof(int32_t) * 4);
     st.hist = (int32_t *)take(sizeof(int32_t) * 3 * (kMaxRounds + 2));
+}
+
+/* The LM state of gm_invert: one allocation sized for `cap` pixels, grown
+ * (after a device synchronisation) when a call needs more. */
+static gm_status ensure_state(gm_ctx *c, int64_t npix)
+{
+    if (npix <= c->st.cap) return GM_OK;
+    const int64_t cap = ((npix + 255) / 256) * 256;
+    const size_t bytes = gm_state_bytes(cap);
+    if (c->ws) {
+        hipDeviceSynchronize();
+        hipFree(c->ws);
+        c->ws = nullptr;
+        c->st.cap = 0;
+    }
+    GM_HIP(hipMalloc(&c->ws, bytes));
+    c->ws_bytes = bytes;
+    gm_carve_state(c->ws, cap, c->lc.np, c->st);
     return GM_OK;
 }
 
@@ -425,9 +429,12 @@ static Events &kernel_events(gm_ctx *c, size_t i)
     return c->kev[i];
 }
 
-extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
-                               const double *init, const gm_out *out, void *stream)
+/* Argument checks shared by gm_invert and gm_invert_noalloc; resets the
+ * statistics.  *done: nothing to launch (empty call). */
+static gm_status invert_prologue(gm_ctx *c, const float *obs, int64_t npix, const double *init,
+                                 const gm_out *out, bool *done)
 {
+    *done = true;
     if (!c || !out || npix < 0) return GM_ERR_INVALID;
     if (npix == 0) {
         std::memset(&c->stats, 0, sizeof c->stats);
@@ -440,24 +447,31 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
     std::memset(&c->stats, 0, sizeof c->stats);
     c->stats.npix = npix;
     c->stats.psf_taps = c->host_ds.use_psf ? (c->host_ds.psf_dhi - c->host_ds.psf_dlo + 1) : 0;
-    if (npix == 0) return GM_OK;
-    GM_TRY(ensure_state(c, npix));
-    hipStream_t s = (hipStream_t)stream;
+    *done = false;
+    return GM_OK;
+}
+
+/* The launches of one inversion over the LM state `st` (capacity >= npix):
+ * start state and first normal equations, miter LM rounds, synthetic output
+ * of the last evaluated models, outputs.  With `timing`, per-kernel events
+ * (created on first use) time each launch. */
+static gm_status invert_launch(gm_ctx *c, const gm_state &st, const float *obs, int64_t npix,
+                               const double *init, const gm_out *out, hipStream_t s, bool timing)
+{
     const int mixed = c->setup.precision == GM_PREC_MIXED;
     const int solver = c->setup.solver;
-    gm_state st = c->st;
     const gm_lanecfg &lc = c->lc;
     int32_t *cnt = st.counts;
     c->kclass.clear();
     c->stats_pending = 0;
     size_t kidx = 0;
     auto kbegin = [&](int cls) {
-        if (!c->timing) return;
+        if (!timing) return;
         hipEventRecord(kernel_events(c, kidx).a, s);
         c->kclass.push_back(cls);
     };
     auto kend = [&]() {
-        if (!c->timing) return;
+        if (!timing) return;
         hipEventRecord(kernel_events(c, kidx).b, s);
         ++kidx;
     };
@@ -527,6 +541,37 @@ extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
     c->stats_pending = 1;
     c->pending_miter = miter;
     return GM_OK;
+}
+
+extern "C" gm_status gm_invert(gm_ctx *c, const float *obs, int64_t npix,
+                               const double *init, const gm_out *out, void *stream)
+{
+    bool done = false;
+    GM_TRY(invert_prologue(c, obs, npix, init, out, &done));
+    if (done) return GM_OK;
+    GM_TRY(ensure_state(c, npix));
+    return invert_launch(c, c->st, obs, npix, init, out, (hipStream_t)stream, c->timing != 0);
+}
+
+/* gm_invert over this device's fixed state array (gm_noalloc.hip), whose
+ * address gm_ctx_create looked up: no allocation of any kind.  The state's
+ * layout is fixed at GM_NOALLOC_MAX_PIXELS pixels.  Everything the kernels
+ * need at zero or at a start value is set by the launches themselves (list
+ * counts, per-round counts, the start state), so the contents left by an
+ * earlier call do not matter. */
+extern "C" gm_status gm_invert_noalloc(gm_ctx *c, const float *obs, int64_t npix,
+                                       const double *init, const gm_out *out, void *stream)
+{
+    if (c && npix > GM_NOALLOC_MAX_PIXELS) return GM_ERR_INVALID;
+    bool done = false;
+    GM_TRY(invert_prologue(c, obs, npix, init, out, &done));
+    if (done) return GM_OK;
+    void *base = gm_scratch_ptr(c->device);
+    if (!base) return GM_ERR_HIP;
+    gm_state st;
+    std::memset(&st, 0, sizeof st);
+    gm_carve_state(base, GM_NOALLOC_MAX_PIXELS, c->lc.np, st);
+    return invert_launch(c, st, obs, npix, init, out, (hipStream_t)stream, false);
 }
 
 /* ---- path pieces ---------------------------------------------------------- */

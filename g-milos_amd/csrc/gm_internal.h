@@ -171,6 +171,32 @@ void gm_launch_ce(const gm_dsetup *dS, const float *obs, int64_t npix, int initi
 void gm_launch_set_count(int32_t *count, int32_t value, hipStream_t stream);
 void gm_launch_copy_count(const int32_t *src, int32_t *dst, hipStream_t stream);
 
+/* LM state of gm_invert_noalloc: a fixed layout for GM_NOALLOC_MAX_PIXELS
+ * pixels (gm_state_bytes of that capacity, rounded up) in a file-scope
+ * device array (gm_noalloc.hip), one per device.                          */
+#define GM_NOALLOC_SCRATCH_BYTES ((size_t)49 << 20)
+#define GM_MAX_ROUNDS 1024   /* LM iteration cap of a set-up (miter) */
+/* Bytes of the LM state for `cap` pixels (cap a multiple of 256): per pixel
+ * two model records, five doubles, the alpha/beta record, two constants
+ * records, two ints and four list entries; then the list counters and the
+ * per-round counts, each array on a 256-byte boundary.                    */
+constexpr size_t gm_state_bytes(int64_t cap)
+{
+    return (size_t)cap * (sizeof(double) * (2 * GM_MODEL_REC + 5) +
+                          sizeof(float) * (GM_AB_REC11 + 2 * GM_PRE_REC) +
+                          sizeof(int32_t) * (2 + 4)) +
+           256 + ((sizeof(int32_t) * 3 * (GM_MAX_ROUNDS + 2) + 255) / 256) * 256;
+}
+static_assert(gm_state_bytes(GM_NOALLOC_MAX_PIXELS) <= GM_NOALLOC_SCRATCH_BYTES,
+              "no-alloc scratch smaller than its state");
+/* Lays the LM state for `cap` pixels out from `base` (256-byte aligned). */
+void gm_carve_state(void *base, int64_t cap, int np, gm_state &st);
+/* Looks the array's device address up for `device`, which must be the
+ * current device; once per device (later calls return at once).          */
+gm_status gm_scratch_init(int device);
+/* The device address found by gm_scratch_init, or NULL before it ran.    */
+void *gm_scratch_ptr(int device);
+
 #define GM_TRY(x)                                                              \
     do {                                                                       \
         gm_status _s = (x);                                                    \

@@ -185,6 +185,19 @@ gm_status gm_last_stats(const gm_ctx *ctx, gm_stats *stats);
 gm_status gm_invert(gm_ctx *ctx, const float *obs, int64_t npix,
                     const double *init, const gm_out *out, void *stream);
 
+/* gm_invert without any allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (work under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The LM state lives
+ * in a fixed device array of about 49 MB per GPU, held from the first
+ * context created on that GPU until the process ends, so npix is at most
+ * GM_NOALLOC_MAX_PIXELS (GM_ERR_INVALID beyond).  Same kernels and results
+ * as gm_invert.  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (other streams, other contexts, other threads) race and must be
+ * serialised by the caller.  Per-kernel timing is not recorded.            */
+#define GM_NOALLOC_MAX_PIXELS 65536
+gm_status gm_invert_noalloc(gm_ctx *ctx, const float *obs, int64_t npix,
+                            const double *init, const gm_out *out, void *stream);
+
 /* ---- path pieces, exposed for parity tests and the serving loop --------- */
 /* Synthesis of npix models (mil_sinrf, mil_sinrf.cu:27-142, with the PSF). */
 gm_status gm_synth(gm_ctx *ctx, const double *models, int64_t npix,
