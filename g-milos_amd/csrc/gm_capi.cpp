@@ -90,12 +90,13 @@ gm_lanecfg gm_lane_config(const gm_dsetup &h)
     else if (n <= 128) { lc.lpp = 64; lc.nlpl = 2; }
     else { lc.lpp = 64; lc.nlpl = 4; }
     lc.ppw = 64 / lc.lpp;
+    lc.ext = h.ext;
+    lc.np = h.np;
+    lc.mfma = (h.use_psf && h.psf_mfma && lc.lpp == 32 && lc.nlpl == 1 && !h.ext) ? 1 : 0;
     const int padlo = h.psf_dlo < 0 ? -h.psf_dlo : 0;
     const int padhi = h.psf_dhi > 0 ? h.psf_dhi : 0;
     lc.padlo = padlo;
     lc.rows = lc.lpp * lc.nlpl + padlo + padhi;
-    lc.ext = h.ext;
-    lc.np = h.np;
     return lc;
 }
 
@@ -163,6 +164,12 @@ static gm_status build_dsetup(const gm_setup &s, gm_dsetup &d)
             cut += take;
             if (lo <= hi) ++jlo; else --jhi;
         }
+        /* many taps, 9 parameters, up to 32 wavelengths: k_rf correlates on
+         * the matrix cores, where the dense product costs the same for all
+         * taps as for a few, so every kernel then applies all of them */
+        const bool ext = s.use_slight || s.fix[9] || s.init_model[9] > 0.0001;
+        d.psf_mfma = (GM_MFMA_RF && n <= 32 && !ext && jhi - jlo + 1 >= GM_MFMA_MIN_TAPS) ? 1 : 0;
+        if (d.psf_mfma) { jlo = 0; jhi = n - 1; }
         d.psf_dlo = jlo - c;
         d.psf_dhi = jhi - c;
         for (int j = jlo; j <= jhi; ++j) d.psf_tap[j - jlo] = (float)s.psf[j];

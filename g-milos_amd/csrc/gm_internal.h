@@ -25,6 +25,19 @@
                                     S1]                                       */
 #define GM_CPT       (GM_MAX_LINES * GM_MAX_COMP)
 
+/* PSF of the 9-parameter response kernel (k_rf) at up to 32 wavelengths on
+ * the matrix cores when at least GM_MFMA_MIN_TAPS taps are applied.  The
+ * dense product costs about as much as the VALU correlation with 15 taps
+ * (FP32 MFMA and FP32 VALU work do not overlap on a SIMD: measured, the
+ * kernel's time is the sum of both), so fewer taps stay on the VALU.
+ * GM_MFMA_RF 0 keeps every set-up on the VALU.                            */
+#ifndef GM_MFMA_RF
+#define GM_MFMA_RF 1
+#endif
+#ifndef GM_MFMA_MIN_TAPS
+#define GM_MFMA_MIN_TAPS 16
+#endif
+
 /* Device copy of the set-up, built once per context (read-only). */
 struct gm_dsetup {
     int32_t nlambda;
@@ -36,6 +49,7 @@ struct gm_dsetup {
     float   fo[GM_MAX_LINES];                /* relative line strengths     */
     float   dl[GM_MAX_LINES][GM_MAX_LAMBDA]; /* lambda - lambda0 (A)        */
     int32_t use_psf;
+    int32_t psf_mfma;                        /* k_rf's PSF on the matrix cores */
     int32_t psf_dlo, psf_dhi;                /* applied tap offsets d=j-c   */
     float   psf_tap[GM_MAX_LAMBDA];          /* tap for offset d: [d-dlo]   */
     float   psf_edge[GM_MAX_LAMBDA];         /* 1 - sum of in-range taps    */
@@ -113,6 +127,8 @@ struct gm_lanecfg {
     int padlo; /* zero rows before wavelength 0                              */
     int ext;   /* macroturbulence / stray-light kernels                      */
     int np;    /* 9 or 11 solved slots                                       */
+    int mfma;  /* k_rf's PSF on the matrix cores (gm_dsetup::psf_mfma): its
+                  staged rows need no zero padding                         */
 };
 
 gm_lanecfg gm_lane_config(const gm_dsetup &h);

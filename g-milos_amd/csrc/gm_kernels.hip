@@ -398,8 +398,9 @@ __device__ __forceinline__ void load_model(const double *__restrict__ models,
 }
 
 /* Reference continuum rule for the I-variant of the PSF: the larger end
- * sample (mil_sinrf.cu:120-123, me_der.cu:171-174).                         */
-__device__ __forceinline__ float end_max(float x0, float xn) { return (x0 > xn) ? x0 : xn; }
+ * sample (mil_sinrf.cu:120-123, me_der.cu:171-174); one v_max_f32 (equal to
+ * the reference's comparison for every finite pair).                       */
+__device__ __forceinline__ float end_max(float x0, float xn) { return fmaxf(x0, xn); }
 
 template <typename T, int W>
 __device__ __forceinline__ T group_sum(T v)
@@ -515,6 +516,55 @@ __device__ __forceinline__ int staged_slot(int p)
     return (p <= 6) ? p + 1 : (p == 7 ? -1 : (p == 8 ? 8 : -2));
 }
 
+/* ------------------------------------------------------------------------ */
+/* The instrument PSF as a matrix product on the matrix cores (up to 32
+ * wavelengths, two pixels per wave).  The reference correlates every
+ * staged vector with all nlambda taps centred at nlambda/2
+ * (convolution.cu:153-241): z[k] = sum_m psf[m - k + n/2] x[m], i.e.
+ * Z = X^T P^T per pixel with P[k][m] the banded Toeplitz matrix of the taps.
+ * Applied densely over m = 0..31 (zero rows and zero table entries past the
+ * grid), all taps cost one fixed product, 32 MFMAs per wave: cheaper than
+ * the VALU correlation from about 16 taps on, dearer below (FP32 matrix and
+ * FP32 VALU work do not overlap on a SIMD), so set-ups with fewer taps
+ * keep the VALU correlation.
+ *
+ * Tap table (LDS, 64 floats, the same for every pixel): tab[u] is the tap
+ * for m - k = u - 31, zero outside the kernel's support.                   */
+__device__ __forceinline__ float psf_tab_entry(const gm_dsetup *__restrict__ S, int u)
+{
+    const int n = S->nlambda;
+    const int j = u - 31 + n / 2;
+    return (u < 63 && j >= 0 && j < n) ? S->psf_full[j] : 0.f;
+}
+
+typedef float gm_f32x32 __attribute__((ext_vector_type(32)));
+
+/* 32 staged values per wavelength (rows of 36 floats, one per wavelength,
+ * rows 0-31 of each pixel's region; xs = the lane's pixel region + k0),
+ * two pixels per wave: v_mfma_f32_32x32x1_2b_f32 with block b = pixel b,
+ * A_b[i] = X_b[m][i] (lane 32b + i), B_b[k] = P[k][m] (lane 32b + k), summed
+ * over m.  D lane L, register r holds block r/16, output k = L%32, value
+ * i = 8((r%16)/4) + 4(L/32) + r%4; one permlane32 swap per register pair
+ * hands each half-wave its own pixel: afterwards registers r and 16 + r of
+ * every lane hold values 8(r/4) + r%4 and 8(r/4) + 4 + r%4 of its pixel at
+ * its own wavelength.                                                      */
+__device__ __forceinline__ void psf_mfma32(const float *__restrict__ xs,
+                                           const float *__restrict__ ts, float (&z)[32])
+{
+    gm_f32x32 acc = {};
+#pragma unroll
+    for (int m = 0; m < 32; ++m)
+        acc = __builtin_amdgcn_mfma_f32_32x32x1f32(xs[m * 36], ts[m], acc, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(acc[r]),
+                                                         __float_as_uint(acc[16 + r]), false,
+                                                         false);
+        z[8 * (r / 4) + (r % 4)] = __uint_as_float(sw[0]);
+        z[8 * (r / 4) + 4 + (r % 4)] = __uint_as_float(sw[1]);
+    }
+}
+
 /* One accept / reject decision (milosUtils.cu:660-698): stop tests on the
  * chi^2 change and value, lambda update (/5 on accept, x10 on reject, with
  * the logclambda factor), lambda bounds and the iteration cap.          */
@@ -598,6 +648,12 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
     const int dlo = S->psf_dlo, dhi = S->psf_dhi;
     const uint32_t fm = S->freemask;
     const int nfree = S->nfree_chi;
+    /* up to 32 wavelengths: the PSF on the matrix cores, all taps; the tap
+     * table after the pixel regions (ordered by the first staging barrier) */
+    constexpr bool MFMA_OK = GM_MFMA_RF && LPP == 32 && NLPL == 1 && XM == 0;
+    const bool mfma = MFMA_OK && use_psf && S->psf_mfma;
+    float *ptab = reinterpret_cast<float *>(lds4 + PPW * region4);
+    if (mfma) ptab[lane] = psf_tab_entry(S, lane);
 
     for (int64_t base = (int64_t)blockIdx.x * PPW; base < cnt;
          base += (int64_t)gridDim.x * PPW) {
@@ -708,6 +764,21 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
                         zm[3] += td * row[5].w;
                     }
                 }
+            } else if (MFMA_OK && mfma) {
+                psf_mfma32(reinterpret_cast<const float *>(conv + padlo * 9) + k0, ptab + 31 - k0, z);
+                /* I-variant (Ic trick) on the Stokes-I block, as below */
+                const float edge = S->psf_edge[k < n ? k : 0];
+                const float4 *r0 = conv + padlo * 9;
+                const float4 *rn = conv + (padlo + n - 1) * 9;
+                const float4 a0 = r0[0], a1 = r0[1];
+                const float4 b0 = rn[0], b1 = rn[1];
+                z[0] += end_max(a0.x, b0.x) * edge;
+                z[1] += end_max(a0.y, b0.y) * edge;
+                z[2] += end_max(a0.z, b0.z) * edge;
+                z[3] += end_max(a0.w, b0.w) * edge;
+                z[4] += end_max(a1.x, b1.x) * edge;
+                z[5] += end_max(a1.y, b1.y) * edge;
+                z[6] += end_max(a1.z, b1.z) * edge;
             } else if (use_psf) {
                 /* the first tap initialises the sums (dlo <= dhi whenever
                  * the PSF is on) */
@@ -2411,18 +2482,27 @@ constexpr int kMaxBlocks = 32768;
 /* Dynamic LDS of k_rf and k_synth: the staged rows of each pixel of the
  * workgroup (or the normal-equation reduction, whichever is larger), plus
  * the macroturbulence kernel rows of the eleven-slot variants.            */
+/* Staged rows per pixel and zero rows before wavelength 0 of each kernel:
+ * the matrix-core PSF reads the 32 rows of the grid only, the VALU
+ * correlation reads rows shifted by every applied tap offset. */
+int rf_rows(const gm_lanecfg &lc) { return lc.mfma ? 32 : lc.rows; }
+int rf_padlo(const gm_lanecfg &lc) { return lc.mfma ? 0 : lc.padlo; }
+int synth_rows(const gm_lanecfg &lc) { return lc.rows; }
+int synth_padlo(const gm_lanecfg &lc) { return lc.padlo; }
+
 size_t rf_lds_bytes(const gm_lanecfg &lc)
 {
     const int xm = lc.np == 11 ? 2 : (lc.ext ? 1 : 0);
     const int red4 = (lc.lpp / 2) * (xm == 2 ? 20 : 15);
-    const int region4 = (lc.rows * 9 > red4) ? lc.rows * 9 : red4;
+    const int region4 = (rf_rows(lc) * 9 > red4) ? rf_rows(lc) * 9 : red4;
     return (size_t)lc.ppw * region4 * sizeof(float4) +
-           (size_t)lc.ppw * (xm == 2 ? 4 : (xm == 1 ? 2 : 0)) * lc.lpp * lc.nlpl * sizeof(float);
+           (size_t)lc.ppw * (xm == 2 ? 4 : (xm == 1 ? 2 : 0)) * lc.lpp * lc.nlpl * sizeof(float) +
+           (lc.mfma ? 64 * sizeof(float) : 0);          /* PSF tap table */
 }
 
 size_t synth_lds_bytes(const gm_lanecfg &lc)
 {
-    return (size_t)lc.ppw * lc.rows * sizeof(float4) +
+    return (size_t)lc.ppw * synth_rows(lc) * sizeof(float4) +
            (lc.ext ? (size_t)lc.ppw * 2 * lc.lpp * lc.nlpl * sizeof(float) : 0);
 }
 
@@ -2436,6 +2516,7 @@ void launch_rf_t(const gm_lanecfg &lc, const gm_dsetup *dS, const double *models
 {
     const int xm = lc.np == 11 ? 2 : (lc.ext ? 1 : 0);
     const size_t shm = rf_lds_bytes(lc);
+    const int rows = rf_rows(lc), padlo = rf_padlo(lc);
     const int g = grid_for(grid_items, lc.ppw, kMaxBlocks);
     /* the LM loop's calls take the specialised kernel (fixed record strides) */
     const int64_t abrec = xm == 2 ? GM_AB_REC11 : GM_AB_REC9;
@@ -2446,12 +2527,12 @@ void launch_rf_t(const gm_lanecfg &lc, const gm_dsetup *dS, const double *models
         if (rec)                                                                                  \
             hipLaunchKernelGGL((k_rf<LPP, NLPL, MX, X, true>), dim3(g), dim3(64), shm, s, dS,     \
                                models, pre, mfs, mps, obs, list, count, n_direct, alpha, beta,    \
-                               aes, aps, bes, bps, chi, synth, rf, lc.rows, lc.padlo, alpha_first, \
+                               aes, aps, bes, bps, chi, synth, rf, rows, padlo, alpha_first,     \
                                model_dst, pre_dst);                                               \
         else                                                                                      \
             hipLaunchKernelGGL((k_rf<LPP, NLPL, MX, X, false>), dim3(g), dim3(64), shm, s, dS,    \
                                models, pre, mfs, mps, obs, list, count, n_direct, alpha, beta,    \
-                               aes, aps, bes, bps, chi, synth, rf, lc.rows, lc.padlo, alpha_first, \
+                               aes, aps, bes, bps, chi, synth, rf, rows, padlo, alpha_first,     \
                                model_dst, pre_dst);                                               \
     } while (0)
     if (xm == 2) {
@@ -2474,6 +2555,7 @@ void launch_synth_t(const gm_lanecfg &lc, const gm_dsetup *dS, const double *mod
                     float *synth, int mixed, hipStream_t s)
 {
     const size_t shm = synth_lds_bytes(lc);
+    const int rows = synth_rows(lc), padlo = synth_padlo(lc);
     const int g = grid_for(grid_items, lc.ppw, kMaxBlocks);
     const bool rec = mfs == 1 && mps == GM_MODEL_REC && pre && list && count && obs && chi;
 #define GM_SYNTH_CASE(MX, EX)                                                                     \
@@ -2481,11 +2563,11 @@ void launch_synth_t(const gm_lanecfg &lc, const gm_dsetup *dS, const double *mod
         if (rec)                                                                                  \
             hipLaunchKernelGGL((k_synth<LPP, NLPL, MX, EX, true>), dim3(g), dim3(64), shm, s, dS, \
                                models, pre, mfs, mps, obs, list, count, n_direct, chi, synth,     \
-                               lc.rows, lc.padlo);                                                \
+                               rows, padlo);                                                \
         else                                                                                      \
             hipLaunchKernelGGL((k_synth<LPP, NLPL, MX, EX, false>), dim3(g), dim3(64), shm, s,    \
                                dS, models, pre, mfs, mps, obs, list, count, n_direct, chi, synth, \
-                               lc.rows, lc.padlo);                                                \
+                               rows, padlo);                                                \
     } while (0)
     if (lc.ext) {
         if (mixed) GM_SYNTH_CASE(true, true);

@@ -371,7 +371,8 @@ def test_inversion_config0_matches_oracle(gm, orc, cuda, precision, solver):
 def test_inversion_config1_psf_matches_oracle(gm, orc, cuda, precision, tail):
     """Config-1 line, grid and PSF (Gaussian FWHM 60 mA) on 2048 pixels, in
     mixed precision and in config 1's own plain FP32; psf_tail_tol 0 applies
-    all 30 taps as the reference does (1e-9 keeps 9)."""
+    all 30 taps as the reference does (the response kernel's correlation
+    then runs on the matrix cores), 1e-9 keeps 9 (on the VALU)."""
     got, ref, truth, obs, ctx = _inversion_errors(gm, orc, cuda, "single", ("gauss", 60.0),
                                                   2048, 200, precision=precision,
                                                   psf_tail_tol=tail)
