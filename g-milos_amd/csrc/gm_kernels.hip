@@ -37,6 +37,15 @@ constexpr double kPI = 3.14159265358979323846264338327950288;
 
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
+/* The first Zeeman sub-component assigns the per-wavelength sums instead of
+ * adding to zeros (response kernel / synthesis kernel). */
+#ifndef GM_PEEL_RF
+#define GM_PEEL_RF 1
+#endif
+#ifndef GM_PEEL_SYNTH
+#define GM_PEEL_SYNTH 1
+#endif
+
 /* ------------------------------------------------------------------------ */
 /* Voigt and Faraday-Voigt functions, FP32: Hui, Armstrong & Wray (1978)
  * rational approximation w(z) ~ P(Z)/Q(Z), Z = a - i u.  The real-coefficient
@@ -194,13 +203,42 @@ template <bool RF>
 __device__ __forceinline__ void eval_lambda(const gm_dsetup *__restrict__ S,
                                             const PixPre &pp, int k, float *out)
 {
-    float fi[3] = {0.f, 0.f, 0.f}, shi[3] = {0.f, 0.f, 0.f};
+    /* profiles and their derivatives summed over lines and sub-components
+     * per component type t = 0 pi, 1 sigma_b, 2 sigma_r; the first
+     * sub-component of the first line assigns the sums */
+    float fi[3], shi[3];
     float dfi[3][4], dshi[3][4];
-#pragma unroll
-    for (int t = 0; t < 3; ++t)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) { dfi[t][j] = 0.f; dshi[t][j] = 0.f; }
     const float shs = S->shi_scale;
+    constexpr bool PEEL = RF ? GM_PEEL_RF : GM_PEEL_SYNTH;
+    auto component = [&](int L, int t, int i, float ulos_ub, float parcial, float auxCte, float fo,
+                         bool first) {
+        const float nu = parcial * S->comp_nu[L][t][i];
+        const float w = S->comp_w[L][t][i] * fo;
+        const float u = ulos_ub - nu * pp.B;
+        float H, F, dHv = 0.f, dFv = 0.f;
+        if (RF)
+            voigt32_d(pp.aa, u, H, F, dHv, dFv);
+        else
+            voigt32(pp.aa, u, H, F);
+        fi[t] = first ? w * H : fmaf(w, H, fi[t]);
+        shi[t] = first ? (2.f * w) * F : fmaf(2.f * w, F, shi[t]);
+        if (RF) {
+            const float dH_u = dHv * w;
+            const float dF_u = dFv * w;
+            const float uu = -u * pp.inv_dopp;
+            const float sdF = shs * dF_u;
+            /* d/d{B, vlos, dopp, a} = (factor) x (per-u derivative) */
+            const float a0[4] = {dH_u, dH_u, dH_u, dF_u};
+            const float f0[4] = {-nu, auxCte, uu, -2.f};
+            const float a1[4] = {sdF, sdF, sdF, shs * dH_u};
+            const float f1[4] = {-nu, auxCte, uu, 0.5f};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                dfi[t][j] = first ? a0[j] * f0[j] : fmaf(a0[j], f0[j], dfi[t][j]);
+                dshi[t][j] = first ? a1[j] * f1[j] : fmaf(a1[j], f1[j], dshi[t][j]);
+            }
+        }
+    };
     const int nlines = S->nlines;
     for (int L = 0; L < nlines; ++L) {
         const float l0 = S->l0[L], fo = S->fo[L];
@@ -211,31 +249,25 @@ __device__ __forceinline__ void eval_lambda(const gm_dsetup *__restrict__ S,
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
             const int nc = S->ncomp[L][t];
-            for (int i = 0; i < nc; ++i) {
-                const float nu = parcial * S->comp_nu[L][t][i];
-                const float w = S->comp_w[L][t][i] * fo;
-                const float u = ub - nu * pp.B;
-                float H, F, dHv = 0.f, dFv = 0.f;
+            int i0 = 0;
+            if (L == 0 && !PEEL) {
+                fi[t] = 0.f;
+                shi[t] = 0.f;
                 if (RF)
-                    voigt32_d(pp.aa, u, H, F, dHv, dFv);
-                else
-                    voigt32(pp.aa, u, H, F);
-                fi[t] += w * H;
-                shi[t] += 2.f * w * F;
-                if (RF) {
-                    const float dH_u = dHv * w;
-                    const float dF_u = dFv * w;
-                    const float uu = -u * pp.inv_dopp;
-                    dfi[t][0] += dH_u * (-nu);
-                    dfi[t][1] += dH_u * auxCte;
-                    dfi[t][2] += dH_u * uu;
-                    dfi[t][3] += -2.f * dF_u;
-                    dshi[t][0] += shs * dF_u * (-nu);
-                    dshi[t][1] += shs * dF_u * auxCte;
-                    dshi[t][2] += shs * dF_u * uu;
-                    dshi[t][3] += shs * dH_u * 0.5f;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) { dfi[t][j] = 0.f; dshi[t][j] = 0.f; }
+            } else if (L == 0) {
+                if (nc > 0) {
+                    component(0, t, 0, ub, parcial, auxCte, fo, true);
+                    i0 = 1;
+                } else {
+                    fi[t] = 0.f;
+                    shi[t] = 0.f;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) { dfi[t][j] = 0.f; dshi[t][j] = 0.f; }
                 }
             }
+            for (int i = i0; i < nc; ++i) component(L, t, i, ub, parcial, auxCte, fo, false);
         }
     }
     const float E0_2 = pp.E0_2;
@@ -303,13 +335,10 @@ __device__ __forceinline__ void eval_lambda(const gm_dsetup *__restrict__ S,
     d_rq[5] = parcial3 * (pp.sindi * pp.cosa);
     d_ru[5] = parcial3 * (pp.sindi * pp.sina);
     d_rv[5] = (shi[2] - shi[1]) * cosdi_E0_2;
-    d_ei[6] = 0.f;
     d_eq[6] = parcial2 * (pp.sinis * pp.cosda);
     d_eu[6] = parcial2 * (pp.sinis * pp.sinda);
-    d_ev[6] = 0.f;
     d_rq[6] = parcial3 * (pp.sinis * pp.cosda);
     d_ru[6] = parcial3 * (pp.sinis * pp.sinda);
-    d_rv[6] = 0.f;
 
     /* chain rule through the ME solution (me_der.cu:317-373).  Every
      * response is linear in its parameter's seven absorption-matrix
@@ -367,19 +396,28 @@ __device__ __forceinline__ void eval_lambda(const gm_dsetup *__restrict__ S,
     for (int st = 0; st < 4; ++st) {
         const float u = sg[st] * dt * dtaux, v = sg[st] * gq[st] * dtaux;
 #pragma unroll
-        for (int m = 0; m < 7; ++m) C[st][m] = u * ca[st][m] - v * cdt[m];
+        for (int m = 0; m < 7; ++m)
+            C[st][m] = (st == 0 && m >= 1 && m <= 3) ? -(v * cdt[m])     /* ca[0][1..3] = 0 */
+                                                     : u * ca[st][m] - v * cdt[m];
     }
+    /* eta0 .. gamma: all seven absorption-matrix derivatives */
 #pragma unroll
-    for (int p = 0; p < 7; ++p) {
+    for (int p = 0; p < 6; ++p) {
         const float dm[7] = {d_ei[p], d_eq[p], d_eu[p], d_ev[p], d_rq[p], d_ru[p], d_rv[p]};
 #pragma unroll
         for (int st = 0; st < 4; ++st) {
-            float o = 0.f;
+            float o = C[st][0] * dm[0];
 #pragma unroll
-            for (int m = 0; m < 7; ++m) o += C[st][m] * dm[m];
+            for (int m = 1; m < 7; ++m) o = fmaf(C[st][m], dm[m], o);
             out[st * 9 + 1 + p] = o;
         }
     }
+    /* phi changes eta_Q,U and rho_Q,U only, and d/dphi of I and V is zero
+     * (me_der.cu:372-373) */
+#pragma unroll
+    for (int st = 1; st < 3; ++st)
+        out[st * 9 + 7] = fmaf(C[st][1], d_eq[6],
+                               fmaf(C[st][2], d_eu[6], fmaf(C[st][4], d_rq[6], C[st][5] * d_ru[6])));
     const float dti2 = -(dti * ah);
     out[0 + 8] = -dti2 * etai_gp3;
     out[9 + 8] = dti2 * g4;
@@ -398,9 +436,8 @@ __device__ __forceinline__ void load_model(const double *__restrict__ models,
 }
 
 /* Reference continuum rule for the I-variant of the PSF: the larger end
- * sample (mil_sinrf.cu:120-123, me_der.cu:171-174); one v_max_f32 (equal to
- * the reference's comparison for every finite pair).                       */
-__device__ __forceinline__ float end_max(float x0, float xn) { return fmaxf(x0, xn); }
+ * sample (mil_sinrf.cu:120-123, me_der.cu:171-174).                         */
+__device__ __forceinline__ float end_max(float x0, float xn) { return (x0 > xn) ? x0 : xn; }
 
 template <typename T, int W>
 __device__ __forceinline__ T group_sum(T v)
@@ -565,6 +602,29 @@ __device__ __forceinline__ void psf_mfma32(const float *__restrict__ xs,
     }
 }
 
+/* Normal-equation entry e of the packed upper triangle (n x n, n = 9 or
+ * 11) -> its parameter pair, for masking fixed parameters after the
+ * reduction. */
+struct PairTable {
+    int8_t p[66], q[66];
+    constexpr PairTable(int nn) : p{}, q{}
+    {
+        int e = 0;
+        for (int a = 0; a < nn; ++a)
+            for (int b = a; b < nn; ++b) { p[e] = (int8_t)a; q[e] = (int8_t)b; ++e; }
+    }
+};
+__constant__ constexpr PairTable kPairs9{9};
+__constant__ constexpr PairTable kPairs11{11};
+
+/* Responses that vanish identically: d/dS0 of Q, U, V (me_der.cu:365-368)
+ * and d/dphi of I and V (me_der.cu:372-373); their normal-equation
+ * products are skipped at compile time. */
+__host__ __device__ constexpr bool rf_nonzero(int st, int p)
+{
+    return !((st != 0 && p == 7) || ((st == 0 || st == 3) && p == 6));
+}
+
 /* One accept / reject decision (milosUtils.cu:660-698): stop tests on the
  * chi^2 change and value, lambda update (/5 on accept, x10 on reject, with
  * the logclambda factor), lambda bounds and the iteration cap.          */
@@ -688,20 +748,18 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
                 ob[s][st] = (has_obs && k < n) ? obs[pix * 4 * n + st * n + k] : 0.f;
         }
 
-        /* zero padding rows (the reduction buffer aliases them) */
+        /* zero rows: the padding and the wavelengths past the grid (the
+         * reduction buffer aliases them) */
         for (int r = k0; r < rows; r += LPP)
-            if (r < padlo || r >= padlo + LPP * NLPL)
+            if (r < padlo || r >= padlo + n)
 #pragma unroll
                 for (int j = 0; j < 9; ++j) conv[r * 9 + j] = make_float4(0.f, 0.f, 0.f, 0.f);
-        /* stage unconvolved values, one row per wavelength */
+        /* stage unconvolved values, one row per wavelength of the grid */
 #pragma unroll
         for (int s = 0; s < NLPL; ++s) {
             const int k = k0 + s * LPP;
             float v[GM_SLOTS];
             eval_lambda<true>(S, pp, k < n ? k : n - 1, v);
-            if (k >= n)
-#pragma unroll
-                for (int j = 0; j < GM_SLOTS; ++j) v[j] = 0.f;
             /* 30 of the 36 values: d/dphi of I and V is zero and the S1
              * responses follow from the convolved synthesis (below) */
             float c[32];
@@ -709,10 +767,12 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
             for (int i = 0; i < 30; ++i) c[i] = v[kStagedSlot[i]];
             c[30] = 0.f;
             c[31] = 0.f;
-            float4 *row = conv + (padlo + k) * 9;
+            if (k < n) {
+                float4 *row = conv + (padlo + k) * 9;
 #pragma unroll
-            for (int j = 0; j < 8; ++j)
-                row[j] = make_float4(c[4 * j], c[4 * j + 1], c[4 * j + 2], c[4 * j + 3]);
+                for (int j = 0; j < 8; ++j)
+                    row[j] = make_float4(c[4 * j], c[4 * j + 1], c[4 * j + 2], c[4 * j + 3]);
+            }
         }
         __syncthreads();
         /* EXT: this pixel's macroturbulence and filling factor */
@@ -726,11 +786,20 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
         }
         const float s0c = (EXT && mac_on) ? gsum : 1.f;
         const float s0rf = (EXT && S->use_slight) ? alv : 1.f;
-        const float inv_s1 = 1.f / pp.S1;
+        const float inv_s1 = frcp(pp.S1);
 
+        /* INIT1 (one wavelength per lane, no stray light): the first Stokes
+         * parameter's products initialise the normal-equation sums, and a
+         * lane past the grid contributes with weight zero instead of being
+         * skipped */
+        constexpr bool INIT1 = NLPL == 1 && XM == 0;
         float acc[NRED];
+        if constexpr (!INIT1) {
 #pragma unroll
-        for (int e = 0; e < NRED; ++e) acc[e] = 0.f;
+            for (int e = 0; e < NRED; ++e) acc[e] = 0.f;
+        } else if constexpr (MIXED) {
+            acc[NRED - 1] = 0.f;
+        }
         double chi = 0.0;
 #pragma unroll
         for (int s = 0; s < NLPL; ++s) {
@@ -842,7 +911,8 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
             y[17] = y[9] * inv_s1;
             y[26] = y[18] * inv_s1;
             y[35] = y[27] * inv_s1;
-            if (k < n) {
+            const bool in_grid = k < n;
+            if (INIT1 || in_grid) {
                 float ra[4] = {0.f, 0.f, 0.f, 0.f};  /* P11: d/dalfa per Stokes */
                 if (EXT && S->use_slight) {
                     /* stray light (mil_sinrf.cu:133-140, me_der.cu:191-207):
@@ -861,6 +931,9 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
                         if (P11) zm[st] *= alv;
                     }
                 }
+                /* responses of every parameter, fixed ones included: their
+                 * rows and columns are zeroed after the reduction
+                 * (FijaACeroDerivadasNoNecesarias, milosUtils.cu:243-271) */
                 float g[4][NP];
                 float r[4];
 #pragma unroll
@@ -869,56 +942,75 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
 #pragma unroll
                     for (int p = 0; p < 9; ++p) {
                         const int js = staged_slot(p);
-                        float gv = (js >= 0) ? y[st * 9 + js] : (st == 0 ? s0rf : 0.f);
-                        g[st][p] = ((fm >> p) & 1u) ? gv : 0.f;
+                        g[st][p] = (js >= 0) ? y[st * 9 + js] : (st == 0 ? s0rf : 0.f);
                     }
                     if constexpr (P11) {
-                        g[st][9] = ((fm >> 9) & 1u) ? zm[st] : 0.f;
-                        g[st][10] = ((fm >> 10) & 1u) ? ra[st] : 0.f;
+                        g[st][9] = zm[st];
+                        g[st][10] = ra[st];
                     }
                 }
 #pragma unroll
                 for (int st = 0; st < 4 && has_obs; ++st) {
-                    const float ws = S->wsig[st];
+                    const float ws = (INIT1 && !in_grid) ? 0.f : S->wsig[st];
+                    const bool first = INIT1 && st == 0;
                     if (MIXED)
                         chi += (double)ws * ((double)r[st] * (double)r[st]);
                     else
-                        acc[NRED - 1] += (r[st] * r[st]) * ws;
+                        acc[NRED - 1] = first ? (r[st] * r[st]) * ws
+                                              : fmaf(r[st] * r[st], ws, acc[NRED - 1]);
                     int e = 0;
 #pragma unroll
                     for (int p = 0; p < NP; ++p) {
                         const float wg = ws * g[st][p];
 #pragma unroll
-                        for (int q = p; q < NP; ++q) acc[e++] += wg * g[st][q];
+                        for (int q = p; q < NP; ++q, ++e) {
+                            if (!rf_nonzero(st, p) || !rf_nonzero(st, q)) {
+                                if (first) acc[e] = 0.f;
+                                continue;
+                            }
+                            acc[e] = first ? wg * g[st][q] : fmaf(wg, g[st][q], acc[e]);
+                        }
                     }
+                    const float wr = ws * r[st];
 #pragma unroll
-                    for (int p = 0; p < NP; ++p) acc[NA + p] += (ws * r[st]) * g[st][p];
+                    for (int p = 0; p < NP; ++p) {
+                        if (!rf_nonzero(st, p)) {
+                            if (first) acc[NA + p] = 0.f;
+                            continue;
+                        }
+                        acc[NA + p] = first ? wr * g[st][p] : fmaf(wr, g[st][p], acc[NA + p]);
+                    }
                 }
-                if (synth_out && valid) {
+                if (in_grid && synth_out && valid) {
                     float *so = synth_out + pix * 4 * n + k;
 #pragma unroll
                     for (int st = 0; st < 4; ++st) so[st * n] = y[st * 9];
                 }
-                if (has_rf && valid) {
+                if (in_grid && has_rf && valid) {
                     if constexpr (P11) {
                         /* reference layout [4][NTERMS][n]: NTERMS 10 keeps
                          * the filling factor in slot 9, 11 has mac in 9 and
-                         * alfa in 10 */
+                         * alfa in 10; fixed parameters' responses are zero */
                         const int nto = S->nt_ref;
                         float *ro = rf_out + pix * 4 * nto * n + k;
 #pragma unroll
                         for (int st = 0; st < 4; ++st) {
 #pragma unroll
-                            for (int p = 0; p < 9; ++p) ro[(st * nto + p) * n] = g[st][p];
-                            ro[(st * nto + 9) * n] = (nto == 11) ? g[st][9] : g[st][10];
-                            if (nto == 11) ro[(st * nto + 10) * n] = g[st][10];
+                            for (int p = 0; p < 9; ++p)
+                                ro[(st * nto + p) * n] = ((fm >> p) & 1u) ? g[st][p] : 0.f;
+                            const float g9 = ((fm >> 9) & 1u) ? g[st][9] : 0.f;
+                            const float g10 = ((fm >> 10) & 1u) ? g[st][10] : 0.f;
+                            ro[(st * nto + 9) * n] = (nto == 11) ? g9 : g10;
+                            if (nto == 11) ro[(st * nto + 10) * n] = g10;
                         }
                     } else {
                         float *ro = rf_out + pix * 4 * 10 * n + k;
 #pragma unroll
                         for (int st = 0; st < 4; ++st)
 #pragma unroll
-                            for (int p = 0; p < 10; ++p) ro[(st * 10 + p) * n] = (p < 9) ? g[st][p] : 0.f;
+                            for (int p = 0; p < 10; ++p)
+                                ro[(st * 10 + p) * n] =
+                                    (p < 9 && ((fm >> p) & 1u)) ? g[st][p] : 0.f;
                     }
                 }
             }
@@ -945,6 +1037,10 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
             float sum = 0.f;
 #pragma unroll 8
             for (int l = 0; l < HL; ++l) sum += red[l * RSTR + e];
+            /* rows and columns of fixed parameters are zero */
+            const int pp_ = (e < NA) ? (P11 ? kPairs11.p[e] : kPairs9.p[e]) : e - NA;
+            const int qq_ = (e < NA) ? (P11 ? kPairs11.q[e] : kPairs9.q[e]) : e - NA;
+            sum = ((fm >> pp_) & (fm >> qq_) & 1u) ? sum : 0.f;
             if (valid) {
                 if (e < NA) {
                     if (has_alpha) alpha[e * aes + pix * aps] = sum;
@@ -1006,8 +1102,10 @@ __global__ __launch_bounds__(64) void k_synth(
     const int dlo = S->psf_dlo, dhi = S->psf_dhi;
     const int nfree = S->nfree_chi;
 
+    /* zero rows, written once: the padding and the wavelengths past the
+     * grid (no lane stages those) */
     for (int r = k0; r < rows; r += LPP)
-        if (r < padlo || r >= padlo + LPP * NLPL) conv[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (r < padlo || r >= padlo + n) conv[r] = make_float4(0.f, 0.f, 0.f, 0.f);
 
     for (int64_t base = (int64_t)blockIdx.x * PPW; base < cnt;
          base += (int64_t)gridDim.x * PPW) {
@@ -1037,8 +1135,7 @@ __global__ __launch_bounds__(64) void k_synth(
             const int k = k0 + s * LPP;
             float v[GM_SLOTS];
             eval_lambda<false>(S, pp, k < n ? k : n - 1, v);
-            conv[padlo + k] = (k < n) ? make_float4(v[0], v[9], v[18], v[27])
-                                      : make_float4(0.f, 0.f, 0.f, 0.f);
+            if (k < n) conv[padlo + k] = make_float4(v[0], v[9], v[18], v[27]);
         }
         __syncthreads();
         bool mac_on = false;
