@@ -247,11 +247,13 @@ def pmc_probe(gm, torch, dev, npix, seed):
                       "k_synth_algorithmic_bytes_per_eval": 20 * 4 + 4 * 30 * 4 + 8}), flush=True)
 
 
-# Serving admission budget: 4 GB of in-flight LM state (about 1-2 M pixels)
-# keeps the GPU saturated while each round, and so each tile's 50 rounds,
-# stays short; measured at 200 tiles/s: p99 837 ms with 24 GB, 487 ms with
-# 8 GB, 372 ms with 4 GB, at the same sustained rate.
-SERVE_HBM_BUDGET, SERVE_POOL_PIXELS = 4 << 30, 1 << 20
+# Serving admission budget: 2 GB of in-flight LM state (about 0.5-1 M
+# pixels) keeps the GPU saturated while each round, and so each tile's 50
+# rounds, stays short; tiles that fit pass one that does not for up to 1 s
+# (the server's default aging).  Measured at 200 tiles/s over 60 s, all at
+# the offered rate: p50 / p99 30 / 200 ms with 2 GB and 1 s aging, 39 / 325
+# ms (30 s) with 4 GB and 0.1 s, 31 / 351 ms (30 s) with 1 GB.
+SERVE_HBM_BUDGET, SERVE_POOL_PIXELS = 2 << 30, 1 << 20
 
 
 def serving_leg(gm, torch, dev, seconds, rate, seed, hbm_budget=SERVE_HBM_BUDGET,

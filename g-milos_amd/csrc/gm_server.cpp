@@ -512,7 +512,7 @@ extern "C" gm_status gm_server_create(const gm_server_config *cfg, gm_server **o
     if (cfg) s->cfg = *cfg;
     if (s->cfg.hbm_budget_bytes <= 0) s->cfg.hbm_budget_bytes = (int64_t)8 << 30;
     if (s->cfg.max_tiles <= 0) s->cfg.max_tiles = 1024;
-    if (s->cfg.aging_ms == 0) s->cfg.aging_ms = 100;
+    if (s->cfg.aging_ms == 0) s->cfg.aging_ms = 1000;
     if (s->cfg.scan <= 0) s->cfg.scan = 64;
     if (s->cfg.pool_pixels < 0 || s->cfg.max_tiles > (1 << 20)) { delete s; return GM_ERR_INVALID; }
     int ndev = 0;

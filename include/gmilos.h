@@ -247,7 +247,7 @@ typedef struct {
     int32_t aging_ms;                /* admission: a queued tile that does not
                                         fit yet is passed by later tiles that
                                         do, until it has waited this long;
-                                        then nothing passes it (0: 100 ms,
+                                        then nothing passes it (0: 1000 ms,
                                         <0: strict arrival order)              */
     int32_t scan;                    /* queued tiles examined per admission
                                         pass (0: 64)                           */
