@@ -715,11 +715,20 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
     float *ptab = reinterpret_cast<float *>(lds4 + PPW * region4);
     if (mfma) ptab[lane] = psf_tab_entry(S, lane);
 
-    for (int64_t base = (int64_t)blockIdx.x * PPW; base < cnt;
-         base += (int64_t)gridDim.x * PPW) {
+    /* grid-stride loop over pixel pairs; the list entry of the next pair
+     * is fetched one iteration ahead, so only the pixel's own records are
+     * waited for at the top of an iteration */
+    const int64_t stride = (int64_t)gridDim.x * PPW;
+    auto fetch_pix = [&](int64_t b) -> int64_t {
+        const int64_t use = (b + h < cnt) ? b + h : b;
+        return has_list ? (int64_t)list[use] : use;
+    };
+    int64_t pix_next = ((int64_t)blockIdx.x * PPW < cnt) ? fetch_pix((int64_t)blockIdx.x * PPW) : 0;
+    for (int64_t base = (int64_t)blockIdx.x * PPW; base < cnt; base += stride) {
         const int64_t item = base + h;
         const bool valid = item < cnt;
-        const int64_t pix = has_list ? (int64_t)list[valid ? item : base] : (valid ? item : base);
+        const int64_t pix = pix_next;
+        if (base + stride < cnt) pix_next = fetch_pix(base + stride);
         PixPre pp;
         if (has_pre) {
             pp = load_pre(pre, pix);
@@ -1107,11 +1116,20 @@ __global__ __launch_bounds__(64) void k_synth(
     for (int r = k0; r < rows; r += LPP)
         if (r < padlo || r >= padlo + n) conv[r] = make_float4(0.f, 0.f, 0.f, 0.f);
 
-    for (int64_t base = (int64_t)blockIdx.x * PPW; base < cnt;
-         base += (int64_t)gridDim.x * PPW) {
+    /* grid-stride loop over pixel pairs; the list entry of the next pair
+     * is fetched one iteration ahead, so only the pixel's own records are
+     * waited for at the top of an iteration */
+    const int64_t stride = (int64_t)gridDim.x * PPW;
+    auto fetch_pix = [&](int64_t b) -> int64_t {
+        const int64_t use = (b + h < cnt) ? b + h : b;
+        return has_list ? (int64_t)list[use] : use;
+    };
+    int64_t pix_next = ((int64_t)blockIdx.x * PPW < cnt) ? fetch_pix((int64_t)blockIdx.x * PPW) : 0;
+    for (int64_t base = (int64_t)blockIdx.x * PPW; base < cnt; base += stride) {
         const int64_t item = base + h;
         const bool valid = item < cnt;
-        const int64_t pix = has_list ? (int64_t)list[valid ? item : base] : (valid ? item : base);
+        const int64_t pix = pix_next;
+        if (base + stride < cnt) pix_next = fetch_pix(base + stride);
         PixPre pp;
         if (has_pre) {
             pp = load_pre(pre, pix);
@@ -1623,7 +1641,9 @@ template <typename T>
 __device__ __forceinline__ void apply_to_trial(const gm_dsetup *__restrict__ S, const gm_state &st,
                                                int64_t pix, const T (&x)[9])
 {
-    const int64_t cap = st.cap;
+    /* free fields of the trial are best - delta; fixed ones keep the
+     * trial's values, which check() may have changed (AplicaDeltaf writes
+     * only free fields into the persisting model, milosUtils.cu:93-159) */
     double best[GM_NMODEL], m[GM_NMODEL];
 #pragma unroll
     for (int f = 0; f < GM_NMODEL; ++f) {
