@@ -26,8 +26,8 @@ PEAK_FP32_TFLOPS = 157.3          # MI355X FP32 vector peak (packed FMA; 78.6 un
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per pixel evaluation of the LM loop's k_rf (FETCH_SIZE +
 # WRITE_SIZE summed over an inversion's launches, scaled by a copy kernel of
-# known size): profiles/k_rf_k_synth_hbm_counters_262144px.md
-K_RF_HBM_BYTES_PER_PIXEL = 1331.0
+# known size): profiles/lm_and_cube_io_hbm_counters_1048576px.md
+K_RF_HBM_BYTES_PER_PIXEL = 1330.0
 # |d| bar of the north star against the FP64 oracle: B [G], vlos [km/s],
 # gamma, phi [deg] (model fields 1, 2, 5, 6)
 BAR = {1: 1.0, 2: 1e-3, 5: 0.1, 6: 0.1}
@@ -683,9 +683,9 @@ def main():
                      "traffic": K_RF_HBM_BYTES_PER_PIXEL * rf_px_per_launch,
                      "traffic_unit": "bytes per launch (mean launch of the timed steps)",
                      "traffic_source": "profile-derived: FETCH_SIZE + WRITE_SIZE per pixel "
-                                       "evaluation of k_rf (profiles/k_rf_k_synth_hbm_counters_"
-                                       "262144px.md, copy-kernel scaled) x this run's "
-                                       "evaluations per launch"},
+                                       "evaluation of k_rf (profiles/lm_and_cube_io_hbm_"
+                                       "counters_1048576px.md, copy-kernel scaled) x this "
+                                       "run's evaluations per launch"},
         "k_synth_tflops": synth_tflops,
         "whole_job_tflops": job_tflops,
         "kernel_ms_per_step": {k[3:]: acc[k] / args.steps for k in
