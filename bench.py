@@ -243,6 +243,7 @@ def pmc_probe(gm, torch, dev, npix, seed):
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes": 4 * n_copy, "npix": npix,
                       "rf_evals": int(st["rf_evals"]), "synth_evals": int(st["synth_evals"]),
+                      "solves": int(st["solves"]), "svd_solves": int(st["svd_solves"]),
                       "k_rf_algorithmic_bytes_per_eval": 20 * 4 + 4 * 30 * 4 + 54 * 4 + 8,
                       "k_synth_algorithmic_bytes_per_eval": 20 * 4 + 4 * 30 * 4 + 8}), flush=True)
 

@@ -1252,6 +1252,48 @@ template <> __device__ __forceinline__ double rsq<double>(double x)
     return y * (1.5 - 0.5 * x * y * y);      /* one Newton step */
 }
 
+/* Rotation that annihilates a_pq of the symmetric 2x2 block
+ * [[a_pp, a_pq], [a_pq, a_qq]] (the smaller of the two angles, |t| <= 1):
+ * t = sign(d) 2 a_pq / (|d| + r), d = a_qq - a_pp, r = sqrt(d^2 + 4 a_pq^2),
+ * c = 1 / sqrt(1 + t^2), s = t c.  In FP64 it is formed from two
+ * reciprocal square roots instead of a square root and a division: with
+ * y = 1/r and h = (1 + |d| y) / 2 (= c^2), z = 1/sqrt(h) gives c = h z,
+ * s = sign(d) a_pq y z and t = s z; c^2 + s^2 = h z^2 = 1.  Each hardware
+ * estimate takes two Newton steps, so the result is the IEEE formula's to
+ * a few units in the last place.  r = 0 (a zero block) gives the identity. */
+__device__ __forceinline__ double rsq_nr2(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * (1.5 - 0.5 * x * y * y);
+    return y * (1.5 - 0.5 * x * y * y);
+}
+
+template <typename T>
+__device__ __forceinline__ void jacobi_angle(T app, T aqq, T apq, T &c, T &sn, T &t)
+{
+    const T d = aqq - app;
+    if constexpr (sizeof(T) == 8) {
+        const double r2 = d * d + 4.0 * apq * apq;
+        /* r2 = 0 only for a_pq = 0 = d: h = 1 gives c = 1, s = t = 0 */
+        const bool rot = r2 > 0.0;
+        const double y = rsq_nr2(rot ? r2 : 1.0);
+        const double h = rot ? 0.5 + 0.5 * (fabs(d) * y) : 1.0;
+        const double z = rsq_nr2(h);
+        double sv = (apq * y) * z;
+        sv = (d < 0.0) ? -sv : sv;
+        c = h * z;
+        sn = sv;
+        t = sv * z;
+    } else {
+        const T den = fabs(d) + sqrt(d * d + T(4) * apq * apq);
+        T ti = (den > T(0)) ? (T(2) * apq) / den : T(0);
+        ti = (d < T(0)) ? -ti : ti;
+        t = ti;
+        c = rsq<T>(T(1) + ti * ti);
+        sn = ti * c;
+    }
+}
+
 /* SVD of the symmetric damped matrix (singular values = |eigenvalues|) by
  * Jacobi rotations, then the reference's pseudo-inverse
  * x = V diag(1/w, |w| > 1e-12) V^T b (milosUtils.cu:296-322).
@@ -1273,13 +1315,7 @@ __device__ __forceinline__ void jacobi_step(T (&a)[45], T (&v)[81])
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const T app = a[ut(P[i], P[i])], aqq = a[ut(Q[i], Q[i])], apq = a[ut(P[i], Q[i])];
-        const T d = aqq - app;
-        const T den = fabs(d) + sqrt(d * d + T(4) * apq * apq);
-        T ti = (den > T(0)) ? (T(2) * apq) / den : T(0);
-        ti = (d < T(0)) ? -ti : ti;
-        t[i] = ti;
-        c[i] = rsq<T>(T(1) + ti * ti);
-        sn[i] = ti * c[i];
+        jacobi_angle<T>(app, aqq, apq, c[i], sn[i], t[i]);
     }
     /* blocks between two pairs: rows of pair i, columns of pair j */
 #pragma unroll
@@ -1383,13 +1419,7 @@ __device__ __forceinline__ void jacobi_step_rows(T (&a)[45], T (&vr)[RR][9])
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const T app = a[ut(P[i], P[i])], aqq = a[ut(Q[i], Q[i])], apq = a[ut(P[i], Q[i])];
-        const T d = aqq - app;
-        const T den = fabs(d) + sqrt(d * d + T(4) * apq * apq);
-        T ti = (den > T(0)) ? (T(2) * apq) / den : T(0);
-        ti = (d < T(0)) ? -ti : ti;
-        t[i] = ti;
-        c[i] = rsq<T>(T(1) + ti * ti);
-        sn[i] = ti * c[i];
+        jacobi_angle<T>(app, aqq, apq, c[i], sn[i], t[i]);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -1766,13 +1796,7 @@ __device__ __forceinline__ void jacobi_step_n(T (&a)[N * (N + 1) / 2], T (&v)[N 
     for (int i = 0; i < H; ++i) {
         const T app = a[utn<N>(P[i], P[i])], aqq = a[utn<N>(Q[i], Q[i])];
         const T apq = a[utn<N>(P[i], Q[i])];
-        const T dd = aqq - app;
-        const T den = fabs(dd) + sqrt(dd * dd + T(4) * apq * apq);
-        T ti = (den > T(0)) ? (T(2) * apq) / den : T(0);
-        ti = (dd < T(0)) ? -ti : ti;
-        t[i] = ti;
-        c[i] = rsq<T>(T(1) + ti * ti);
-        sn[i] = ti * c[i];
+        jacobi_angle<T>(app, aqq, apq, c[i], sn[i], t[i]);
     }
 #pragma unroll
     for (int i = 0; i < H; ++i)
@@ -2167,10 +2191,13 @@ __global__ __launch_bounds__(64) void k_solve(const gm_dsetup *__restrict__ S,
  * kSvdLanes lanes per pixel.  Few pixels fall back per round, so the launch
  * lasts as long as one group's instruction stream: every lane repeats the
  * rotations of the matrix, but each keeps fewer rows of V the more lanes
- * share a pixel.  Per 1M-pixel inversion (solve, both tiers): 8.3 ms with
- * 2 lanes, 8.1 with 4, 7.9 with 8, 7.75 with 16. */
+ * share a pixel, and a wave waits for the slowest of its pixels' sweeps.
+ * Per 1M-pixel inversion (solve, both tiers), with the IEEE square root
+ * and division in the rotation: 8.3 ms with 2 lanes, 8.1 with 4, 7.9 with
+ * 8, 7.75 with 16; with the reciprocal-square-root rotation: 7.09 with 16,
+ * 7.06 with 32, 6.99 with 64 (one pixel per wave). */
 #ifndef GM_SVD_LANES
-#define GM_SVD_LANES 16
+#define GM_SVD_LANES 64
 #endif
 constexpr int kSvdLanes = GM_SVD_LANES;
 template <bool MIXED>
