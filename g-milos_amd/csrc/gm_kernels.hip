@@ -439,6 +439,32 @@ __device__ __forceinline__ void load_model(const double *__restrict__ models,
  * sample (mil_sinrf.cu:120-123, me_der.cu:171-174).                         */
 __device__ __forceinline__ float end_max(float x0, float xn) { return (x0 > xn) ? x0 : xn; }
 
+/* Folds two per-lane partial sums across the lane halves of an H-lane
+ * pair of groups (H = 16: lanes L and L ^ 16; H = 32: L and L ^ 32): the
+ * lower lane of each pair gets a's total, the upper one b's.  One
+ * v_permlane16/32_swap (a VALU instruction that trades the upper group of
+ * its first operand with the lower group of its second) and one add per
+ * two sums, instead of a ds_bpermute per sum through the LDS pipeline,
+ * which the PSF and the reduction already keep busy. */
+template <int H>
+__device__ __forceinline__ float fold_pair(float a, float b)
+{
+    static_assert(H == 16 || H == 32, "fold width");
+    unsigned lo, hi;
+    if constexpr (H == 16) {
+        const auto sw = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b),
+                                                         false, false);
+        lo = sw[0];
+        hi = sw[1];
+    } else {
+        const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b),
+                                                         false, false);
+        lo = sw[0];
+        hi = sw[1];
+    }
+    return __uint_as_float(lo) + __uint_as_float(hi);
+}
+
 template <typename T, int W>
 __device__ __forceinline__ T group_sum(T v)
 {
@@ -1024,28 +1050,37 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
                 }
             }
         }
-        /* cross-lane reduction of the 55 partial sums: fold the upper half of
-         * the pixel's lanes onto the lower half with lane swaps, then sum the
-         * HL remaining rows through LDS (aliasing the staged rows) */
+        /* cross-lane reduction of the normal-equation partial sums: fold
+         * the two lane halves of the pixel pairwise (lane l < HL keeps the
+         * even entries, lane l + HL the odd ones), then sum the HL rows of
+         * each half through LDS (aliasing the staged rows): entry e lives
+         * at red[l * RSTR + (e & 1) * HALF + e / 2] */
         constexpr int HL = LPP / 2;
+        constexpr int NSUM = NRED - 1;                  /* alpha and beta entries */
+        constexpr int NPAIR = (NSUM + 1) / 2;
+        constexpr int HALF = (NPAIR + 3) / 4 * 4;
+        static_assert(2 * HALF <= RSTR, "reduction row");
         const float chi_part = acc[NRED - 1];    /* FP32 chi^2 term of this lane */
+        float fs[NPAIR];
 #pragma unroll
-        for (int e = 0; e < NRED; ++e) acc[e] += __shfl_xor(acc[e], HL, 64);
+        for (int j = 0; j < NPAIR; ++j)
+            fs[j] = fold_pair<HL>(acc[2 * j], (2 * j + 1 < NSUM) ? acc[2 * j + 1] : 0.f);
         __syncthreads();
-        if (k0 < HL) {
-            float4 *rrow = reinterpret_cast<float4 *>(red + k0 * RSTR);
+        {
+            float4 *rrow = reinterpret_cast<float4 *>(red + (k0 % HL) * RSTR + (k0 / HL) * HALF);
 #pragma unroll
-            for (int j = 0; j < (NRED + 3) / 4; ++j)
-                rrow[j] = make_float4((4 * j < NRED) ? acc[4 * j] : 0.f,
-                                      (4 * j + 1 < NRED) ? acc[4 * j + 1] : 0.f,
-                                      (4 * j + 2 < NRED) ? acc[4 * j + 2] : 0.f,
-                                      (4 * j + 3 < NRED) ? acc[4 * j + 3] : 0.f);
+            for (int j = 0; j < HALF / 4; ++j)
+                rrow[j] = make_float4((4 * j < NPAIR) ? fs[4 * j] : 0.f,
+                                      (4 * j + 1 < NPAIR) ? fs[4 * j + 1] : 0.f,
+                                      (4 * j + 2 < NPAIR) ? fs[4 * j + 2] : 0.f,
+                                      (4 * j + 3 < NPAIR) ? fs[4 * j + 3] : 0.f);
         }
         __syncthreads();
-        for (int e = k0; e < NRED - 1; e += LPP) {
+        for (int e = k0; e < NSUM; e += LPP) {
+            const float *col = red + (e & 1) * HALF + (e >> 1);
             float sum = 0.f;
 #pragma unroll 8
-            for (int l = 0; l < HL; ++l) sum += red[l * RSTR + e];
+            for (int l = 0; l < HL; ++l) sum += col[l * RSTR];
             /* rows and columns of fixed parameters are zero */
             const int pp_ = (e < NA) ? (P11 ? kPairs11.p[e] : kPairs9.p[e]) : e - NA;
             const int qq_ = (e < NA) ? (P11 ? kPairs11.q[e] : kPairs9.q[e]) : e - NA;
