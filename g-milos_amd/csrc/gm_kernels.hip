@@ -465,6 +465,49 @@ __device__ __forceinline__ float fold_pair(float a, float b)
     return __uint_as_float(lo) + __uint_as_float(hi);
 }
 
+/* Sum over the W-lane group (W = 16, 32 or 64), valid in the group's first
+ * lane (in every lane of its even 16-lane rows): four DPP steps give every
+ * lane its 16-lane row's sum (partners l^8 by row rotation, l^7 by the
+ * half-row mirror, l^2 and l^1 by quad permutations: together they span all
+ * sixteen), then the rows are folded with lane swaps -- VALU instructions
+ * only, where __shfl_xor would issue a ds_bpermute per step. */
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) { return __uint_as_float(dpp_u32<CTRL>(__float_as_uint(v))); }
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v)
+{
+    return __hiloint2double((int)dpp_u32<CTRL>((uint32_t)__double2hiint(v)),
+                            (int)dpp_u32<CTRL>((uint32_t)__double2loint(v)));
+}
+__device__ __forceinline__ float upper_row_partner(float v, bool half32)
+{
+    const auto sw = half32 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false)
+                           : __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(sw[1]);
+}
+__device__ __forceinline__ double upper_row_partner(double v, bool half32)
+{
+    return __hiloint2double(__float_as_int(upper_row_partner(__int_as_float(__double2hiint(v)), half32)),
+                            __float_as_int(upper_row_partner(__int_as_float(__double2loint(v)), half32)));
+}
+template <typename T, int W>
+__device__ __forceinline__ T group_sum_first(T v)
+{
+    static_assert(W == 16 || W == 32 || W == 64, "group width");
+    v += dpp_move<0x128>(v);            /* row_ror:8       -> l ^ 8       */
+    v += dpp_move<0x141>(v);            /* row_half_mirror -> l ^ 7       */
+    v += dpp_move<0x4E>(v);             /* quad_perm 2301  -> l ^ 2       */
+    v += dpp_move<0xB1>(v);             /* quad_perm 1032  -> l ^ 1       */
+    if constexpr (W >= 32) v += upper_row_partner(v, false);
+    if constexpr (W == 64) v += upper_row_partner(v, true);
+    return v;
+}
+
 template <typename T, int W>
 __device__ __forceinline__ T group_sum(T v)
 {
@@ -1094,9 +1137,9 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
             }
         }
         if (MIXED) {
-            chi = group_sum<double, LPP>(chi);
+            chi = group_sum_first<double, LPP>(chi);
         } else {
-            const float c = group_sum<float, LPP>(chi_part);
+            const float c = group_sum_first<float, LPP>(chi_part);
             chi = (double)(c / (float)nfree);
         }
         if (MIXED) chi = chi / (double)nfree;
@@ -1261,9 +1304,9 @@ __global__ __launch_bounds__(64) void k_synth(
             }
         }
         if (MIXED) {
-            chi = group_sum<double, LPP>(chi) / (double)nfree;
+            chi = group_sum_first<double, LPP>(chi) / (double)nfree;
         } else {
-            chif = group_sum<float, LPP>(chif);
+            chif = group_sum_first<float, LPP>(chif);
             chi = (double)(chif / (float)nfree);
         }
         if (has_chi && valid && k0 == 0) chi_out[pix] = chi;
