@@ -785,19 +785,19 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
     if (mfma) ptab[lane] = psf_tab_entry(S, lane);
 
     /* grid-stride loop over pixel pairs; the list entry of the next pair
-     * is fetched one iteration ahead, so only the pixel's own records are
-     * waited for at the top of an iteration */
+     * is loaded one iteration ahead and kept as loaded, 32 bits wide:
+     * widened to 64 bits at once, it would be waited for right there */
     const int64_t stride = (int64_t)gridDim.x * PPW;
-    auto fetch_pix = [&](int64_t b) -> int64_t {
-        const int64_t use = (b + h < cnt) ? b + h : b;
-        return has_list ? (int64_t)list[use] : use;
+    auto fetch_entry = [&](int64_t b) -> int32_t {
+        return list[(b + h < cnt) ? b + h : b];
     };
-    int64_t pix_next = ((int64_t)blockIdx.x * PPW < cnt) ? fetch_pix((int64_t)blockIdx.x * PPW) : 0;
+    int32_t entry_next =
+        (has_list && (int64_t)blockIdx.x * PPW < cnt) ? fetch_entry((int64_t)blockIdx.x * PPW) : 0;
     for (int64_t base = (int64_t)blockIdx.x * PPW; base < cnt; base += stride) {
         const int64_t item = base + h;
         const bool valid = item < cnt;
-        const int64_t pix = pix_next;
-        if (base + stride < cnt) pix_next = fetch_pix(base + stride);
+        const int64_t pix = has_list ? (int64_t)entry_next : (valid ? item : base);
+        if (has_list && base + stride < cnt) entry_next = fetch_entry(base + stride);
         PixPre pp;
         if (has_pre) {
             pp = load_pre(pre, pix);
@@ -1195,19 +1195,19 @@ __global__ __launch_bounds__(64) void k_synth(
         if (r < padlo || r >= padlo + n) conv[r] = make_float4(0.f, 0.f, 0.f, 0.f);
 
     /* grid-stride loop over pixel pairs; the list entry of the next pair
-     * is fetched one iteration ahead, so only the pixel's own records are
-     * waited for at the top of an iteration */
+     * is loaded one iteration ahead and kept as loaded, 32 bits wide:
+     * widened to 64 bits at once, it would be waited for right there */
     const int64_t stride = (int64_t)gridDim.x * PPW;
-    auto fetch_pix = [&](int64_t b) -> int64_t {
-        const int64_t use = (b + h < cnt) ? b + h : b;
-        return has_list ? (int64_t)list[use] : use;
+    auto fetch_entry = [&](int64_t b) -> int32_t {
+        return list[(b + h < cnt) ? b + h : b];
     };
-    int64_t pix_next = ((int64_t)blockIdx.x * PPW < cnt) ? fetch_pix((int64_t)blockIdx.x * PPW) : 0;
+    int32_t entry_next =
+        (has_list && (int64_t)blockIdx.x * PPW < cnt) ? fetch_entry((int64_t)blockIdx.x * PPW) : 0;
     for (int64_t base = (int64_t)blockIdx.x * PPW; base < cnt; base += stride) {
         const int64_t item = base + h;
         const bool valid = item < cnt;
-        const int64_t pix = pix_next;
-        if (base + stride < cnt) pix_next = fetch_pix(base + stride);
+        const int64_t pix = has_list ? (int64_t)entry_next : (valid ? item : base);
+        if (has_list && base + stride < cnt) entry_next = fetch_entry(base + stride);
         PixPre pp;
         if (has_pre) {
             pp = load_pre(pre, pix);
