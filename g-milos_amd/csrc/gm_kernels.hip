@@ -201,7 +201,7 @@ __device__ __forceinline__ PixPre make_pre(const double *m)
  * Voigt derivatives are summed over the sub-components of each type.        */
 template <bool RF>
 __device__ __forceinline__ void eval_lambda(const gm_dsetup *__restrict__ S,
-                                            const PixPre &pp, int k, float *out)
+                                            const PixPre &pp, int k, float *out, float dl0)
 {
     /* profiles and their derivatives summed over lines and sub-components
      * per component type t = 0 pi, 1 sigma_b, 2 sigma_r; the first
@@ -239,13 +239,17 @@ __device__ __forceinline__ void eval_lambda(const gm_dsetup *__restrict__ S,
             }
         }
     };
-    const int nlines = S->nlines;
-    for (int L = 0; L < nlines; ++L) {
+    /* one line; the first is peeled with its lambda - lambda0 from the
+     * caller (dl0 = S->dl[0][k], read once per kernel), so that a
+     * single-line set-up issues no vector load here: a load waited for
+     * in the middle of the pixel would also wait for every memory
+     * operation issued before it */
+    auto line = [&](int L, float dlk) {
         const float l0 = S->l0[L], fo = S->fo[L];
         const float ulos = pp.vlos * l0 * pp.inv_dopp * (1.f / kVLIGHT);
         const float parcial = l0 * l0 * pp.inv_dopp * kCTE;
         const float auxCte = -l0 * pp.inv_dopp * (1.f / kVLIGHT);
-        const float ub = S->dl[L][k] * pp.inv_dopp - ulos;
+        const float ub = dlk * pp.inv_dopp - ulos;
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
             const int nc = S->ncomp[L][t];
@@ -269,7 +273,10 @@ __device__ __forceinline__ void eval_lambda(const gm_dsetup *__restrict__ S,
             }
             for (int i = i0; i < nc; ++i) component(L, t, i, ub, parcial, auxCte, fo, false);
         }
-    }
+    };
+    line(0, dl0);
+    const int nlines = S->nlines;
+    for (int L = 1; L < nlines; ++L) line(L, S->dl[L][k]);
     const float E0_2 = pp.E0_2;
     const float parcial1 = fi[1] + fi[2];
     const float parcial2 = E0_2 * (fi[0] - 0.5f * parcial1);
@@ -434,6 +441,26 @@ __device__ __forceinline__ void load_model(const double *__restrict__ models,
 #pragma unroll
     for (int f = 0; f < GM_NMODEL; ++f) m[f] = models[f * fstride + pix * pstride];
 }
+
+/* Global -> LDS copy of 16 bytes per active lane (global_load_lds_dwordx4):
+ * lane L's bytes land at LDS byte address base + 16 L (M0 holds the base;
+ * checked on the device: lanes 0-4 and 32-36 of a wave fill slots 0-4 and
+ * 32-36).  Written as inline assembly: issued through the builtin, the
+ * compiler makes every later LDS access wait for the copy, which would
+ * stall the staging writes of the pixel being computed.  The copy has no
+ * register destination; the reader waits for it with lds_copy_wait(). */
+__device__ __forceinline__ void lds_copy16(const float *src, const float4 *dst_lds)
+{
+    const uint32_t base = (uint32_t)reinterpret_cast<uintptr_t>(
+        (__attribute__((address_space(3))) const void *)dst_lds);
+    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off"
+                 :
+                 : "s"(base), "v"(src)
+                 : "memory", "m0");
+}
+/* s_waitcnt vmcnt(0): every vector memory operation of the wave, the LDS
+ * copies included, has completed */
+__device__ __forceinline__ void lds_copy_wait() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 /* Reference continuum rule for the I-variant of the PSF: the larger end
  * sample (mil_sinrf.cu:120-123, me_der.cu:171-174).                         */
@@ -784,27 +811,86 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
     float *ptab = reinterpret_cast<float *>(lds4 + PPW * region4);
     if (mfma) ptab[lane] = psf_tab_entry(S, lane);
 
-    /* grid-stride loop over pixel pairs; the list entry of the next pair
-     * is loaded one iteration ahead and kept as loaded, 32 bits wide:
-     * widened to 64 bits at once, it would be waited for right there */
+    /* per-lane set-up values that are the same for every pixel, read once:
+     * inside the pixel loop each is a vector load waited for at once, and
+     * that wait also waits for every memory operation issued before it */
+    float dl0[NLPL], edge_k[NLPL];
+#pragma unroll
+    for (int s = 0; s < NLPL; ++s) {
+        const int k = k0 + s * LPP;
+        dl0[s] = S->dl[0][k < n ? k : n - 1];
+        edge_k[s] = S->psf_edge[k < n ? k : 0];
+    }
+    /* normal-equation entries this lane writes (e = k0, k0 + LPP, ...):
+     * bit i set when entry k0 + i LPP pairs two free parameters */
+    uint32_t keep_e = 0;
+    for (int e = k0, it = 0; e < NA + NP; e += LPP, ++it) {
+        const int pp_ = (e < NA) ? (P11 ? kPairs11.p[e] : kPairs9.p[e]) : e - NA;
+        const int qq_ = (e < NA) ? (P11 ? kPairs11.q[e] : kPairs9.q[e]) : e - NA;
+        keep_e |= ((fm >> pp_) & (fm >> qq_) & 1u) << it;
+    }
+
+    /* grid-stride loop over pixel pairs; list entries are loaded ahead and
+     * kept as loaded, 32 bits wide (widened to 64 bits at once, a load
+     * would be waited for right there).  REC: the constants record of the
+     * next pair is copied global -> LDS (global_load_lds_dwordx4, lanes
+     * 0-4 of each pixel, one float4 each, to LDS slot `lane` of the other
+     * stage) while this pair is computed, so its memory latency is off the
+     * critical path; this needs the list two pairs ahead.                 */
     const int64_t stride = (int64_t)gridDim.x * PPW;
     auto fetch_entry = [&](int64_t b) -> int32_t {
         return list[(b + h < cnt) ? b + h : b];
     };
-    int32_t entry_next =
-        (has_list && (int64_t)blockIdx.x * PPW < cnt) ? fetch_entry((int64_t)blockIdx.x * PPW) : 0;
-    for (int64_t base = (int64_t)blockIdx.x * PPW; base < cnt; base += stride) {
+    constexpr int PFS = (PPW - 1) * LPP + GM_PRE_REC / 4;   /* slots per stage */
+    __shared__ float4 pfb[REC ? 2 * PFS : 1];
+    auto prefetch_pre = [&](int32_t e, int stage) {
+        if (k0 < GM_PRE_REC / 4)
+            lds_copy16(pre + (int64_t)e * GM_PRE_REC + 4 * k0, pfb + stage * PFS);
+    };
+    const int64_t first = (int64_t)blockIdx.x * PPW;
+    int32_t entry_next = (has_list && first < cnt) ? fetch_entry(first) : 0;
+    int32_t entry_after = 0;
+    int stage = 0;
+    if constexpr (REC) {
+        if (first + stride < cnt) entry_after = fetch_entry(first + stride);
+        if (first < cnt) prefetch_pre(entry_next, 0);
+    }
+    for (int64_t base = first; base < cnt; base += stride) {
         const int64_t item = base + h;
         const bool valid = item < cnt;
         const int64_t pix = has_list ? (int64_t)entry_next : (valid ? item : base);
-        if (has_list && base + stride < cnt) entry_next = fetch_entry(base + stride);
         PixPre pp;
-        if (has_pre) {
-            pp = load_pre(pre, pix);
-        } else {
-            double m[GM_NMODEL];
-            load_model(models, mfs, mps, pix, m);
-            pp = make_pre(m);
+        if constexpr (REC) {
+            /* the copy into this stage, issued an iteration ago (waited for
+             * before anything else is issued: the wait covers every
+             * outstanding vector memory operation) */
+            lds_copy_wait();
+            float *v = reinterpret_cast<float *>(&pp);
+            const float4 *r = pfb + stage * PFS + h * LPP;
+#pragma unroll
+            for (int q = 0; q < GM_PRE_REC / 4; ++q) {
+                const float4 x = r[q];
+                v[4 * q] = x.x;
+                v[4 * q + 1] = x.y;
+                v[4 * q + 2] = x.z;
+                if (q < GM_PRE_REC / 4 - 1) v[4 * q + 3] = x.w;
+            }
+            const int32_t e2 = (base + 2 * stride < cnt) ? fetch_entry(base + 2 * stride) : 0;
+            if (base + stride < cnt) prefetch_pre(entry_after, stage ^ 1);
+            entry_next = entry_after;
+            entry_after = e2;
+            stage ^= 1;
+        } else if (has_list && base + stride < cnt) {
+            entry_next = fetch_entry(base + stride);
+        }
+        if constexpr (!REC) {
+            if (has_pre) {
+                pp = load_pre(pre, pix);
+            } else {
+                double m[GM_NMODEL];
+                load_model(models, mfs, mps, pix, m);
+                pp = make_pre(m);
+            }
         }
         /* the accepted trial becomes the best state here rather than in
          * k_decide: lanes 0-10 load its model and constants now and store
@@ -837,7 +923,7 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
         for (int s = 0; s < NLPL; ++s) {
             const int k = k0 + s * LPP;
             float v[GM_SLOTS];
-            eval_lambda<true>(S, pp, k < n ? k : n - 1, v);
+            eval_lambda<true>(S, pp, k < n ? k : n - 1, v, dl0[s]);
             /* 30 of the 36 values: d/dphi of I and V is zero and the S1
              * responses follow from the convolved synthesis (below) */
             float c[32];
@@ -914,7 +1000,7 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
             } else if (MFMA_OK && mfma) {
                 psf_mfma32(reinterpret_cast<const float *>(conv + padlo * 9) + k0, ptab + 31 - k0, z);
                 /* I-variant (Ic trick) on the Stokes-I block, as below */
-                const float edge = S->psf_edge[k < n ? k : 0];
+                const float edge = edge_k[s];
                 const float4 *r0 = conv + padlo * 9;
                 const float4 *rn = conv + (padlo + n - 1) * 9;
                 const float4 a0 = r0[0], a1 = r0[1];
@@ -955,7 +1041,7 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
                 }
                 /* I-variant (Ic trick) on the Stokes-I block: adds
                  * Ic * (1 - sum of in-range taps), convolution.cu:200-241 */
-                const float edge = S->psf_edge[k < n ? k : 0];
+                const float edge = edge_k[s];
                 const float4 *r0 = conv + padlo * 9;
                 const float4 *rn = conv + (padlo + n - 1) * 9;
                 const float4 a0 = r0[0], a1 = r0[1];
@@ -1119,15 +1205,13 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
                                       (4 * j + 3 < NPAIR) ? fs[4 * j + 3] : 0.f);
         }
         __syncthreads();
-        for (int e = k0; e < NSUM; e += LPP) {
+        for (int e = k0, it = 0; e < NSUM; e += LPP, ++it) {
             const float *col = red + (e & 1) * HALF + (e >> 1);
             float sum = 0.f;
 #pragma unroll 8
             for (int l = 0; l < HL; ++l) sum += col[l * RSTR];
             /* rows and columns of fixed parameters are zero */
-            const int pp_ = (e < NA) ? (P11 ? kPairs11.p[e] : kPairs9.p[e]) : e - NA;
-            const int qq_ = (e < NA) ? (P11 ? kPairs11.q[e] : kPairs9.q[e]) : e - NA;
-            sum = ((fm >> pp_) & (fm >> qq_) & 1u) ? sum : 0.f;
+            sum = ((keep_e >> it) & 1u) ? sum : 0.f;
             if (valid) {
                 if (e < NA) {
                     if (has_alpha) alpha[e * aes + pix * aps] = sum;
@@ -1194,6 +1278,17 @@ __global__ __launch_bounds__(64) void k_synth(
     for (int r = k0; r < rows; r += LPP)
         if (r < padlo || r >= padlo + n) conv[r] = make_float4(0.f, 0.f, 0.f, 0.f);
 
+    /* per-lane set-up values that are the same for every pixel, read once:
+     * inside the pixel loop each is a vector load waited for at once, and
+     * that wait also waits for every memory operation issued before it */
+    float dl0[NLPL], edge_k[NLPL];
+#pragma unroll
+    for (int s = 0; s < NLPL; ++s) {
+        const int k = k0 + s * LPP;
+        dl0[s] = S->dl[0][k < n ? k : n - 1];
+        edge_k[s] = S->psf_edge[k < n ? k : 0];
+    }
+
     /* grid-stride loop over pixel pairs; the list entry of the next pair
      * is loaded one iteration ahead and kept as loaded, 32 bits wide:
      * widened to 64 bits at once, it would be waited for right there */
@@ -1230,7 +1325,7 @@ __global__ __launch_bounds__(64) void k_synth(
         for (int s = 0; s < NLPL; ++s) {
             const int k = k0 + s * LPP;
             float v[GM_SLOTS];
-            eval_lambda<false>(S, pp, k < n ? k : n - 1, v);
+            eval_lambda<false>(S, pp, k < n ? k : n - 1, v, dl0[s]);
             if (k < n) conv[padlo + k] = make_float4(v[0], v[9], v[18], v[27]);
         }
         __syncthreads();
@@ -1275,7 +1370,7 @@ __global__ __launch_bounds__(64) void k_synth(
                     y.z += tap * x.z;
                     y.w += tap * x.w;
                 }
-                const float edge = S->psf_edge[k < n ? k : 0];
+                const float edge = edge_k[s];
                 y.x += end_max(conv[padlo].x, conv[padlo + n - 1].x) * edge;
             } else {
                 y = conv[padlo + k];
