@@ -462,6 +462,76 @@ __device__ __forceinline__ void lds_copy16(const float *src, const float4 *dst_l
  * copies included, has completed */
 __device__ __forceinline__ void lds_copy_wait() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
+/* Pixel pairs of a grid-stride loop (PPW = 64 / LPP pixels per wave).  List
+ * entries are loaded ahead and kept 32 bits wide: widened to 64 bits at
+ * once, a load would be waited for right there.  With REC (the LM loop's
+ * per-pixel records) the constants record of the next pair is also copied
+ * global -> LDS while this pair is computed (lanes 0-4 of each pixel, one
+ * float4 each, to slot `lane` of the other stage of `buf`), so its memory
+ * latency is off the critical path; that needs the list two pairs ahead. */
+template <int LPP, bool REC>
+struct PairFeed {
+    static constexpr int PPW = 64 / LPP;
+    static constexpr int PFS = (PPW - 1) * LPP + GM_PRE_REC / 4;   /* LDS slots per stage */
+    const int32_t *list;
+    const float *pre;
+    float4 *buf;
+    int64_t cnt, stride;
+    int h, k0;
+    bool has_list;
+    int32_t next = 0, after = 0;
+    int stage = 0;
+
+    __device__ int32_t fetch(int64_t b) const { return list[(b + h < cnt) ? b + h : b]; }
+    __device__ void copy(int32_t e, int st) const
+    {
+        if (k0 < GM_PRE_REC / 4) lds_copy16(pre + (int64_t)e * GM_PRE_REC + 4 * k0, buf + st * PFS);
+    }
+    __device__ void start(int64_t first)
+    {
+        if (has_list && first < cnt) next = fetch(first);
+        if constexpr (REC) {
+            if (first + stride < cnt) after = fetch(first + stride);
+            if (first < cnt) copy(next, 0);
+        }
+    }
+    /* this pair's pixel: the list entry, or the item itself */
+    __device__ int64_t pixel(int64_t base) const
+    {
+        return has_list ? (int64_t)next : ((base + h < cnt) ? base + h : base);
+    }
+    /* REC: this pair's constants record; then the loads for the next pairs.
+     * The copy into this stage was issued an iteration ago; it is waited
+     * for before anything else is issued (the wait covers every outstanding
+     * vector memory operation). */
+    __device__ PixPre advance_rec(int64_t base)
+    {
+        lds_copy_wait();
+        PixPre pp;
+        float *v = reinterpret_cast<float *>(&pp);
+        const float4 *r = buf + stage * PFS + h * LPP;
+#pragma unroll
+        for (int q = 0; q < GM_PRE_REC / 4; ++q) {
+            const float4 x = r[q];
+            v[4 * q] = x.x;
+            v[4 * q + 1] = x.y;
+            v[4 * q + 2] = x.z;
+            if (q < GM_PRE_REC / 4 - 1) v[4 * q + 3] = x.w;
+        }
+        const int32_t e2 = (base + 2 * stride < cnt) ? fetch(base + 2 * stride) : 0;
+        if (base + stride < cnt) copy(after, stage ^ 1);
+        next = after;
+        after = e2;
+        stage ^= 1;
+        return pp;
+    }
+    /* without records: the next list entry */
+    __device__ void advance(int64_t base)
+    {
+        if (has_list && base + stride < cnt) next = fetch(base + stride);
+    }
+};
+
 /* Reference continuum rule for the I-variant of the PSF: the larger end
  * sample (mil_sinrf.cu:120-123, me_der.cu:171-174).                         */
 __device__ __forceinline__ float end_max(float x0, float xn) { return (x0 > xn) ? x0 : xn; }
@@ -830,60 +900,22 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
         keep_e |= ((fm >> pp_) & (fm >> qq_) & 1u) << it;
     }
 
-    /* grid-stride loop over pixel pairs; list entries are loaded ahead and
-     * kept as loaded, 32 bits wide (widened to 64 bits at once, a load
-     * would be waited for right there).  REC: the constants record of the
-     * next pair is copied global -> LDS (global_load_lds_dwordx4, lanes
-     * 0-4 of each pixel, one float4 each, to LDS slot `lane` of the other
-     * stage) while this pair is computed, so its memory latency is off the
-     * critical path; this needs the list two pairs ahead.                 */
+    /* grid-stride loop over pixel pairs (PairFeed: list entries and, for
+     * REC, constants records prefetched) */
     const int64_t stride = (int64_t)gridDim.x * PPW;
-    auto fetch_entry = [&](int64_t b) -> int32_t {
-        return list[(b + h < cnt) ? b + h : b];
-    };
-    constexpr int PFS = (PPW - 1) * LPP + GM_PRE_REC / 4;   /* slots per stage */
-    __shared__ float4 pfb[REC ? 2 * PFS : 1];
-    auto prefetch_pre = [&](int32_t e, int stage) {
-        if (k0 < GM_PRE_REC / 4)
-            lds_copy16(pre + (int64_t)e * GM_PRE_REC + 4 * k0, pfb + stage * PFS);
-    };
+    __shared__ float4 pfb[REC ? 2 * PairFeed<LPP, REC>::PFS : 1];
+    PairFeed<LPP, REC> feed{list, pre, pfb, cnt, stride, h, k0, has_list};
     const int64_t first = (int64_t)blockIdx.x * PPW;
-    int32_t entry_next = (has_list && first < cnt) ? fetch_entry(first) : 0;
-    int32_t entry_after = 0;
-    int stage = 0;
-    if constexpr (REC) {
-        if (first + stride < cnt) entry_after = fetch_entry(first + stride);
-        if (first < cnt) prefetch_pre(entry_next, 0);
-    }
+    feed.start(first);
     for (int64_t base = first; base < cnt; base += stride) {
         const int64_t item = base + h;
         const bool valid = item < cnt;
-        const int64_t pix = has_list ? (int64_t)entry_next : (valid ? item : base);
+        const int64_t pix = feed.pixel(base);
         PixPre pp;
         if constexpr (REC) {
-            /* the copy into this stage, issued an iteration ago (waited for
-             * before anything else is issued: the wait covers every
-             * outstanding vector memory operation) */
-            lds_copy_wait();
-            float *v = reinterpret_cast<float *>(&pp);
-            const float4 *r = pfb + stage * PFS + h * LPP;
-#pragma unroll
-            for (int q = 0; q < GM_PRE_REC / 4; ++q) {
-                const float4 x = r[q];
-                v[4 * q] = x.x;
-                v[4 * q + 1] = x.y;
-                v[4 * q + 2] = x.z;
-                if (q < GM_PRE_REC / 4 - 1) v[4 * q + 3] = x.w;
-            }
-            const int32_t e2 = (base + 2 * stride < cnt) ? fetch_entry(base + 2 * stride) : 0;
-            if (base + stride < cnt) prefetch_pre(entry_after, stage ^ 1);
-            entry_next = entry_after;
-            entry_after = e2;
-            stage ^= 1;
-        } else if (has_list && base + stride < cnt) {
-            entry_next = fetch_entry(base + stride);
-        }
-        if constexpr (!REC) {
+            pp = feed.advance_rec(base);
+        } else {
+            feed.advance(base);
             if (has_pre) {
                 pp = load_pre(pre, pix);
             } else {
@@ -1289,27 +1321,29 @@ __global__ __launch_bounds__(64) void k_synth(
         edge_k[s] = S->psf_edge[k < n ? k : 0];
     }
 
-    /* grid-stride loop over pixel pairs; the list entry of the next pair
-     * is loaded one iteration ahead and kept as loaded, 32 bits wide:
-     * widened to 64 bits at once, it would be waited for right there */
+    /* grid-stride loop over pixel pairs (PairFeed: list entries and, for
+     * REC, constants records prefetched) */
     const int64_t stride = (int64_t)gridDim.x * PPW;
-    auto fetch_entry = [&](int64_t b) -> int32_t {
-        return list[(b + h < cnt) ? b + h : b];
-    };
-    int32_t entry_next =
-        (has_list && (int64_t)blockIdx.x * PPW < cnt) ? fetch_entry((int64_t)blockIdx.x * PPW) : 0;
-    for (int64_t base = (int64_t)blockIdx.x * PPW; base < cnt; base += stride) {
+    __shared__ float4 pfb[REC ? 2 * PairFeed<LPP, REC>::PFS : 1];
+    PairFeed<LPP, REC> feed{list, pre, pfb, cnt, stride, h, k0, has_list};
+    const int64_t first = (int64_t)blockIdx.x * PPW;
+    feed.start(first);
+    for (int64_t base = first; base < cnt; base += stride) {
         const int64_t item = base + h;
         const bool valid = item < cnt;
-        const int64_t pix = has_list ? (int64_t)entry_next : (valid ? item : base);
-        if (has_list && base + stride < cnt) entry_next = fetch_entry(base + stride);
+        const int64_t pix = feed.pixel(base);
         PixPre pp;
-        if (has_pre) {
-            pp = load_pre(pre, pix);
+        if constexpr (REC) {
+            pp = feed.advance_rec(base);
         } else {
-            double m[GM_NMODEL];
-            load_model(models, mfs, mps, pix, m);
-            pp = make_pre(m);
+            feed.advance(base);
+            if (has_pre) {
+                pp = load_pre(pre, pix);
+            } else {
+                double m[GM_NMODEL];
+                load_model(models, mfs, mps, pix, m);
+                pp = make_pre(m);
+            }
         }
         /* observed samples of this lane's wavelengths, loaded before the
          * synthesis so that their latency is hidden behind it */
