@@ -856,6 +856,7 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
     const bool has_alpha = REC || alpha != nullptr;
     const bool has_beta = REC || beta != nullptr;
     const bool has_rf = !REC && rf_out != nullptr;
+    const bool want_chi = chi_out != nullptr;
     extern __shared__ float4 lds4[];
     const int lane = threadIdx.x;
     const int h = lane / LPP;
@@ -1149,11 +1150,6 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
                 for (int st = 0; st < 4 && has_obs; ++st) {
                     const float ws = (INIT1 && !in_grid) ? 0.f : S->wsig[st];
                     const bool first = INIT1 && st == 0;
-                    if (MIXED)
-                        chi += (double)ws * ((double)r[st] * (double)r[st]);
-                    else
-                        acc[NRED - 1] = first ? (r[st] * r[st]) * ws
-                                              : fmaf(r[st] * r[st], ws, acc[NRED - 1]);
                     int e = 0;
 #pragma unroll
                     for (int p = 0; p < NP; ++p) {
@@ -1175,6 +1171,20 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
                             continue;
                         }
                         acc[NA + p] = first ? wr * g[st][p] : fmaf(wr, g[st][p], acc[NA + p]);
+                    }
+                }
+                /* chi^2 only where it is asked for: the LM loop's calls
+                 * take it from the trial synthesis */
+                if (want_chi) {
+#pragma unroll
+                    for (int st = 0; st < 4 && has_obs; ++st) {
+                        const float ws = (INIT1 && !in_grid) ? 0.f : S->wsig[st];
+                        if (MIXED)
+                            chi += (double)ws * ((double)r[st] * (double)r[st]);
+                        else
+                            acc[NRED - 1] = (INIT1 && st == 0)
+                                                ? (r[st] * r[st]) * ws
+                                                : fmaf(r[st] * r[st], ws, acc[NRED - 1]);
                     }
                 }
                 if (in_grid && synth_out && valid) {
@@ -1252,14 +1262,15 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
                 }
             }
         }
-        if (MIXED) {
-            chi = group_sum_first<double, LPP>(chi);
-        } else {
-            const float c = group_sum_first<float, LPP>(chi_part);
-            chi = (double)(c / (float)nfree);
+        if (want_chi) {
+            if (MIXED) {
+                chi = group_sum_first<double, LPP>(chi) / (double)nfree;
+            } else {
+                const float c = group_sum_first<float, LPP>(chi_part);
+                chi = (double)(c / (float)nfree);
+            }
+            if (valid && k0 == 0) chi_out[pix] = chi;
         }
-        if (MIXED) chi = chi / (double)nfree;
-        if (chi_out && valid && k0 == 0) chi_out[pix] = chi;
         if (copy_rec) {
             if (k0 < GM_MODEL_REC / 2)
                 reinterpret_cast<float4 *>(model_dst + pix * GM_MODEL_REC)[k0] = rec;
