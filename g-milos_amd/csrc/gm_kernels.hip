@@ -317,22 +317,16 @@ __device__ __forceinline__ void eval_lambda(const gm_dsetup *__restrict__ S,
     d_ei[0] = etain * iE0; d_eq[0] = etaq * iE0; d_eu[0] = etau * iE0;
     d_ev[0] = etav * iE0; d_rq[0] = rhoq * iE0; d_ru[0] = rhou * iE0;
     d_rv[0] = rhov * iE0;
+    /* for B, vlos, dopp and a (j = 1..4) the seven derivatives are fixed
+     * combinations of the six per-type profile derivatives dfi[t], dshi[t]
+     * (me_der.cu:254-312):
+     *   d eta_I = A dfi0 + B (dfi1 + dfi2),  d eta_Q,U = (sc, ss) (dfi0 -
+     *   (dfi1 + dfi2) / 2),  d eta_V = E (dfi2 - dfi1), and the same for
+     *   rho_Q,U,V with dshi; they enter the chain rule below folded into its
+     *   coefficients */
     const float sinis_E0_2 = pp.sinis * E0_2, cosis_2_E0_2 = pp.cosis_2 * E0_2;
     const float sc = pp.sinis_cosa * E0_2, ss = pp.sinis_sina * E0_2;
     const float cosi_2_E0 = pp.cosi * E0_2;
-#pragma unroll
-    for (int j = 1; j < 5; ++j) {
-        const float dfisum = dfi[1][j - 1] + dfi[2][j - 1];
-        d_ei[j] = dfi[0][j - 1] * sinis_E0_2 + dfisum * cosis_2_E0_2;
-        const float aux = dfi[0][j - 1] - 0.5f * dfisum;
-        d_eq[j] = aux * sc;
-        d_eu[j] = aux * ss;
-        d_ev[j] = (dfi[2][j - 1] - dfi[1][j - 1]) * cosi_2_E0;
-        const float aux2 = dshi[0][j - 1] - 0.5f * (dshi[1][j - 1] + dshi[2][j - 1]);
-        d_rq[j] = aux2 * sc;
-        d_ru[j] = aux2 * ss;
-        d_rv[j] = (dshi[2][j - 1] - dshi[1][j - 1]) * cosi_2_E0;
-    }
     const float cosi_cosdi = pp.cosi * pp.cosdi * E0_2, sindi_E0_2 = pp.sindi * E0_2;
     const float cosdi_E0_2 = E0_2 * pp.cosdi;
     d_ei[5] = fi[0] * sindi_E0_2 + parcial1 * cosi_cosdi;
@@ -407,9 +401,9 @@ __device__ __forceinline__ void eval_lambda(const gm_dsetup *__restrict__ S,
             C[st][m] = (st == 0 && m >= 1 && m <= 3) ? -(v * cdt[m])     /* ca[0][1..3] = 0 */
                                                      : u * ca[st][m] - v * cdt[m];
     }
-    /* eta0 .. gamma: all seven absorption-matrix derivatives */
+    /* eta0 and gamma: all seven absorption-matrix derivatives */
 #pragma unroll
-    for (int p = 0; p < 6; ++p) {
+    for (int p = 0; p < 6; p += 5) {
         const float dm[7] = {d_ei[p], d_eq[p], d_eu[p], d_ev[p], d_rq[p], d_ru[p], d_rv[p]};
 #pragma unroll
         for (int st = 0; st < 4; ++st) {
@@ -417,6 +411,28 @@ __device__ __forceinline__ void eval_lambda(const gm_dsetup *__restrict__ S,
 #pragma unroll
             for (int m = 1; m < 7; ++m) o = fmaf(C[st][m], dm[m], o);
             out[st * 9 + 1 + p] = o;
+        }
+    }
+    /* B, vlos, dopp, a: out = sum_t D[t] dfi[t] + D[3 + t] dshi[t], with the
+     * combinations above folded into D once per wavelength (six products
+     * per response instead of seven after forming the seven derivatives) */
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+        const float q1 = fmaf(C[st][1], sc, C[st][2] * ss);   /* eta_Q,U */
+        const float q2 = fmaf(C[st][4], sc, C[st][5] * ss);   /* rho_Q,U */
+        const float h1 = fmaf(C[st][0], cosis_2_E0_2, -0.5f * q1);
+        const float h2 = -0.5f * q2;
+        const float e3 = C[st][3] * cosi_2_E0, e6 = C[st][6] * cosi_2_E0;
+        const float D[6] = {fmaf(C[st][0], sinis_E0_2, q1), h1 - e3, h1 + e3, q2, h2 - e6, h2 + e6};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float o = D[0] * dfi[0][j];
+            o = fmaf(D[1], dfi[1][j], o);
+            o = fmaf(D[2], dfi[2][j], o);
+            o = fmaf(D[3], dshi[0][j], o);
+            o = fmaf(D[4], dshi[1][j], o);
+            o = fmaf(D[5], dshi[2][j], o);
+            out[st * 9 + 2 + j] = o;
         }
     }
     /* phi changes eta_Q,U and rho_Q,U only, and d/dphi of I and V is zero
