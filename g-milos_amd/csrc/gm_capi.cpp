@@ -188,6 +188,7 @@ static gm_status build_dsetup(const gm_setup &s, gm_dsetup &d)
         d.weight[i] = (float)s.weight[i];
         d.wsig[i] = (float)(s.weight[i] / s.noise);
     }
+    d.wsig_equal = d.wsig[0] == d.wsig[1] && d.wsig[0] == d.wsig[2] && d.wsig[0] == d.wsig[3];
     d.mu = (float)s.mu;
     d.shi_scale = s.rf_reference_rules ? 1.0f : 2.0f;
     d.freemask = 0;

@@ -55,6 +55,7 @@ struct gm_dsetup {
     float   psf_edge[GM_MAX_LAMBDA];         /* 1 - sum of in-range taps    */
     float   weight[4];
     float   wsig[4];                         /* weight / noise              */
+    int32_t wsig_equal;                      /* all four wsig the same       */
     float   mu;
     float   shi_scale;                       /* 1 reference rule, 2 exact   */
     uint32_t freemask;                       /* bit p: RF slot p free (0-8) */

@@ -906,8 +906,16 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
     for (int s = 0; s < NLPL; ++s) {
         const int k = k0 + s * LPP;
         dl0[s] = S->dl[0][k < n ? k : n - 1];
-        edge_k[s] = S->psf_edge[k < n ? k : 0];
+        edge_k[s] = k < n ? S->psf_edge[k] : 0.f;
     }
+    /* equal Stokes weights (the usual case): the normal-equation products
+     * are summed unweighted and scaled once after the reduction.  A lane
+     * past the grid must then contribute zero by itself: its PSF taps, edge
+     * weight and S0 terms are scaled by zero (tapscale) -- the weighted sums
+     * give it a zero weight instead.  One wavelength per lane, VALU PSF. */
+    const bool uw = NLPL == 1 && XM == 0 && S->wsig_equal && !mfma;
+    const float tapscale = (uw && k0 >= n) ? 0.f : 1.f;
+    const float uwscale = uw ? S->wsig[0] : 1.f;
     /* normal-equation entries this lane writes (e = k0, k0 + LPP, ...):
      * bit i set when entry k0 + i LPP pairs two free parameters */
     uint32_t keep_e = 0;
@@ -1065,7 +1073,7 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
                 /* the first tap initialises the sums (dlo <= dhi whenever
                  * the PSF is on) */
                 {
-                    const float tap = S->psf_tap[0];
+                    const float tap = S->psf_tap[0] * tapscale;
                     const float4 *row = conv + (padlo + k + dlo) * 9;
 #pragma unroll
                     for (int j = 0; j < 8; ++j) {
@@ -1077,7 +1085,7 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
                     }
                 }
                 for (int d = dlo + 1; d <= dhi; ++d) {
-                    const float tap = S->psf_tap[d - dlo];
+                    const float tap = S->psf_tap[d - dlo] * tapscale;
                     const float4 *row = conv + (padlo + k + d) * 9;
 #pragma unroll
                     for (int j = 0; j < 8; ++j) {
@@ -1120,7 +1128,7 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
             for (int i = 0; i < 30; ++i) y[kStagedSlot[i]] = z[i];
             y[7] = 0.f;
             y[34] = 0.f;
-            y[8] = (y[0] - pp.S0 * s0c) * inv_s1;
+            y[8] = (y[0] - pp.S0 * (s0c * tapscale)) * inv_s1;
             y[17] = y[9] * inv_s1;
             y[26] = y[18] * inv_s1;
             y[35] = y[27] * inv_s1;
@@ -1155,40 +1163,47 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
 #pragma unroll
                     for (int p = 0; p < 9; ++p) {
                         const int js = staged_slot(p);
-                        g[st][p] = (js >= 0) ? y[st * 9 + js] : (st == 0 ? s0rf : 0.f);
+                        g[st][p] = (js >= 0) ? y[st * 9 + js] : (st == 0 ? s0rf * tapscale : 0.f);
                     }
                     if constexpr (P11) {
                         g[st][9] = zm[st];
                         g[st][10] = ra[st];
                     }
                 }
+                auto accumulate = [&](auto weighted) {
+                    constexpr bool W = decltype(weighted)::value;
 #pragma unroll
-                for (int st = 0; st < 4 && has_obs; ++st) {
-                    const float ws = (INIT1 && !in_grid) ? 0.f : S->wsig[st];
-                    const bool first = INIT1 && st == 0;
-                    int e = 0;
+                    for (int st = 0; st < 4 && has_obs; ++st) {
+                        const float ws = (INIT1 && !in_grid) ? 0.f : S->wsig[st];
+                        const bool first = INIT1 && st == 0;
+                        int e = 0;
 #pragma unroll
-                    for (int p = 0; p < NP; ++p) {
-                        const float wg = ws * g[st][p];
+                        for (int p = 0; p < NP; ++p) {
+                            const float wg = W ? ws * g[st][p] : g[st][p];
 #pragma unroll
-                        for (int q = p; q < NP; ++q, ++e) {
-                            if (!rf_nonzero(st, p) || !rf_nonzero(st, q)) {
-                                if (first) acc[e] = 0.f;
+                            for (int q = p; q < NP; ++q, ++e) {
+                                if (!rf_nonzero(st, p) || !rf_nonzero(st, q)) {
+                                    if (first) acc[e] = 0.f;
+                                    continue;
+                                }
+                                acc[e] = first ? wg * g[st][q] : fmaf(wg, g[st][q], acc[e]);
+                            }
+                        }
+                        const float wr = W ? ws * r[st] : r[st];
+#pragma unroll
+                        for (int p = 0; p < NP; ++p) {
+                            if (!rf_nonzero(st, p)) {
+                                if (first) acc[NA + p] = 0.f;
                                 continue;
                             }
-                            acc[e] = first ? wg * g[st][q] : fmaf(wg, g[st][q], acc[e]);
+                            acc[NA + p] = first ? wr * g[st][p] : fmaf(wr, g[st][p], acc[NA + p]);
                         }
                     }
-                    const float wr = ws * r[st];
-#pragma unroll
-                    for (int p = 0; p < NP; ++p) {
-                        if (!rf_nonzero(st, p)) {
-                            if (first) acc[NA + p] = 0.f;
-                            continue;
-                        }
-                        acc[NA + p] = first ? wr * g[st][p] : fmaf(wr, g[st][p], acc[NA + p]);
-                    }
-                }
+                };
+                if (uw)
+                    accumulate(std::false_type{});
+                else
+                    accumulate(std::true_type{});
                 /* chi^2 only where it is asked for: the LM loop's calls
                  * take it from the trial synthesis */
                 if (want_chi) {
@@ -1269,7 +1284,7 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
 #pragma unroll 8
             for (int l = 0; l < HL; ++l) sum += col[l * RSTR];
             /* rows and columns of fixed parameters are zero */
-            sum = ((keep_e >> it) & 1u) ? sum : 0.f;
+            sum = ((keep_e >> it) & 1u) ? sum * uwscale : 0.f;
             if (valid) {
                 if (e < NA) {
                     if (has_alpha) alpha[e * aes + pix * aps] = sum;
@@ -1345,7 +1360,7 @@ __global__ __launch_bounds__(64) void k_synth(
     for (int s = 0; s < NLPL; ++s) {
         const int k = k0 + s * LPP;
         dl0[s] = S->dl[0][k < n ? k : n - 1];
-        edge_k[s] = S->psf_edge[k < n ? k : 0];
+        edge_k[s] = k < n ? S->psf_edge[k] : 0.f;
     }
 
     /* grid-stride loop over pixel pairs (PairFeed: list entries and, for
