@@ -115,10 +115,19 @@ def test_response_functions_match_oracle(gm, orc, cuda, kind, psfname, rules):
     assert np.all(rf[:, :, 9, :] == 0)
 
 
-@pytest.mark.parametrize("precision,free", [(1, None), (0, None),
-                                            (1, ("eta0", "vlos", "dopp", "aa", "az", "S1"))])
-def test_normal_equations_match_oracle(gm, orc, cuda, precision, free):
+UNEQUAL_WEIGHTS = (1.0, 0.7, 0.8, 1.3)
+
+
+@pytest.mark.parametrize("precision,free,weights",
+                         [(1, None, None), (0, None, None),
+                          (1, ("eta0", "vlos", "dopp", "aa", "az", "S1"), None),
+                          (1, None, UNEQUAL_WEIGHTS), (0, None, UNEQUAL_WEIGHTS)])
+def test_normal_equations_match_oracle(gm, orc, cuda, precision, free, weights):
+    """Equal Stokes weights take the kernel's unweighted sums scaled after
+    the reduction; unequal ones the per-product weights."""
     kw = {"free": free} if free else {}
+    if weights:
+        kw["weights"] = weights
     p, o = pair(gm, orc, "single", ("gauss", 60.0), precision=precision, **kw)
     truth = gm.random_models(128, seed=13)
     obs = observations(orc, o, truth, seed=13)
@@ -391,6 +400,17 @@ def test_inversion_config1_psf_matches_oracle(gm, orc, cuda, precision, tail):
     dev = (np.abs(got["synth"] - sp_best) / ic).max(axis=(1, 2))
     print("synth vs synth(best): median %.2e, 99%% %.2e" % (np.median(dev), np.percentile(dev, 99)))
     assert np.median(dev) < 1e-5 and np.percentile(dev, 99) < 1e-2
+
+
+def test_inversion_unequal_weights_matches_oracle(gm, orc, cuda):
+    """Config-1 line and PSF with Stokes weights 1, 0.7, 0.8, 1.3 (the
+    response kernel's weighted normal-equation sums) on 1024 pixels."""
+    got, ref, truth, obs, ctx = _inversion_errors(gm, orc, cuda, "single", ("gauss", 60.0),
+                                                  1024, 200, weights=UNEQUAL_WEIGHTS)
+    o = pair(gm, orc, "single", ("gauss", 60.0), weights=UNEQUAL_WEIGHTS)[1]
+    check_inversion(orc, o, obs, got, ref, truth, "unequal weights")
+    rel_c2 = np.abs(got["chisqr"] - ref["chisqr"]) / ref["chisqr"]
+    assert np.median(rel_c2) < 1e-4
 
 
 def test_inversion_two_lines_matches_oracle(gm, orc, cuda):
