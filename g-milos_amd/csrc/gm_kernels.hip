@@ -2509,7 +2509,17 @@ __global__ __launch_bounds__(64) void k_step_solve(const float *__restrict__ alp
  * next active list and the list of accepted pixels that need new normal
  * equations, with one atomic per 1024-pixel workgroup and list.            */
 template <bool MIXED>
-__global__ __launch_bounds__(1024) void k_decide(const gm_dsetup *__restrict__ S, gm_state st,
+/* Workgroup size and grid cap of k_decide.  Per 1M-pixel inversion the
+ * decisions take the same 2.5 ms (with the other small kernels) with caps of
+ * 256, 512 or 1024 workgroups of 1024; workgroups of 512 or 256 (more of
+ * them, so more list atomics) take 3.1 and 4.6 ms. */
+#ifndef GM_DECIDE_THREADS
+#define GM_DECIDE_THREADS 1024
+#endif
+#ifndef GM_DECIDE_BLOCKS
+#define GM_DECIDE_BLOCKS 512
+#endif
+__global__ __launch_bounds__(GM_DECIDE_THREADS) void k_decide(const gm_dsetup *__restrict__ S, gm_state st,
                                                  const int32_t *__restrict__ list,
                                                  const int32_t *__restrict__ count,
                                                  int32_t *__restrict__ next_list,
@@ -2522,13 +2532,14 @@ __global__ __launch_bounds__(1024) void k_decide(const gm_dsetup *__restrict__ S
                                                  int defer_copy)
 {
     using T = typename std::conditional<MIXED, double, float>::type;
-    __shared__ int wc[16], wa[16], wd[16];
+    __shared__ int wc[GM_DECIDE_THREADS / 64], wa[GM_DECIDE_THREADS / 64], wd[GM_DECIDE_THREADS / 64];
     __shared__ int base_c, base_a, base_d;
     if (blockIdx.x == 0 && threadIdx.x == 0 && reset_fb) *reset_fb = 0;
     const int64_t cnt = *count;
     const int64_t cap = st.cap;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int64_t base = (int64_t)blockIdx.x * 1024; base < cnt; base += (int64_t)gridDim.x * 1024) {
+    for (int64_t base = (int64_t)blockIdx.x * GM_DECIDE_THREADS; base < cnt;
+         base += (int64_t)gridDim.x * GM_DECIDE_THREADS) {
         const int64_t i = base + tid;
         const bool valid = i < cnt;
         int32_t pix = valid ? list[i] : 0;
@@ -3119,13 +3130,13 @@ void gm_launch_decide(const gm_dsetup *dS, gm_state st, const int32_t *list,
                       int32_t *reset_fb, int32_t *done_list, int32_t *done_count, int mixed,
                       hipStream_t s, int defer_copy)
 {
-    const int g = grid_for(grid_items, 1024, 512);
+    const int g = grid_for(grid_items, GM_DECIDE_THREADS, GM_DECIDE_BLOCKS);
     if (mixed)
-        hipLaunchKernelGGL((k_decide<true>), dim3(g), dim3(1024), 0, s, dS, st, list, count,
-                           next_list, next_count, acc_list, acc_count, reset_fb, done_list,
+        hipLaunchKernelGGL((k_decide<true>), dim3(g), dim3(GM_DECIDE_THREADS), 0, s, dS, st, list,
+                           count, next_list, next_count, acc_list, acc_count, reset_fb, done_list,
                            done_count, defer_copy);
     else
-        hipLaunchKernelGGL((k_decide<false>), dim3(g), dim3(1024), 0, s, dS, st, list, count,
+        hipLaunchKernelGGL((k_decide<false>), dim3(g), dim3(GM_DECIDE_THREADS), 0, s, dS, st, list, count,
                            next_list, next_count, acc_list, acc_count, reset_fb, done_list,
                            done_count, defer_copy);
 }
