@@ -1916,6 +1916,20 @@ __device__ __forceinline__ void apply_delta_check(const gm_dsetup *__restrict__ 
     m[10] = m[10] < 0 ? 0.0 : (m[10] > 1.0 ? 1.0 : m[10]);
 }
 
+/* the same with the best and trial records already in registers (m is
+ * updated in place) */
+template <typename T>
+__device__ __forceinline__ void apply_loaded_to_trial(const gm_dsetup *__restrict__ S,
+                                                      const gm_state &st, int64_t pix,
+                                                      const T (&x)[9], const double (&best)[GM_NMODEL],
+                                                      double (&m)[GM_NMODEL])
+{
+    apply_delta_check<T>(S, best, x, m);
+#pragma unroll
+    for (int f = 0; f < GM_NMODEL; ++f) st.trial[pix * GM_MODEL_REC + f] = m[f];
+    store_pre(st.pre_trial, pix, make_pre(m));
+}
+
 template <typename T>
 __device__ __forceinline__ void apply_to_trial(const gm_dsetup *__restrict__ S, const gm_state &st,
                                                int64_t pix, const T (&x)[9])
@@ -1929,10 +1943,7 @@ __device__ __forceinline__ void apply_to_trial(const gm_dsetup *__restrict__ S, 
         best[f] = st.best[pix * GM_MODEL_REC + f];
         m[f] = st.trial[pix * GM_MODEL_REC + f];
     }
-    apply_delta_check<T>(S, best, x, m);
-#pragma unroll
-    for (int f = 0; f < GM_NMODEL; ++f) st.trial[pix * GM_MODEL_REC + f] = m[f];
-    store_pre(st.pre_trial, pix, make_pre(m));
+    apply_loaded_to_trial<T>(S, st, pix, x, best, m);
 }
 
 /* ---- eleven-slot systems (free filling factor) ----------------------------
@@ -2382,6 +2393,10 @@ __global__ __launch_bounds__(64) void k_step_solve11(const float *__restrict__ a
     }
 }
 
+#ifndef GM_SOLVE_PREFETCH
+#define GM_SOLVE_PREFETCH 1
+#endif
+
 /* Step solve for the active list.  SOLVER = GM_SOLVE_SVD: LDL^T where it
  * provably equals the SVD pseudo-inverse, otherwise the pixel goes to the
  * fallback list solved by k_solve_svd; GM_SOLVE_LDL: LDL^T only;
@@ -2415,6 +2430,18 @@ __global__ __launch_bounds__(64) void k_solve(const gm_dsetup *__restrict__ S,
          i += (int64_t)gridDim.x * 64) {
         const int64_t pix = list[i];
         T a[45], b[9], x[9];
+        /* the best and trial records are loaded together with the normal
+         * equations, so their latency overlaps the solve's instead of
+         * following it */
+        constexpr bool PREF = GM_SOLVE_PREFETCH && (SOLVER == GM_SOLVE_SVD || SOLVER == GM_SOLVE_LDL);
+        double best_m[GM_NMODEL], trial_m[GM_NMODEL];
+        if constexpr (PREF) {
+#pragma unroll
+            for (int f = 0; f < GM_NMODEL; ++f) {
+                best_m[f] = st.best[pix * GM_MODEL_REC + f];
+                trial_m[f] = st.trial[pix * GM_MODEL_REC + f];
+            }
+        }
         load_damped<T>(st.alpha + pix * st.ab_rec, st.beta + pix * st.ab_rec, 1,
                        (T)st.flambda[pix], a, b);
         if constexpr (SOLVER == GM_SOLVE_CORDIC) {
@@ -2432,7 +2459,10 @@ __global__ __launch_bounds__(64) void k_solve(const gm_dsetup *__restrict__ S,
             fb_list[atomicAdd(fb_count, 1)] = (int32_t)pix;
             continue;
         }
-        apply_to_trial<T>(S, st, pix, x);
+        if constexpr (PREF)
+            apply_loaded_to_trial<T>(S, st, pix, x, best_m, trial_m);
+        else
+            apply_to_trial<T>(S, st, pix, x);
     }
 }
 
