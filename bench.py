@@ -28,7 +28,7 @@ PEAK_HBM_GBS = 8000.0
 # WRITE_SIZE summed over an inversion's launches, scaled by a copy kernel of
 # known size): profiles/lm_and_cube_io_hbm_counters_1048576px.md
 K_RF_HBM_BYTES_PER_PIXEL = 1330.0
-# |d| bar of the north star against the FP64 oracle: B [G], vlos [km/s],
+# |d| accuracy bar against the FP64 oracle: B [G], vlos [km/s],
 # gamma, phi [deg] (model fields 1, 2, 5, 6)
 BAR = {1: 1.0, 2: 1e-3, 5: 0.1, 6: 0.1}
 
@@ -437,7 +437,7 @@ def guided_leg(gm, torch, dev, npix, n_train, steps, seed, oracle_sample,
             "plain_fp32_px_per_s": n1 / t_p1, "guided_px_per_s": n1 / t_g1,
             "speedup_vs_plain_fp32": t_p1 / t_g1, "n_fallback": st1["n_fallback"],
             "guided_ms": {k: st1[k] for k in ("ms_predict", "ms_lm", "ms_fallback", "ms_total")}}
-    # the north star's own fallback rule: every pixel whose chi^2 ends worse
+    # the per-pixel fallback rule: every pixel whose chi^2 ends worse
     # than the plain path's is re-inverted from init.mod, which needs the
     # plain inversion of every pixel -- timed as guided LM (no threshold) +
     # the plain inversion, the plain result taken where it fits better
@@ -446,7 +446,7 @@ def guided_leg(gm, torch, dev, npix, n_train, steps, seed, oracle_sample,
     worse = (out_nf["chisqr"] > ref["chisqr"]).cpu().numpy()
     rule = out_nf["model"].cpu().numpy().copy()
     rule[worse] = ref["model"].cpu().numpy()[worse]
-    res["north_star_fallback_rule"] = {
+    res["per_pixel_fallback_rule"] = {
         "rule": "re-invert from init.mod every pixel whose chi^2 is worse than the plain "
                 "path's (the plain inversion of the whole cube is part of its cost)",
         "px_per_s": npix / (t_plain + t_guided_nf), "n_worse": int(worse.sum()),
@@ -474,8 +474,8 @@ def guided_leg(gm, torch, dev, npix, n_train, steps, seed, oracle_sample,
             "plain_fp32": med(ref["model"][:k].cpu().numpy()),
             "plain_fp32_same_stop": med(ref_stop["model"][:k].cpu().numpy()),
             "guided": med(out["model"][:k].cpu().numpy()),
-            "north_star_rule": med(rule[:k])}
-        res["north_star_fallback_rule"]["median_err_vs_fp64_oracle"] = med(rule[:k])
+            "per_pixel_fallback_rule": med(rule[:k])}
+        res["per_pixel_fallback_rule"]["median_err_vs_fp64_oracle"] = med(rule[:k])
         # matched error: the plain FP32 path with looser stopping rules
         # (toplim, per cent chi^2 change), each timed and held against the
         # oracle; the plain rate at the guided path's median |dB| is

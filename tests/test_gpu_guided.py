@@ -84,7 +84,7 @@ def test_learned_guess_cuts_iterations_at_equal_error(gm, orc, cuda, trained):
           flush=True)
     # the plain path runs the reference's settings: its medians are the
     # parity tests' (FP32 rounding); the guided path stops at a 1 % chi^2
-    # change and is held to the north-star bar
+    # change and is held to the accuracy bar
     assert d_plain[1] <= 0.01 and d_plain[2] <= 1e-5 and d_plain[5] <= 1e-3 and d_plain[6] <= 1e-3
     assert d_guided[1] <= 1.0 and d_guided[2] <= 1e-3 and d_guided[5] <= 0.1 and d_guided[6] <= 0.1
     # fallback: exactly the pixels above the threshold of the learned run

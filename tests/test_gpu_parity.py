@@ -260,7 +260,7 @@ def _inversion_errors(gm, orc, cuda, kind, psf, npix, seed, **kw):
     return got, ref, truth, obs, ctx
 
 
-# |d| bars of the north star: B [G], vlos [km/s], gamma, phi [deg]
+# |d| accuracy bar against the FP64 oracle: B [G], vlos [km/s], gamma, phi [deg]
 BAR = {1: 1.0, 2: 1e-3, 5: 0.1, 6: 0.1}
 # medians near the achieved ones (~3e-4 G, ~2e-4 m/s, ~2e-5 deg, ~3e-5 deg)
 MED = {1: 0.01, 2: 1e-5, 5: 1e-3, 6: 1e-3}
