@@ -361,7 +361,7 @@ def _assert_medians(d, label=""):
 
 
 @pytest.mark.parametrize("precision,solver", [(1, 0), (0, 0), (1, 1), (1, 2), (1, 3), (0, 3)])
-def test_inversion_config0_matches_oracle(gm, orc, cuda, precision, solver):
+def test_inversion_6173_30l_nopsf_matches_oracle(gm, orc, cuda, precision, solver):
     """64 x 64 pixels, Fe I 6173, 30 wavelengths, no PSF, 9 free, 50 it;
     solver 3 (the reference's CORDIC solve) against the oracle's CORDIC LM."""
     got, ref, truth, obs, ctx = _inversion_errors(gm, orc, cuda, "single", None, 4096, 100,
@@ -377,9 +377,9 @@ def test_inversion_config0_matches_oracle(gm, orc, cuda, precision, solver):
 
 
 @pytest.mark.parametrize("precision,tail", [(1, 1e-9), (0, 1e-9), (0, 0.0)])
-def test_inversion_config1_psf_matches_oracle(gm, orc, cuda, precision, tail):
-    """Config-1 line, grid and PSF (Gaussian FWHM 60 mA) on 2048 pixels, in
-    mixed precision and in config 1's own plain FP32; psf_tail_tol 0 applies
+def test_inversion_6173_30l_psf_matches_oracle(gm, orc, cuda, precision, tail):
+    """Fe I 6173, 30 wavelengths, Gaussian PSF (FWHM 60 mA) on 2048 pixels, in
+    mixed precision and in the reference's plain FP32; psf_tail_tol 0 applies
     all 30 taps as the reference does (the response kernel's correlation
     then runs on the matrix cores), 1e-9 keeps 9 (on the VALU)."""
     got, ref, truth, obs, ctx = _inversion_errors(gm, orc, cuda, "single", ("gauss", 60.0),
@@ -387,7 +387,7 @@ def test_inversion_config1_psf_matches_oracle(gm, orc, cuda, precision, tail):
                                                   psf_tail_tol=tail)
     assert ctx.stats()["psf_taps"] == (30 if tail == 0.0 else 9)
     o = pair(gm, orc, "single", ("gauss", 60.0))[1]
-    check_inversion(orc, o, obs, got, ref, truth, f"cfg1 prec {precision} "
+    check_inversion(orc, o, obs, got, ref, truth, f"6173 PSF prec {precision} "
                     f"taps {ctx.stats()['psf_taps']}")
     rel_c2 = np.abs(got["chisqr"] - ref["chisqr"]) / ref["chisqr"]
     assert np.median(rel_c2) < 1e-4
@@ -403,8 +403,9 @@ def test_inversion_config1_psf_matches_oracle(gm, orc, cuda, precision, tail):
 
 
 def test_inversion_unequal_weights_matches_oracle(gm, orc, cuda):
-    """Config-1 line and PSF with Stokes weights 1, 0.7, 0.8, 1.3 (the
-    response kernel's weighted normal-equation sums) on 1024 pixels."""
+    """Fe I 6173, 30 wavelengths and the Gaussian PSF with Stokes weights 1,
+    0.7, 0.8, 1.3 (the response kernel's weighted normal-equation sums) on
+    1024 pixels."""
     got, ref, truth, obs, ctx = _inversion_errors(gm, orc, cuda, "single", ("gauss", 60.0),
                                                   1024, 200, weights=UNEQUAL_WEIGHTS)
     o = pair(gm, orc, "single", ("gauss", 60.0), weights=UNEQUAL_WEIGHTS)[1]
@@ -414,7 +415,7 @@ def test_inversion_unequal_weights_matches_oracle(gm, orc, cuda):
 
 
 def test_inversion_two_lines_matches_oracle(gm, orc, cuda):
-    """Config-2 shape: 6301.5 + 6302.5, 112 wavelengths, PSF, weak-field and
+    """Two lines (6301.5 + 6302.5), 112 wavelengths, PSF, weak-field and
     high-noise pixels, on 512 pixels."""
     got, ref, truth, obs, ctx = _inversion_errors(gm, orc, cuda, "two", ("gauss", 60.0), 512,
                                                   300)
