@@ -500,9 +500,10 @@ static gm_status invert_launch(gm_ctx *c, const gm_state &st, const float *obs, 
     kend();
     int cur = 0;
     const int miter = c->setup.miter > 0 ? c->setup.miter : 1;  /* do-while */
-    /* hist row r: [active in round r, unused, accepted in round r]; the solve
-     * kernel of round r writes hist[3r] and the accepted count of round r-1
-     * into hist[3r - 1]; empty trailing rounds cost four short launches */
+    /* hist row r: [active in round r, SVD solves in round r, accepted in
+     * round r]; the solve kernel of round r writes hist[3r] and the accepted
+     * count of round r-1 into hist[3r - 1], the decision kernel hist[3r + 1];
+     * empty trailing rounds cost four short launches */
     int32_t *hist = st.hist + 3;
     for (int r = 0; r < miter; ++r) {
         const int nxt = 1 - cur;
@@ -517,7 +518,8 @@ static gm_status invert_launch(gm_ctx *c, const gm_state &st, const float *obs, 
         kend();
         kbegin(3);
         gm_launch_decide(c->dS, st, st.list[cur], &cnt[cur], npix, st.list[nxt], &cnt[nxt],
-                         st.list[2], &cnt[2], &cnt[3], nullptr, nullptr, mixed, s, 1);
+                         st.list[2], &cnt[2], &cnt[3], nullptr, nullptr, mixed, s, 1,
+                         hist + 3 * r + 1);
         kend();
         kbegin(1);
         /* alpha_first = 0: the filling-factor response of the reference's

@@ -1653,128 +1653,173 @@ __device__ __forceinline__ void solve_svd(T (&a)[45], const T (&b)[9], T (&x)[9]
     }
 }
 
-/* The same Jacobi SVD with V split over a group of G lanes: every lane
- * rotates the (identical) matrix, lane g keeps rows R*g .. R*g+R-1 of V
- * (R = ceil(9/G), rows past 8 are zero dummies), so a lane holds 45 + 9R
- * values instead of 45 + 81.  With G = 4 the FP64 version fits the 256
- * architectural VGPRs and each lane updates 3 rows of V per rotation
- * instead of 5.  x is complete in every lane of the group.                 */
-template <typename T, int K, int RR>
-__device__ __forceinline__ void jacobi_step_rows(T (&a)[45], T (&vr)[RR][9])
+/* The same Jacobi SVD by one whole wave for one matrix: lane l holds one
+ * element of A (packed upper-triangle entry l, lanes 0-44) or of V (entry
+ * l - 45, lanes 45-63) and one more of V (entry l + 19, lanes 0-61), so a
+ * rotation step costs each lane two element updates instead of every lane
+ * repeating the whole step.  The elements live in LDS (full symmetric A and
+ * V, row-major); a step reads the three partners each element needs, lanes
+ * 0-3 form the four rotations and publish (c, s, t), every lane reads the
+ * rotations of its row and column, updates and writes back.  The lanes of
+ * one wave reach LDS in program order, so no barrier is needed between the
+ * phases, only a fence that keeps the compiler from reordering them.  The
+ * rotations and update formulas are those of jacobi_step; the sweep test
+ * sums the same squares (in another order).                                */
+template <typename T>
+struct WaveSvd {
+    T rot[5][4];           /* c, s, t of pair m (1-4); pair 0: the waiting index */
+    T a[81];
+    T v[81];
+    T b[9], aux[9], x[9];
+};
+
+__device__ __forceinline__ void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); }
+
+template <typename T>
+__device__ __forceinline__ T wave_uniform_sum(T v)
 {
-    constexpr int P[4] = {(K + 1) % 9, (K + 2) % 9, (K + 3) % 9, (K + 4) % 9};
-    constexpr int Q[4] = {(K + 8) % 9, (K + 7) % 9, (K + 6) % 9, (K + 5) % 9};
-    T c[4], sn[4], t[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const T app = a[ut(P[i], P[i])], aqq = a[ut(Q[i], Q[i])], apq = a[ut(P[i], Q[i])];
-        jacobi_angle<T>(app, aqq, apq, c[i], sn[i], t[i]);
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = i + 1; j < 4; ++j) {
-            const T m00 = a[ut(P[i], P[j])], m01 = a[ut(P[i], Q[j])];
-            const T m10 = a[ut(Q[i], P[j])], m11 = a[ut(Q[i], Q[j])];
-            const T n00 = c[j] * m00 - sn[j] * m01, n01 = sn[j] * m00 + c[j] * m01;
-            const T n10 = c[j] * m10 - sn[j] * m11, n11 = sn[j] * m10 + c[j] * m11;
-            a[ut(P[i], P[j])] = c[i] * n00 - sn[i] * n10;
-            a[ut(P[i], Q[j])] = c[i] * n01 - sn[i] * n11;
-            a[ut(Q[i], P[j])] = sn[i] * n00 + c[i] * n10;
-            a[ut(Q[i], Q[j])] = sn[i] * n01 + c[i] * n11;
-        }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const T akp = a[ut(K, P[i])], akq = a[ut(K, Q[i])];
-        a[ut(K, P[i])] = c[i] * akp - sn[i] * akq;
-        a[ut(K, Q[i])] = sn[i] * akp + c[i] * akq;
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const T apq = a[ut(P[i], Q[i])];
-        a[ut(P[i], P[i])] -= t[i] * apq;
-        a[ut(Q[i], Q[i])] += t[i] * apq;
-        a[ut(P[i], Q[i])] = T(0);
-    }
-#pragma unroll
-    for (int r = 0; r < RR; ++r)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const T vp = vr[r][P[i]], vq = vr[r][Q[i]];
-            vr[r][P[i]] = c[i] * vp - sn[i] * vq;
-            vr[r][Q[i]] = sn[i] * vp + c[i] * vq;
-        }
+    v = group_sum_first<T, 64>(v);
+    if constexpr (sizeof(T) == 8)
+        return __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(v)),
+                                __builtin_amdgcn_readfirstlane(__double2loint(v)));
+    else
+        return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
-template <typename T, int MAXSWEEP, int G>
-__device__ __forceinline__ void solve_svd_group(T (&a)[45], const float *__restrict__ beta,
-                                                int g, T (&x)[9])
+/* Step K of the round-robin ordering: index i is paired with (2K - i) mod 9;
+ * with m = (i - K) mod 9 it is the p of pair m (m = 1-4, sigma = -s) or the
+ * q of pair 9 - m (m = 5-8, sigma = +s); m = 0 is the waiting index.       */
+template <typename T, int K>
+__device__ __forceinline__ void wave_jacobi_step(WaveSvd<T> &L, int lane, bool isA, int i0, int j0,
+                                                 int i1, int j1, bool has1, T *B0, int o0, int o0t,
+                                                 int o1, T &v0, T &v1)
 {
-    constexpr int RR = (9 + G - 1) / G;
-    T vr[RR][9];
+    constexpr int K2 = (2 * K) % 9;
+    const auto partner = [](int i) { const int r = K2 - i; return (r < 0) ? r + 9 : r; };
+    const auto pairno = [](int i, bool &pside) {
+        int m = i - K;
+        m += (m < 0) ? 9 : 0;
+        pside = m <= 4;
+        return pside ? m : 9 - m;
+    };
+    /* lanes 0-3: the rotation of pair mm (the other lanes repeat theirs) */
+    const int mm = (lane & 3) + 1;
+    int p = K + mm, q = K - mm;
+    p -= (p >= 9) ? 9 : 0;
+    q += (q < 0) ? 9 : 0;
+    const T app = L.a[p * 10], aqq = L.a[q * 10], apq = L.a[p * 9 + q];
+    const int pi0 = partner(i0), pj0 = partner(j0), pj1 = partner(j1);
+    const T y0 = B0[i0 * 9 + pj0], z0 = B0[pi0 * 9 + j0], w0 = B0[pi0 * 9 + pj0];
+    const T y1 = L.v[i1 * 9 + pj1];
+    T c, sn, t;
+    jacobi_angle<T>(app, aqq, apq, c, sn, t);
+    if (lane < 4) {
+        L.rot[mm][0] = c;
+        L.rot[mm][1] = sn;
+        L.rot[mm][2] = t;
+    }
+    wave_fence();
+    bool ps_i, ps_j, ps_j1;
+    const int ni = pairno(i0, ps_i), nj = pairno(j0, ps_j), nj1 = pairno(j1, ps_j1);
+    T ci = L.rot[ni][0], si = L.rot[ni][1], ti = L.rot[ni][2];
+    T cj = L.rot[nj][0], sj = L.rot[nj][1];
+    T cj1 = L.rot[nj1][0], sj1 = L.rot[nj1][1];
+    /* all seven loaded here, together: otherwise the compiler moves some of
+     * them into the branches it makes of the selects below, behind a wait
+     * for the others */
+    asm volatile("" : "+v"(ci), "+v"(si), "+v"(ti), "+v"(cj), "+v"(sj), "+v"(cj1), "+v"(sj1));
+    const T gi = ps_i ? -si : si, gj = ps_j ? -sj : sj, gj1 = ps_j1 ? -sj1 : sj1;
+    /* A' = J^T A J, V' = V J: the column rotation of this element's row,
+     * then (A) the row rotation; a pair's diagonal gets a_pp - t a_pq,
+     * a_qq + t a_pq and its own off-diagonal element 0.  Selects, not
+     * branches: a branch would let the compiler sink the loads of z0, w0
+     * behind the rotation reads, one more LDS round trip per step. */
+    const T row = cj * v0 + gj * y0;
+    const T gen = ci * row + gi * (cj * z0 + gj * w0);
+    const T dia = v0 + (ps_i ? -ti : ti) * y0;
+    const bool pair_off = (j0 == pi0) && (i0 != j0);
+    T n0 = (i0 == j0) ? dia : (pair_off ? T(0) : gen);
+    n0 = isA ? n0 : row;
+    const T n1 = cj1 * v1 + gj1 * y1;
+    B0[o0] = n0;
+    B0[o0t] = n0;
+    if (has1) L.v[o1] = n1;
+    wave_fence();
+    v0 = n0;
+    v1 = n1;
+}
+
+/* Pseudo-inverse step of one pixel's damped system by the whole wave (all
+ * 64 lanes active, the same pointers in every lane): al = packed 45-entry
+ * matrix, be = right-hand side, damp = 1 + lambda on the diagonal.  x is
+ * returned in every lane.                                                  */
+template <typename T, int MAXSWEEP>
+__device__ __forceinline__ void wave_svd_solve(WaveSvd<T> &L, const float *__restrict__ al,
+                                               const float *__restrict__ be, T damp, T (&x)[9])
+{
+    const int lane = threadIdx.x & 63;
+    const bool isA = lane < 45;
+    int i0, j0;
+    if (isA) {
+        /* row of packed entry `lane`: ut(r, r) = 0, 9, 17, 24, 30, 35, 39, 42, 44 */
+        i0 = 0;
 #pragma unroll
-    for (int r = 0; r < RR; ++r)
-#pragma unroll
-        for (int j = 0; j < 9; ++j) vr[r][j] = (g * RR + r == j) ? T(1) : T(0);
+        for (int r = 1; r < 9; ++r) i0 += (lane >= ut(r, r)) ? 1 : 0;
+        j0 = i0 + lane - (i0 * 9 - (i0 * (i0 - 1)) / 2);
+    } else {
+        i0 = (lane - 45) / 9;
+        j0 = (lane - 45) - 9 * i0;
+    }
+    const bool has1 = lane < 62;
+    const int e1 = has1 ? lane + 19 : 80;
+    const int i1 = e1 / 9, j1 = e1 - 9 * i1;
+    T *const B0 = isA ? L.a : L.v;
+    const int o0 = i0 * 9 + j0, o0t = isA ? j0 * 9 + i0 : o0, o1 = i1 * 9 + j1;
+    T v0 = isA ? (T)al[lane] : ((i0 == j0) ? T(1) : T(0));
+    if (isA && i0 == j0) v0 = v0 * damp;
+    T v1 = (i1 == j1) ? T(1) : T(0);
+    B0[o0] = v0;
+    B0[o0t] = v0;
+    if (has1) L.v[o1] = v1;
+    if (lane < 4) L.rot[0][lane] = (lane == 0) ? T(1) : T(0);
+    if (lane < 9) L.b[lane] = (T)be[lane];
+    wave_fence();
     const T eps2 = (sizeof(T) == 8) ? T(1e-30) : T(1e-14);
-    bool done = false;
     for (int sweep = 0; sweep < MAXSWEEP; ++sweep) {
-        T off = 0, dg = 0;
-#pragma unroll
-        for (int p = 0; p < 9; ++p) {
-            dg += a[ut(p, p)] * a[ut(p, p)];
-#pragma unroll
-            for (int q = p + 1; q < 9; ++q) off += a[ut(p, q)] * a[ut(p, q)];
-        }
-        done = done || (off <= eps2 * dg);
-        if (__all(done)) break;
-        if (!done) {
-            jacobi_step_rows<T, 0, RR>(a, vr);
-            jacobi_step_rows<T, 1, RR>(a, vr);
-            jacobi_step_rows<T, 2, RR>(a, vr);
-            jacobi_step_rows<T, 3, RR>(a, vr);
-            jacobi_step_rows<T, 4, RR>(a, vr);
-            jacobi_step_rows<T, 5, RR>(a, vr);
-            jacobi_step_rows<T, 6, RR>(a, vr);
-            jacobi_step_rows<T, 7, RR>(a, vr);
-            jacobi_step_rows<T, 8, RR>(a, vr);
-        }
+        const T sq = isA ? v0 * v0 : T(0);
+        const T off = wave_uniform_sum<T>((i0 != j0) ? sq : T(0));
+        const T dg = wave_uniform_sum<T>((i0 == j0) ? sq : T(0));
+        if (off <= eps2 * dg) break;
+        wave_jacobi_step<T, 0>(L, lane, isA, i0, j0, i1, j1, has1, B0, o0, o0t, o1, v0, v1);
+        wave_jacobi_step<T, 1>(L, lane, isA, i0, j0, i1, j1, has1, B0, o0, o0t, o1, v0, v1);
+        wave_jacobi_step<T, 2>(L, lane, isA, i0, j0, i1, j1, has1, B0, o0, o0t, o1, v0, v1);
+        wave_jacobi_step<T, 3>(L, lane, isA, i0, j0, i1, j1, has1, B0, o0, o0t, o1, v0, v1);
+        wave_jacobi_step<T, 4>(L, lane, isA, i0, j0, i1, j1, has1, B0, o0, o0t, o1, v0, v1);
+        wave_jacobi_step<T, 5>(L, lane, isA, i0, j0, i1, j1, has1, B0, o0, o0t, o1, v0, v1);
+        wave_jacobi_step<T, 6>(L, lane, isA, i0, j0, i1, j1, has1, B0, o0, o0t, o1, v0, v1);
+        wave_jacobi_step<T, 7>(L, lane, isA, i0, j0, i1, j1, has1, B0, o0, o0t, o1, v0, v1);
+        wave_jacobi_step<T, 8>(L, lane, isA, i0, j0, i1, j1, has1, B0, o0, o0t, o1, v0, v1);
     }
-    /* the right-hand side is read only now: held through the sweeps it
-     * would add 9 values to the register pressure of the loop */
-    T b[9];
-#pragma unroll
-    for (int p = 0; p < 9; ++p) b[p] = (T)beta[p];
-    /* V^T b: this lane's rows, summed over the group */
-    T aux[9];
-#pragma unroll
-    for (int j = 0; j < 9; ++j) {
+    /* x = V diag(1/w, w > 1e-12) V^T b, w = |a_jj|: lane j forms entry j of
+     * each product */
+    if (lane < 9) {
         T sum = 0;
 #pragma unroll
-        for (int r = 0; r < RR; ++r) {
-            const int row = g * RR + r;
-            T br = b[0];
-#pragma unroll
-            for (int k = 1; k < 9; ++k) br = (row == k) ? b[k] : br;
-            sum += (row < 9) ? vr[r][j] * br : T(0);
-        }
-#pragma unroll
-        for (int m = 1; m < G; m <<= 1) sum += __shfl_xor(sum, m, 64);
-        const T w = fabs(a[ut(j, j)]);
-        aux[j] = (w > T(1e-12)) ? sum / w : T(0);
+        for (int k = 0; k < 9; ++k) sum += L.b[k] * L.v[k * 9 + lane];
+        const T w = fabs(L.a[lane * 10]);
+        L.aux[lane] = (w > T(1e-12)) ? sum / w : T(0);
     }
-    T xr[RR];
-#pragma unroll
-    for (int r = 0; r < RR; ++r) {
+    wave_fence();
+    if (lane < 9) {
         T sum = 0;
 #pragma unroll
-        for (int k = 0; k < 9; ++k) sum += vr[r][k] * aux[k];
-        xr[r] = sum;
+        for (int k = 0; k < 9; ++k) sum += L.v[lane * 9 + k] * L.aux[k];
+        L.x[lane] = sum;
     }
-    /* x_i lives in lane i / RR of the group, row i % RR */
-    const int base = (int)(threadIdx.x & 63) & ~(G - 1);
+    wave_fence();
 #pragma unroll
-    for (int i = 0; i < 9; ++i) x[i] = __shfl(xr[i % RR], base + i / RR, 64);
+    for (int i = 0; i < 9; ++i) x[i] = L.x[i];
+    wave_fence();
 }
 
 /* LDL^T solve of the damped matrix (symmetric positive semi-definite by
@@ -2396,6 +2441,8 @@ __global__ __launch_bounds__(64) void k_step_solve11(const float *__restrict__ a
 #ifndef GM_SOLVE_PREFETCH
 #define GM_SOLVE_PREFETCH 1
 #endif
+template <typename T>
+constexpr int svd_sweeps() { return sizeof(T) == 8 ? 12 : 10; }
 
 /* Step solve for the active list.  SOLVER = GM_SOLVE_SVD: LDL^T where it
  * provably equals the SVD pseudo-inverse, otherwise the pixel goes to the
@@ -2413,11 +2460,11 @@ __global__ __launch_bounds__(64) void k_solve(const gm_dsetup *__restrict__ S,
                                               int32_t *__restrict__ fb_count)
 {
     /* hist[0] = active pixels this round, hist[-1] = accepted pixels of the
-     * previous round (read before that list's counter is reset) */
+     * previous round (read before that list's counter is reset); hist[1],
+     * the fallback count, is written by k_decide */
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (hist) {
             hist[0] = *count;
-            hist[1] = 0;                     /* Jacobi fallbacks, set by k_solve_svd */
             if (reset1) hist[-1] = *reset1;
         }
         if (reset0) *reset0 = 0;
@@ -2466,43 +2513,24 @@ __global__ __launch_bounds__(64) void k_solve(const gm_dsetup *__restrict__ S,
     }
 }
 
-/* Jacobi SVD for the pixels the LDL^T test could not certify, a group of
- * kSvdLanes lanes per pixel.  Few pixels fall back per round, so the launch
- * lasts as long as one group's instruction stream: every lane repeats the
- * rotations of the matrix, but each keeps fewer rows of V the more lanes
- * share a pixel, and a wave waits for the slowest of its pixels' sweeps.
- * Per 1M-pixel inversion (solve, both tiers), with the IEEE square root
- * and division in the rotation: 8.3 ms with 2 lanes, 8.1 with 4, 7.9 with
- * 8, 7.75 with 16; with the reciprocal-square-root rotation: 7.09 with 16,
- * 7.06 with 32, 6.99 with 64 (one pixel per wave). */
-#ifndef GM_SVD_LANES
-#define GM_SVD_LANES 64
-#endif
-constexpr int kSvdLanes = GM_SVD_LANES;
+/* Jacobi SVD of the fallback list, one wave per pixel (wave_svd_solve).  Per 1M-pixel
+ * inversion with the two-tier solve, the fallback pixels' Jacobi took 1.5 ms
+ * with every lane repeating the rotations of the whole matrix. */
 template <bool MIXED>
 __global__ __launch_bounds__(64) void k_solve_svd(const gm_dsetup *__restrict__ S, gm_state st,
                                                   const int32_t *__restrict__ fb_list,
-                                                  const int32_t *__restrict__ fb_count,
-                                                  int32_t *__restrict__ hist)
+                                                  const int32_t *__restrict__ fb_count)
 {
     using T = typename std::conditional<MIXED, double, float>::type;
+    __shared__ WaveSvd<T> L;
     const int64_t cnt = *fb_count;
-    if (hist && blockIdx.x == 0 && threadIdx.x == 0) hist[1] = (int32_t)cnt;
     const int64_t cap = st.cap;
-    constexpr int G = kSvdLanes;
-    const int g = threadIdx.x & (G - 1);
-    for (int64_t i = ((int64_t)blockIdx.x * 64 + threadIdx.x) / G; i < cnt;
-         i += ((int64_t)gridDim.x * 64) / G) {
+    for (int64_t i = blockIdx.x; i < cnt; i += gridDim.x) {
         const int64_t pix = fb_list[i];
-        T a[45], x[9];
-        const float *al = st.alpha + pix * st.ab_rec;
-        const T damp = T(1) + (T)st.flambda[pix];
-#pragma unroll
-        for (int e = 0; e < 45; ++e) a[e] = (T)al[e];
-#pragma unroll
-        for (int p = 0; p < 9; ++p) a[ut(p, p)] = a[ut(p, p)] * damp;
-        solve_svd_group<T, (sizeof(T) == 8 ? 12 : 10), G>(a, st.beta + pix * st.ab_rec, g, x);
-        if (g == 0) apply_to_trial<T>(S, st, pix, x);
+        T x[9];
+        wave_svd_solve<T, svd_sweeps<T>()>(L, st.alpha + pix * st.ab_rec, st.beta + pix * st.ab_rec,
+                                           T(1) + (T)st.flambda[pix], x);
+        if (threadIdx.x == 0) apply_to_trial<T>(S, st, pix, x);
     }
 }
 
@@ -2513,25 +2541,46 @@ __global__ __launch_bounds__(64) void k_step_solve(const float *__restrict__ alp
                                                    int64_t n, double *__restrict__ delta)
 {
     using T = typename std::conditional<MIXED, double, float>::type;
-    for (int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * 64) {
-        /* caller layout: alpha [n][45], beta [n][9] */
-        T a[45], b[9], x[9];
-        load_damped<T>(alpha + i * 45, beta + i * 9, 1, (T)flambda[i], a, b);
-        if constexpr (SOLVER == GM_SOLVE_CORDIC) {
-            __shared__ float th[cordic::TH_FLOATS];
-            cordic::solve9<T>(a, b, x, th + threadIdx.x);
-        } else if (SOLVER == GM_SOLVE_JACOBI) {
-            solve_svd<T, (sizeof(T) == 8 ? 12 : 10)>(a, b, x);
-        } else {
-            T a2[45];
+    for (int64_t base = (int64_t)blockIdx.x * 64; base < n; base += (int64_t)gridDim.x * 64) {
+        const int64_t i = base + threadIdx.x;
+        bool need_svd = false;
+        if (i < n) {
+            /* caller layout: alpha [n][45], beta [n][9] */
+            T a[45], b[9], x[9];
+            load_damped<T>(alpha + i * 45, beta + i * 9, 1, (T)flambda[i], a, b);
+            if constexpr (SOLVER == GM_SOLVE_CORDIC) {
+                __shared__ float th[cordic::TH_FLOATS];
+                cordic::solve9<T>(a, b, x, th + threadIdx.x);
+            } else if (SOLVER == GM_SOLVE_JACOBI) {
+                solve_svd<T, svd_sweeps<T>()>(a, b, x);
+            } else {
+                need_svd = !solve_ldl<T>(a, b, x) && SOLVER == GM_SOLVE_SVD;
+            }
+            if (!need_svd) {
 #pragma unroll
-            for (int e = 0; e < 45; ++e) a2[e] = a[e];
-            const bool proven = solve_ldl<T>(a, b, x);
-            if (SOLVER == GM_SOLVE_SVD && !proven) solve_svd<T, (sizeof(T) == 8 ? 12 : 10)>(a2, b, x);
+                for (int p = 0; p < 9; ++p) delta[i * 9 + p] = (double)x[p];
+            }
         }
+        /* the uncertified systems: the Jacobi SVD of the whole wave, one
+         * system after the other, as in the LM loop */
+        if constexpr (SOLVER == GM_SOLVE_SVD) {
+            uint64_t need = __ballot(need_svd);
+            if (need) {
+                __shared__ WaveSvd<T> L;
+                while (need) {
+                    const int l = __builtin_ctzll(need);
+                    need &= need - 1;
+                    const int64_t fi = base + l;
+                    T x[9];
+                    wave_svd_solve<T, svd_sweeps<T>()>(L, alpha + fi * 45, beta + fi * 9,
+                                                       T(1) + (T)flambda[fi], x);
+                    if ((int)threadIdx.x == l) {
 #pragma unroll
-        for (int p = 0; p < 9; ++p) delta[i * 9 + p] = (double)x[p];
+                        for (int p = 0; p < 9; ++p) delta[fi * 9 + p] = (double)x[p];
+                    }
+                }
+            }
+        }
     }
 }
 
@@ -2559,12 +2608,16 @@ __global__ __launch_bounds__(GM_DECIDE_THREADS) void k_decide(const gm_dsetup *_
                                                  int32_t *__restrict__ reset_fb,
                                                  int32_t *__restrict__ done_list,
                                                  int32_t *__restrict__ done_count,
-                                                 int defer_copy)
+                                                 int defer_copy, int32_t *__restrict__ fb_hist)
 {
     using T = typename std::conditional<MIXED, double, float>::type;
     __shared__ int wc[GM_DECIDE_THREADS / 64], wa[GM_DECIDE_THREADS / 64], wd[GM_DECIDE_THREADS / 64];
     __shared__ int base_c, base_a, base_d;
-    if (blockIdx.x == 0 && threadIdx.x == 0 && reset_fb) *reset_fb = 0;
+    /* the solve kernel of this round is complete: its SVD count is final */
+    if (blockIdx.x == 0 && threadIdx.x == 0 && reset_fb) {
+        if (fb_hist) *fb_hist = *reset_fb;
+        *reset_fb = 0;
+    }
     const int64_t cnt = *count;
     const int64_t cap = st.cap;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -2654,41 +2707,48 @@ __device__ void classical_estimates(const gm_dsetup *__restrict__ S, const float
     for (i = hi - 1; i >= lo; i--) { if (sI[i] < 0) end--; else break; }
     /* all-negative I: the reference would read I[-1]; use the first sample */
     const double Ic = (end > lo) ? sI[end - 1] : sI[lo];
-    double x = 0, y = 0, x_vlos = 0, y_vlos = 0;
-    for (i = begin; i < end - 1; i++) {
-        if (sI[i] > -1 && sV[i] > -1) {
-            const double aux = (Ic - (double)(sI[i] + sV[i]));
-            const double aux_vlos = (Ic - sI[i]);
-            x += aux * (lambda[i] - lambda_0);
-            x_vlos += aux_vlos * (lambda[i] - lambda_0);
+    /* The reference's four sample loops as one loop over [lo, hi) whose
+     * bounds are the same in every lane (the line window), each sum masked
+     * to its own range and conditions: the wavelength loads are then
+     * uniform and the samples of all four loops are read once.  Each sum
+     * still adds its terms in ascending wavelength order.                  */
+    double x = 0, y = 0, x_vlos = 0, y_vlos = 0, xm = 0, ym = 0, xg = 0, yg = 0;
+    double sum_u = 0.0, sum_q = 0.0;
+    for (i = lo; i < hi; i++) {
+        const double dl = lambda[i] - lambda_0;
+        const float fI = sI[i], fQ = sQ[i], fU = sU[i], fV = sV[i];
+        const bool in = (i >= begin) && (i < end - 1);
+        if (in && fI > -1 && fV > -1) {
+            const double aux = (Ic - (double)(fI + fV));
+            const double aux_vlos = (Ic - fI);
+            x += aux * dl;
+            x_vlos += aux_vlos * dl;
             y += aux;
             y_vlos += aux_vlos;
+            const double auxm = (Ic - (double)(fI - fV));
+            xm += auxm * dl;
+            ym += auxm;
+        }
+        if (in && fQ > -1 && fU > -1 && fV > -1) {
+            const double L = fabs(sqrtf(fQ * fQ + fU * fU));
+            const double mm = fabs((4 * dl * L));
+            xg = xg + fabsf(fV) * mm;
+            yg = yg + fabsf(fV) * fabsf(fV);
+        }
+        if (fU > -1 && fQ > -1) {
+            if (fabs(fU) > 0.0001 || fabs(fQ) > 0.0001) {
+                sum_u += fU;
+                sum_q += fQ;
+            }
         }
     }
     const double lp = (fabs(y) > 1e-15) ? x / y : 0;
-    x = 0; y = 0;
-    for (i = begin; i < end - 1; i++) {
-        if (sI[i] > -1 && sV[i] > -1) {
-            const double aux = (Ic - (double)(sI[i] - sV[i]));
-            x += aux * (lambda[i] - lambda_0);
-            y += aux;
-        }
-    }
-    const double lm = (fabs(y) > 1e-15) ? x / y : 0;
+    const double lm = (fabs(ym) > 1e-15) ? xm / ym : 0;
     const double C = (4.6686411e-13 * (lambda_0 * lambda_0) * S->ce_geff);
     const double Blos = (1 / C) * ((lp - lm) / 2);
     double Vlos = (2.99792458e+5 / lambda_0) * ((x_vlos / y_vlos) / 2);
-    x = 0; y = 0;
-    for (i = begin; i < end - 1; i++) {
-        if (sQ[i] > -1 && sU[i] > -1 && sV[i] > -1) {
-            const double L = fabs(sqrtf(sQ[i] * sQ[i] + sU[i] * sU[i]));
-            const double mm = fabs((4 * (lambda[i] - lambda_0) * L));
-            x = x + fabsf(sV[i]) * mm;
-            y = y + fabsf(sV[i]) * fabsf(sV[i]);
-        }
-    }
-    y = y * fabs((3 * C * Blos));
-    const double tan_gamma = fabs(sqrt(x / y));
+    yg = yg * fabs((3 * C * Blos));
+    const double tan_gamma = fabs(sqrt(xg / yg));
     double gamma_rad = atan(tan_gamma);
     double gamma = gamma_rad * (180 / kPI);
     if (initial) {
@@ -2696,15 +2756,6 @@ __device__ void classical_estimates(const gm_dsetup *__restrict__ S, const float
         if (gamma > 90 && gamma <= 95) gamma_rad = 95 * (kPI / 180);
     }
     if (Blos < 0) gamma = (180) - gamma;
-    double sum_u = 0.0, sum_q = 0.0;
-    for (i = lo; i < hi; i++) {
-        if (sU[i] > -1 && sQ[i] > -1) {
-            if (fabs(sU[i]) > 0.0001 || fabs(sQ[i]) > 0.0001) {
-                sum_u += sU[i];
-                sum_q += sQ[i];
-            }
-        }
-    }
     const double tan2phi = sum_u / sum_q;
     double phi = (atan(tan2phi) * 180 / kPI) / 2;
     if (sum_u > 0 && sum_q > 0) phi = phi;
@@ -2762,11 +2813,40 @@ __global__ __launch_bounds__(64) void k_init_state(const gm_dsetup *__restrict__
     const int n4 = 4 * n, stride = 4 * n + 1;
     const int64_t cap = st.cap;
     const int tid = threadIdx.x;
+    const bool vec4 = (reinterpret_cast<uintptr_t>(obs) & 15) == 0;
     for (int64_t base = (int64_t)blockIdx.x * ppb; base < npix; base += (int64_t)gridDim.x * ppb) {
         const int nb = (int)((npix - base < ppb) ? npix - base : ppb);
         if (init_mode == GM_INIT_CE) {
             const float *src = obs + (slot0 + base) * n4;
-            for (int i = tid; i < nb * n4; i += 64) spec[(i / n4) * stride + (i % n4)] = src[i];
+            const int total = nb * n4;
+            if (vec4) {
+                /* float4 loads, eight in flight per lane before their LDS
+                 * stores (one at a time, each store waited for its load:
+                 * about 60 serial load latencies per block); n4 is a
+                 * multiple of 4, so a float4 never straddles two spectra */
+                for (int i0 = 4 * tid; i0 < total; i0 += 8 * 256) {
+                    float4 v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int i = i0 + 256 * u;
+                        if (i < total) v[u] = *reinterpret_cast<const float4 *>(src + i);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int i = i0 + 256 * u;
+                        if (i < total) {
+                            const int r = i / n4;
+                            float *d = spec + r * stride + (i - r * n4);
+                            d[0] = v[u].x;
+                            d[1] = v[u].y;
+                            d[2] = v[u].z;
+                            d[3] = v[u].w;
+                        }
+                    }
+                }
+            } else {
+                for (int i = tid; i < total; i += 64) spec[(i / n4) * stride + (i % n4)] = src[i];
+            }
             __syncthreads();
         }
         if (tid < nb) {
@@ -3118,13 +3198,11 @@ void gm_launch_solve(const gm_dsetup *dS, gm_state st, const int32_t *list,
     }
 #undef GM_SOLVE_CASE
     if (solver == GM_SOLVE_SVD || solver == GM_SOLVE_JACOBI) {
-        const int gf = grid_for(grid_items, 64 / kSvdLanes, 1024);
+        const int gf = grid_for(grid_items, 1, 1024);
         if (mixed)
-            hipLaunchKernelGGL((k_solve_svd<true>), dim3(gf), dim3(64), 0, s, dS, st, fb_list, fb_count,
-                               hist);
+            hipLaunchKernelGGL((k_solve_svd<true>), dim3(gf), dim3(64), 0, s, dS, st, fb_list, fb_count);
         else
-            hipLaunchKernelGGL((k_solve_svd<false>), dim3(gf), dim3(64), 0, s, dS, st, fb_list,
-                               fb_count, hist);
+            hipLaunchKernelGGL((k_solve_svd<false>), dim3(gf), dim3(64), 0, s, dS, st, fb_list, fb_count);
     }
 }
 
@@ -3158,19 +3236,22 @@ void gm_launch_decide(const gm_dsetup *dS, gm_state st, const int32_t *list,
                       const int32_t *count, int64_t grid_items, int32_t *next_list,
                       int32_t *next_count, int32_t *acc_list, int32_t *acc_count,
                       int32_t *reset_fb, int32_t *done_list, int32_t *done_count, int mixed,
-                      hipStream_t s, int defer_copy)
+                      hipStream_t s, int defer_copy, int32_t *fb_hist)
 {
     const int g = grid_for(grid_items, GM_DECIDE_THREADS, GM_DECIDE_BLOCKS);
     if (mixed)
         hipLaunchKernelGGL((k_decide<true>), dim3(g), dim3(GM_DECIDE_THREADS), 0, s, dS, st, list,
                            count, next_list, next_count, acc_list, acc_count, reset_fb, done_list,
-                           done_count, defer_copy);
+                           done_count, defer_copy, fb_hist);
     else
         hipLaunchKernelGGL((k_decide<false>), dim3(g), dim3(GM_DECIDE_THREADS), 0, s, dS, st, list, count,
                            next_list, next_count, acc_list, acc_count, reset_fb, done_list,
-                           done_count, defer_copy);
+                           done_count, defer_copy, fb_hist);
 }
 
+#ifndef GM_INIT_PPB
+#define GM_INIT_PPB 32
+#endif
 void gm_launch_init_state(const gm_dsetup *dS, gm_state st, const float *obs,
                           const double *init, int init_mode, int nlambda, int64_t slot0,
                           int64_t npix, int32_t *list0, hipStream_t s)
@@ -3180,7 +3261,7 @@ void gm_launch_init_state(const gm_dsetup *dS, gm_state st, const float *obs,
      * overlap (64 pixels, five blocks per CU: 0.23 ms slower per 1M px) */
     const int stride = 4 * nlambda + 1;
     int ppb = 8192 / stride;
-    ppb = ppb < 1 ? 1 : (ppb > 32 ? 32 : ppb);
+    ppb = ppb < 1 ? 1 : (ppb > GM_INIT_PPB ? GM_INIT_PPB : ppb);
     const size_t shm = (init_mode == GM_INIT_CE) ? (size_t)ppb * stride * sizeof(float) : 0;
     const int g = grid_for(npix, ppb, kMaxBlocks);
     hipLaunchKernelGGL(k_init_state, dim3(g), dim3(64), shm, s, dS, st, obs, init, init_mode, slot0,
