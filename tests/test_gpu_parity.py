@@ -193,17 +193,18 @@ def test_step_solve_matches_oracle(gm, orc, cuda, precision, solver):
         assert np.abs(d[i] - x).max() <= rtol * np.abs(x).max() + 1e-12, (i, d[i], x)
 
 
-@pytest.mark.parametrize("solver", [0, 2])
-def test_step_solve_pseudo_inverse_cases(gm, orc, cuda, solver):
+@pytest.mark.parametrize("solver,n", [(0, 128), (2, 128), (0, 77)])
+def test_step_solve_pseudo_inverse_cases(gm, orc, cuda, solver, n):
     """Exactly singular normal matrices with no zero row: A = J^T J / 64 with
     small-integer J whose last one or two columns are sums of the others, so
     A is exact in float32 and its null eigenvalues fall far below the 1e-12
     cut.  The pseudo-inverse drops them; the LDL^T certificate has to fail
-    and hand those pixels to the Jacobi SVD.  Odd pixels are full rank."""
+    and hand those pixels to the Jacobi SVD (with GM_SOLVE_SVD one wave
+    solves the uncertified systems of its 64 in turn; n = 77 leaves the
+    second wave partly empty).  Odd pixels are full rank."""
     p, _ = pair(gm, orc, "single", None, precision=1, solver=solver)
     ctx = gm.Context(p)
     rng = np.random.default_rng(17)
-    n = 128
     packed = np.zeros((n, 45), dtype=np.float32)
     A = np.zeros((n, 9, 9))
     for i in range(n):
@@ -305,7 +306,7 @@ def fp64_chisqr(orc, o, obs, models):
 
 
 def assert_inversion(d, label="", truth=None, fit=None, med=MED, p99=P99, frac=0.97,
-                     frac_wp=0.99):
+                     frac_wp=0.99, worse=0.01):
     """Bounds the per-pixel differences to the oracle's inversion of the
     same pixels.
 
@@ -319,8 +320,8 @@ def assert_inversion(d, label="", truth=None, fit=None, med=MED, p99=P99, frac=0
       the iteration cap, are where LM paths part;
     * fit = (chi2 of the GPU models, chi2 of the oracle models), both in
       FP64: where the two part, the GPU's fit is as good -- the median
-      relative difference below 1e-5 and at most 1 % of the pixels worse by
-      more than 1e-3."""
+      relative difference below 1e-5 and at most `worse` (1 %) of the
+      pixels worse by more than 1e-3."""
     st = inversion_stats(d, truth)
     msg = "%s n %d within-bar %.4f " % (label, st["n"], st["within"])
     if truth is not None:
@@ -343,7 +344,7 @@ def assert_inversion(d, label="", truth=None, fit=None, med=MED, p99=P99, frac=0
             assert st["p99_wp"][f] <= p99[f], (label, NAMES[f], "p99", st["p99_wp"][f])
     if fit is not None:
         assert np.median(np.abs(rel)) <= 1e-5, (label, "chi2 median", np.median(np.abs(rel)))
-        assert (rel > 1e-3).mean() <= 0.01, (label, "chi2 worse fraction", (rel > 1e-3).mean())
+        assert (rel > 1e-3).mean() <= worse, (label, "chi2 worse fraction", (rel > 1e-3).mean())
     return st
 
 
@@ -360,20 +361,39 @@ def _assert_medians(d, label=""):
     return assert_inversion(d, label)
 
 
-@pytest.mark.parametrize("precision,solver", [(1, 0), (0, 0), (1, 1), (1, 2), (1, 3), (0, 3)])
+@pytest.mark.parametrize("precision,solver", [(1, 0), (0, 0), (1, 1), (1, 2), (0, 2), (1, 3),
+                                              (0, 3)])
 def test_inversion_6173_30l_nopsf_matches_oracle(gm, orc, cuda, precision, solver):
     """64 x 64 pixels, Fe I 6173, 30 wavelengths, no PSF, 9 free, 50 it;
-    solver 3 (the reference's CORDIC solve) against the oracle's CORDIC LM."""
+    solver 3 (the reference's CORDIC solve) against the oracle's CORDIC LM.
+
+    FP32 with every step through the Jacobi SVD (precision 0, solver 2) has
+    a wider bar: in FP32 the near-null eigenvalues of the ill-conditioned
+    systems come out at ~1e-7 of the largest, far above the pseudo-inverse's
+    1e-12 cut, so those steps keep directions the FP64 oracle drops (97.0 %
+    of the pixels inside the bar and 1.25 % with a fit worse by 1e-3, the
+    same with the earlier Jacobi that repeated the rotations in every
+    lane); the two-tier solve of the same precision (0, 0) meets the
+    default bar."""
     got, ref, truth, obs, ctx = _inversion_errors(gm, orc, cuda, "single", None, 4096, 100,
                                                   precision=precision, solver=solver)
     o = pair(gm, orc, "single", None, precision=precision, solver=solver)[1]
-    check_inversion(orc, o, obs, got, ref, truth, f"prec {precision} solver {solver}")
+    kw = dict(frac=0.96, worse=0.02) if (precision, solver) == (0, 2) else {}
+    check_inversion(orc, o, obs, got, ref, truth, f"prec {precision} solver {solver}", **kw)
     assert np.all(np.isfinite(got["model"])) and np.all(got["iters"] >= 1)
     assert np.all(got["iters"] <= 50)
     rel_c2 = np.abs(got["chisqr"] - ref["chisqr"]) / ref["chisqr"]
     assert np.median(rel_c2) < 1e-3
     st = ctx.stats()
     assert st["npix"] == 4096 and st["lm_rounds"] >= 1
+    # the Jacobi SVD count: every step with GM_SOLVE_JACOBI, the uncertified
+    # ones with GM_SOLVE_SVD, none otherwise
+    if solver == gm.GM_SOLVE_JACOBI:
+        assert st["svd_solves"] == st["solves"] > 0
+    elif solver == gm.GM_SOLVE_SVD:
+        assert 0 <= st["svd_solves"] < st["solves"]
+    else:
+        assert st["svd_solves"] == 0
 
 
 @pytest.mark.parametrize("precision,tail", [(1, 1e-9), (0, 1e-9), (0, 0.0)])
