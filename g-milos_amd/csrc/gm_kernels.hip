@@ -2449,8 +2449,14 @@ constexpr int svd_sweeps() { return sizeof(T) == 8 ? 12 : 10; }
  * fallback list solved by k_solve_svd; GM_SOLVE_LDL: LDL^T only;
  * GM_SOLVE_JACOBI: always the Jacobi SVD; GM_SOLVE_CORDIC: the reference's
  * 18-step FP32 CORDIC solve (cordic::solve10). */
+/* Minimum waves per SIMD k_solve is compiled for: 3 fits it in 166 VGPRs
+ * without spills (178 unconstrained, 2 waves); per 1M-pixel inversion the
+ * solves take 6.28 -> 6.20 ms; at 4 (128 VGPRs, 108 B of spills) 7.61 ms. */
+#ifndef GM_SOLVE_WAVES
+#define GM_SOLVE_WAVES 3
+#endif
 template <bool MIXED, int SOLVER>
-__global__ __launch_bounds__(64) void k_solve(const gm_dsetup *__restrict__ S,
+__global__ __launch_bounds__(64, GM_SOLVE_WAVES) void k_solve(const gm_dsetup *__restrict__ S,
                                               gm_state st, const int32_t *__restrict__ list,
                                               const int32_t *__restrict__ count,
                                               int32_t *__restrict__ reset0,
