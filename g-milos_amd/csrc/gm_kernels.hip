@@ -1313,8 +1313,14 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
 }
 
 /* Synthesis + PSF + chi^2 only (rejected steps need nothing more). */
+/* Minimum waves per SIMD k_synth is compiled for (1: no constraint; the
+ * 30-lambda kernel takes 68 VGPRs, 7 waves).  At 8 it fits 64 VGPRs with
+ * 12 B of spills: 8.25 -> 8.65 ms of synthesis per 1M-pixel inversion. */
+#ifndef GM_SYNTH_WAVES
+#define GM_SYNTH_WAVES 1
+#endif
 template <int LPP, int NLPL, bool MIXED, bool EXT, bool REC>
-__global__ __launch_bounds__(64) void k_synth(
+__global__ __launch_bounds__(64, GM_SYNTH_WAVES) void k_synth(
     const gm_dsetup *__restrict__ S, const double *__restrict__ models,
     const float *__restrict__ pre, int64_t mfs, int64_t mps, const float *__restrict__ obs,
     const int32_t *__restrict__ list, const int32_t *__restrict__ count,
