@@ -2453,16 +2453,19 @@ constexpr int svd_sweeps() { return sizeof(T) == 8 ? 12 : 10; }
 /* Step solve for the active list.  SOLVER = GM_SOLVE_SVD: LDL^T where it
  * provably equals the SVD pseudo-inverse, otherwise the pixel goes to the
  * fallback list solved by k_solve_svd; GM_SOLVE_LDL: LDL^T only;
- * GM_SOLVE_JACOBI: always the Jacobi SVD; GM_SOLVE_CORDIC: the reference's
- * 18-step FP32 CORDIC solve (cordic::solve10). */
+ * GM_SOLVE_JACOBI: always the Jacobi SVD, here; GM_SOLVE_CORDIC: the
+ * reference's 18-step FP32 CORDIC solve (cordic::solve10).  fb_count
+ * counts the Jacobi solves of both Jacobi modes. */
 /* Minimum waves per SIMD k_solve is compiled for: 3 fits it in 166 VGPRs
  * without spills (178 unconstrained, 2 waves); per 1M-pixel inversion the
- * solves take 6.28 -> 6.20 ms; at 4 (128 VGPRs, 108 B of spills) 7.61 ms. */
+ * solves take 6.28 -> 6.20 ms; at 4 (128 VGPRs, 108 B of spills) 7.61 ms.
+ * The Jacobi-always instantiation (a pixel's whole matrix and V in one
+ * lane) is left unconstrained. */
 #ifndef GM_SOLVE_WAVES
 #define GM_SOLVE_WAVES 3
 #endif
 template <bool MIXED, int SOLVER>
-__global__ __launch_bounds__(64, GM_SOLVE_WAVES) void k_solve(const gm_dsetup *__restrict__ S,
+__global__ __launch_bounds__(64, SOLVER == GM_SOLVE_JACOBI ? 1 : GM_SOLVE_WAVES) void k_solve(const gm_dsetup *__restrict__ S,
                                               gm_state st, const int32_t *__restrict__ list,
                                               const int32_t *__restrict__ count,
                                               int32_t *__restrict__ reset0,
@@ -2510,7 +2513,12 @@ __global__ __launch_bounds__(64, GM_SOLVE_WAVES) void k_solve(const gm_dsetup *_
             continue;
         }
         if (SOLVER == GM_SOLVE_JACOBI) {
-            fb_list[atomicAdd(fb_count, 1)] = (int32_t)pix;
+            /* every pixel through the Jacobi SVD: one pixel per lane (the
+             * wave Jacobi of the fallback list is one pixel per wave, for
+             * the few uncertified pixels of the two-tier solve) */
+            solve_svd<T, svd_sweeps<T>()>(a, b, x);
+            atomicAdd(fb_count, 1);
+            apply_to_trial<T>(S, st, pix, x);
             continue;
         }
         const bool proven = solve_ldl<T>(a, b, x);
@@ -2525,9 +2533,9 @@ __global__ __launch_bounds__(64, GM_SOLVE_WAVES) void k_solve(const gm_dsetup *_
     }
 }
 
-/* Jacobi SVD of the fallback list, one wave per pixel (wave_svd_solve).  Per 1M-pixel
- * inversion with the two-tier solve, the fallback pixels' Jacobi took 1.5 ms
- * with every lane repeating the rotations of the whole matrix. */
+/* Jacobi SVD of the fallback list of the two-tier solve, one wave per pixel
+ * (wave_svd_solve).  Per 1M-pixel inversion the fallback pixels' Jacobi
+ * took 1.5 ms with every lane repeating the rotations of the whole matrix. */
 template <bool MIXED>
 __global__ __launch_bounds__(64) void k_solve_svd(const gm_dsetup *__restrict__ S, gm_state st,
                                                   const int32_t *__restrict__ fb_list,
@@ -3213,7 +3221,7 @@ void gm_launch_solve(const gm_dsetup *dS, gm_state st, const int32_t *list,
         else GM_SOLVE_CASE(false, GM_SOLVE_JACOBI);
     }
 #undef GM_SOLVE_CASE
-    if (solver == GM_SOLVE_SVD || solver == GM_SOLVE_JACOBI) {
+    if (solver == GM_SOLVE_SVD) {
         const int gf = grid_for(grid_items, 1, 1024);
         if (mixed)
             hipLaunchKernelGGL((k_solve_svd<true>), dim3(gf), dim3(64), 0, s, dS, st, fb_list, fb_count);
