@@ -8,7 +8,8 @@
  *                        <base>_stokes.per, or of a FITS cube ->
  *                        <base>_mod.fits and <base>_stokes.fits
  * Options: --mixed (FP64 solve and chi^2 test), --ldl (LDL^T solve),
- * --exact-rf (exact magneto-optical response functions).
+ * --cordic (the reference's 18-step CORDIC step solve), --jacobi (every step
+ * by the Jacobi SVD), --exact-rf (exact magneto-optical response functions).
  */
 #include <hip/hip_runtime.h>
 
@@ -49,12 +50,13 @@ std::string base_of(const gmio_control &c)
 
 int main(int argc, char **argv)
 {
-    if (argc < 2) return fail("usage: gmilos <file.mtrol> [--mixed] [--ldl] [--cordic] [--exact-rf]");
+    if (argc < 2) return fail("usage: gmilos <file.mtrol> [--mixed] [--ldl] [--cordic] [--jacobi] [--exact-rf]");
     int precision = GM_PREC_FP32, solver = GM_SOLVE_SVD, exact_rf = 0;
     for (int i = 2; i < argc; ++i) {
         if (!std::strcmp(argv[i], "--mixed")) precision = GM_PREC_MIXED;
         else if (!std::strcmp(argv[i], "--ldl")) solver = GM_SOLVE_LDL;
         else if (!std::strcmp(argv[i], "--cordic")) solver = GM_SOLVE_CORDIC;   /* the reference's solve */
+        else if (!std::strcmp(argv[i], "--jacobi")) solver = GM_SOLVE_JACOBI;   /* every step by the SVD */
         else if (!std::strcmp(argv[i], "--exact-rf")) exact_rf = 1;
         else return fail("unknown option ", argv[i]);
     }

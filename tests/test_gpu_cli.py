@@ -178,6 +178,9 @@ def test_cli_inverts_reference_example(refdir):
     run(refdir, "run/invert.mtrol", "--cordic")          # the reference's own step solve
     m3 = read_mod(refdir / "data" / "synthesis_gpu_model.mod")
     np.testing.assert_allclose(model_row(m3)[:9], model_row(truth)[:9], rtol=5e-3, atol=2e-3)
+    run(refdir, "run/invert.mtrol", "--jacobi")          # every step through the SVD
+    m4 = read_mod(refdir / "data" / "synthesis_gpu_model.mod")
+    np.testing.assert_allclose(model_row(m4)[:9], model_row(truth)[:9], rtol=5e-3, atol=2e-3)
 
 
 def test_cli_classical_estimates_mode(refdir):
