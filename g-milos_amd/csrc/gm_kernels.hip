@@ -1659,24 +1659,28 @@ __device__ __forceinline__ void solve_svd(T (&a)[45], const T (&b)[9], T (&x)[9]
     }
 }
 
-/* The same Jacobi SVD by one whole wave for one matrix: lane l holds one
- * element of A (packed upper-triangle entry l, lanes 0-44) or of V (entry
- * l - 45, lanes 45-63) and one more of V (entry l + 19, lanes 0-61), so a
- * rotation step costs each lane two element updates instead of every lane
- * repeating the whole step.  The elements live in LDS (full symmetric A and
- * V, row-major); a step reads the three partners each element needs, lanes
- * 0-3 form the four rotations and publish (c, s, t), every lane reads the
- * rotations of its row and column, updates and writes back.  The lanes of
- * one wave reach LDS in program order, so no barrier is needed between the
- * phases, only a fence that keeps the compiler from reordering them.  The
- * rotations and update formulas are those of jacobi_step; the sweep test
- * sums the same squares (in another order).                                */
-template <typename T>
+/* The same Jacobi SVD by one whole wave for one N x N matrix (N = 9, or 11
+ * for the eleven-slot systems): element e of the list [A's packed upper
+ * triangle, N(N+1)/2 entries; V row-major, N^2 entries] belongs to lane
+ * e % 64, slot e / 64 (N = 9: A entry l or V entry l - 45 in slot 0, V entry
+ * l + 19 in slot 1; N = 11: three slots), so a rotation step costs each lane
+ * two or three element updates instead of every lane repeating the whole
+ * step.  The elements live in registers and in LDS (A in full and V,
+ * row-major); a step reads the three partner elements each update needs,
+ * lanes 0 .. (N-1)/2 - 1 form the step's rotations and publish (c, s, t),
+ * every lane reads the rotations of its elements' rows and columns, updates
+ * and writes back.  The lanes of one wave reach LDS in program order, so no
+ * barrier is needed between the phases, only a fence that keeps the
+ * compiler from reordering them.  The rotations and update formulas are
+ * those of jacobi_step; the sweep test sums the same squares (in another
+ * order).                                                                   */
+template <typename T, int N>
 struct WaveSvd {
-    T rot[5][4];           /* c, s, t of pair m (1-4); pair 0: the waiting index */
-    T a[81];
-    T v[81];
-    T b[9], aux[9], x[9];
+    static constexpr int P = (N - 1) / 2;   /* rotations per step */
+    T rot[P + 1][4];       /* c, s, t of pair m (1..P); pair 0: the waiting index */
+    T a[N * N];
+    T v[N * N];
+    T b[N], aux[N], x[N];
 };
 
 __device__ __forceinline__ void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); }
@@ -1692,139 +1696,218 @@ __device__ __forceinline__ T wave_uniform_sum(T v)
         return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
-/* Step K of the round-robin ordering: index i is paired with (2K - i) mod 9;
- * with m = (i - K) mod 9 it is the p of pair m (m = 1-4, sigma = -s) or the
- * q of pair 9 - m (m = 5-8, sigma = +s); m = 0 is the waiting index.       */
-template <typename T, int K>
-__device__ __forceinline__ void wave_jacobi_step(WaveSvd<T> &L, int lane, bool isA, int i0, int j0,
-                                                 int i1, int j1, bool has1, T *B0, int o0, int o0t,
-                                                 int o1, T &v0, T &v1)
+template <int N>
+__host__ __device__ constexpr int wpk(int i, int j) { return i * N - (i * (i - 1)) / 2 + (j - i); }
+
+/* One slot of a lane: its element (row i, column j of A or V), the LDS
+ * offsets it is written to (o, and ot = the mirrored entry of A), and
+ * whether it exists (the last slot has more lanes than elements). */
+struct WaveElem {
+    int i, j, o, ot;
+    bool isA, valid;
+};
+template <int N>
+struct WaveLayout {
+    static constexpr int NA = N * (N + 1) / 2, E = NA + N * N, S = (E + 63) / 64;
+    /* 0: every element of slot s is in V, 1: every one in A, 2: both */
+    static constexpr int kind(int s) { return 64 * s >= NA ? 0 : (64 * (s + 1) <= NA ? 1 : 2); }
+};
+template <int N>
+__device__ __forceinline__ WaveElem wave_elem(int e)
 {
-    constexpr int K2 = (2 * K) % 9;
-    const auto partner = [](int i) { const int r = K2 - i; return (r < 0) ? r + 9 : r; };
+    constexpr int NA = WaveLayout<N>::NA, E = WaveLayout<N>::E;
+    WaveElem r;
+    r.valid = e < E;
+    r.isA = e < NA;
+    if (r.isA) {
+        int i = 0;
+#pragma unroll
+        for (int q = 1; q < N; ++q) i += (e >= wpk<N>(q, q)) ? 1 : 0;
+        r.i = i;
+        r.j = i + e - (i * N - (i * (i - 1)) / 2);
+    } else {
+        const int f = r.valid ? e - NA : N * N - 1;
+        r.i = f / N;
+        r.j = f - N * r.i;
+    }
+    r.o = r.i * N + r.j;
+    r.ot = r.isA ? r.j * N + r.i : r.o;
+    return r;
+}
+
+/* Step K of the round-robin ordering: index i is paired with (2K - i) mod N;
+ * with m = (i - K) mod N it is the p of pair m (m = 1..P, sigma = -s) or the
+ * q of pair N - m (m = P+1..N-1, sigma = +s); m = 0 is the waiting index.  */
+template <typename T, int N, int K>
+__device__ __forceinline__ void wave_jacobi_step(WaveSvd<T, N> &L, int lane,
+                                                 const WaveElem (&el)[WaveLayout<N>::S],
+                                                 T (&val)[WaveLayout<N>::S])
+{
+    constexpr int S = WaveLayout<N>::S, P = WaveSvd<T, N>::P, K2 = (2 * K) % N;
+    const auto partner = [](int i) { const int r = K2 - i; return (r < 0) ? r + N : r; };
     const auto pairno = [](int i, bool &pside) {
         int m = i - K;
-        m += (m < 0) ? 9 : 0;
-        pside = m <= 4;
-        return pside ? m : 9 - m;
+        m += (m < 0) ? N : 0;
+        pside = m <= P;
+        return pside ? m : N - m;
     };
-    /* lanes 0-3: the rotation of pair mm (the other lanes repeat theirs) */
-    const int mm = (lane & 3) + 1;
+    /* lanes 0 .. P-1: the rotation of pair mm (the other lanes repeat one) */
+    const int mm = (lane % P) + 1;
     int p = K + mm, q = K - mm;
-    p -= (p >= 9) ? 9 : 0;
-    q += (q < 0) ? 9 : 0;
-    const T app = L.a[p * 10], aqq = L.a[q * 10], apq = L.a[p * 9 + q];
-    const int pi0 = partner(i0), pj0 = partner(j0), pj1 = partner(j1);
-    const T y0 = B0[i0 * 9 + pj0], z0 = B0[pi0 * 9 + j0], w0 = B0[pi0 * 9 + pj0];
-    const T y1 = L.v[i1 * 9 + pj1];
+    p -= (p >= N) ? N : 0;
+    q += (q < 0) ? N : 0;
+    const T app = L.a[p * (N + 1)], aqq = L.a[q * (N + 1)], apq = L.a[p * N + q];
+    int pi[S], pj[S];
+    T y[S], z[S], w[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        T *const B = (WaveLayout<N>::kind(s) == 0) ? L.v
+                     : (WaveLayout<N>::kind(s) == 1) ? L.a
+                     : (el[s].isA ? L.a : L.v);
+        pi[s] = partner(el[s].i);
+        pj[s] = partner(el[s].j);
+        y[s] = B[el[s].i * N + pj[s]];
+        if (WaveLayout<N>::kind(s) != 0) {
+            z[s] = B[pi[s] * N + el[s].j];
+            w[s] = B[pi[s] * N + pj[s]];
+        }
+    }
     T c, sn, t;
     jacobi_angle<T>(app, aqq, apq, c, sn, t);
-    if (lane < 4) {
+    if (lane < P) {
         L.rot[mm][0] = c;
         L.rot[mm][1] = sn;
         L.rot[mm][2] = t;
     }
     wave_fence();
-    bool ps_i, ps_j, ps_j1;
-    const int ni = pairno(i0, ps_i), nj = pairno(j0, ps_j), nj1 = pairno(j1, ps_j1);
-    T ci = L.rot[ni][0], si = L.rot[ni][1], ti = L.rot[ni][2];
-    T cj = L.rot[nj][0], sj = L.rot[nj][1];
-    T cj1 = L.rot[nj1][0], sj1 = L.rot[nj1][1];
-    /* all seven loaded here, together: otherwise the compiler moves some of
-     * them into the branches it makes of the selects below, behind a wait
-     * for the others */
-    asm volatile("" : "+v"(ci), "+v"(si), "+v"(ti), "+v"(cj), "+v"(sj), "+v"(cj1), "+v"(sj1));
-    const T gi = ps_i ? -si : si, gj = ps_j ? -sj : sj, gj1 = ps_j1 ? -sj1 : sj1;
-    /* A' = J^T A J, V' = V J: the column rotation of this element's row,
-     * then (A) the row rotation; a pair's diagonal gets a_pp - t a_pq,
-     * a_qq + t a_pq and its own off-diagonal element 0.  Selects, not
-     * branches: a branch would let the compiler sink the loads of z0, w0
-     * behind the rotation reads, one more LDS round trip per step. */
-    const T row = cj * v0 + gj * y0;
-    const T gen = ci * row + gi * (cj * z0 + gj * w0);
-    const T dia = v0 + (ps_i ? -ti : ti) * y0;
-    const bool pair_off = (j0 == pi0) && (i0 != j0);
-    T n0 = (i0 == j0) ? dia : (pair_off ? T(0) : gen);
-    n0 = isA ? n0 : row;
-    const T n1 = cj1 * v1 + gj1 * y1;
-    B0[o0] = n0;
-    B0[o0t] = n0;
-    if (has1) L.v[o1] = n1;
+    T nv[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        bool ps_i, ps_j;
+        const int ni = pairno(el[s].i, ps_i), nj = pairno(el[s].j, ps_j);
+        T cj = L.rot[nj][0], sj = L.rot[nj][1];
+        if (WaveLayout<N>::kind(s) == 0) {
+            asm volatile("" : "+v"(cj), "+v"(sj));
+            const T gj = ps_j ? -sj : sj;
+            nv[s] = cj * val[s] + gj * y[s];
+        } else {
+            T ci = L.rot[ni][0], si = L.rot[ni][1], ti = L.rot[ni][2];
+            /* all five loaded here, together: otherwise the compiler moves
+             * some of them into the branches it makes of the selects below,
+             * behind a wait for the others */
+            asm volatile("" : "+v"(ci), "+v"(si), "+v"(ti), "+v"(cj), "+v"(sj));
+            const T gi = ps_i ? -si : si, gj = ps_j ? -sj : sj;
+            /* A' = J^T A J, V' = V J: the column rotation of this element's
+             * row, then (A) the row rotation; a pair's diagonal gets
+             * a_pp - t a_pq, a_qq + t a_pq and its own off-diagonal element 0 */
+            const T row = cj * val[s] + gj * y[s];
+            const T gen = ci * row + gi * (cj * z[s] + gj * w[s]);
+            const T dia = val[s] + (ps_i ? -ti : ti) * y[s];
+            const bool pair_off = (el[s].j == pi[s]) && (el[s].i != el[s].j);
+            T n = (el[s].i == el[s].j) ? dia : (pair_off ? T(0) : gen);
+            if (WaveLayout<N>::kind(s) == 2) n = el[s].isA ? n : row;
+            nv[s] = n;
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        T *const B = (WaveLayout<N>::kind(s) == 0) ? L.v
+                     : (WaveLayout<N>::kind(s) == 1) ? L.a
+                     : (el[s].isA ? L.a : L.v);
+        if (el[s].valid) {
+            B[el[s].o] = nv[s];
+            if (WaveLayout<N>::kind(s) != 0) B[el[s].ot] = nv[s];
+        }
+        val[s] = nv[s];
+    }
     wave_fence();
-    v0 = n0;
-    v1 = n1;
 }
 
-/* Pseudo-inverse step of one pixel's damped system by the whole wave (all
- * 64 lanes active, the same pointers in every lane): al = packed 45-entry
- * matrix, be = right-hand side, damp = 1 + lambda on the diagonal.  x is
- * returned in every lane.                                                  */
-template <typename T, int MAXSWEEP>
-__device__ __forceinline__ void wave_svd_solve(WaveSvd<T> &L, const float *__restrict__ al,
-                                               const float *__restrict__ be, T damp, T (&x)[9])
+template <typename T, int N>
+__device__ __forceinline__ void wave_sweep(WaveSvd<T, N> &L, int lane,
+                                           const WaveElem (&el)[WaveLayout<N>::S],
+                                           T (&val)[WaveLayout<N>::S])
 {
-    const int lane = threadIdx.x & 63;
-    const bool isA = lane < 45;
-    int i0, j0;
-    if (isA) {
-        /* row of packed entry `lane`: ut(r, r) = 0, 9, 17, 24, 30, 35, 39, 42, 44 */
-        i0 = 0;
-#pragma unroll
-        for (int r = 1; r < 9; ++r) i0 += (lane >= ut(r, r)) ? 1 : 0;
-        j0 = i0 + lane - (i0 * 9 - (i0 * (i0 - 1)) / 2);
-    } else {
-        i0 = (lane - 45) / 9;
-        j0 = (lane - 45) - 9 * i0;
+    wave_jacobi_step<T, N, 0>(L, lane, el, val);
+    wave_jacobi_step<T, N, 1>(L, lane, el, val);
+    wave_jacobi_step<T, N, 2>(L, lane, el, val);
+    wave_jacobi_step<T, N, 3>(L, lane, el, val);
+    wave_jacobi_step<T, N, 4>(L, lane, el, val);
+    wave_jacobi_step<T, N, 5>(L, lane, el, val);
+    wave_jacobi_step<T, N, 6>(L, lane, el, val);
+    wave_jacobi_step<T, N, 7>(L, lane, el, val);
+    wave_jacobi_step<T, N, 8>(L, lane, el, val);
+    if constexpr (N > 9) {
+        wave_jacobi_step<T, N, 9>(L, lane, el, val);
+        wave_jacobi_step<T, N, 10>(L, lane, el, val);
     }
-    const bool has1 = lane < 62;
-    const int e1 = has1 ? lane + 19 : 80;
-    const int i1 = e1 / 9, j1 = e1 - 9 * i1;
-    T *const B0 = isA ? L.a : L.v;
-    const int o0 = i0 * 9 + j0, o0t = isA ? j0 * 9 + i0 : o0, o1 = i1 * 9 + j1;
-    T v0 = isA ? (T)al[lane] : ((i0 == j0) ? T(1) : T(0));
-    if (isA && i0 == j0) v0 = v0 * damp;
-    T v1 = (i1 == j1) ? T(1) : T(0);
-    B0[o0] = v0;
-    B0[o0t] = v0;
-    if (has1) L.v[o1] = v1;
+}
+
+/* Pseudo-inverse step of one pixel's damped N x N system by the whole wave
+ * (all 64 lanes active, the same pointers in every lane): al = packed
+ * N(N+1)/2-entry matrix, be = right-hand side, damp = 1 + lambda on the
+ * diagonal.  x is returned in every lane.                                  */
+template <typename T, int N, int MAXSWEEP>
+__device__ __forceinline__ void wave_svd_solve(WaveSvd<T, N> &L, const float *__restrict__ al,
+                                               const float *__restrict__ be, T damp, T (&x)[N])
+{
+    static_assert(N == 9 || N == 11, "wave Jacobi size");
+    constexpr int S = WaveLayout<N>::S;
+    const int lane = threadIdx.x & 63;
+    WaveElem el[S];
+    T val[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        el[s] = wave_elem<N>(64 * s + lane);
+        if (el[s].isA) {
+            val[s] = (T)al[64 * s + lane];
+            if (el[s].i == el[s].j) val[s] = val[s] * damp;
+        } else {
+            val[s] = (el[s].i == el[s].j) ? T(1) : T(0);
+        }
+        T *const B = el[s].isA ? L.a : L.v;
+        if (el[s].valid) {
+            B[el[s].o] = val[s];
+            B[el[s].ot] = val[s];
+        }
+    }
     if (lane < 4) L.rot[0][lane] = (lane == 0) ? T(1) : T(0);
-    if (lane < 9) L.b[lane] = (T)be[lane];
+    if (lane < N) L.b[lane] = (T)be[lane];
     wave_fence();
     const T eps2 = (sizeof(T) == 8) ? T(1e-30) : T(1e-14);
     for (int sweep = 0; sweep < MAXSWEEP; ++sweep) {
-        const T sq = isA ? v0 * v0 : T(0);
-        const T off = wave_uniform_sum<T>((i0 != j0) ? sq : T(0));
-        const T dg = wave_uniform_sum<T>((i0 == j0) ? sq : T(0));
+        T off = 0, dg = 0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            if (WaveLayout<N>::kind(s) == 0) continue;
+            const T sq = (el[s].isA && el[s].valid) ? val[s] * val[s] : T(0);
+            off += (el[s].i != el[s].j) ? sq : T(0);
+            dg += (el[s].i == el[s].j) ? sq : T(0);
+        }
+        off = wave_uniform_sum<T>(off);
+        dg = wave_uniform_sum<T>(dg);
         if (off <= eps2 * dg) break;
-        wave_jacobi_step<T, 0>(L, lane, isA, i0, j0, i1, j1, has1, B0, o0, o0t, o1, v0, v1);
-        wave_jacobi_step<T, 1>(L, lane, isA, i0, j0, i1, j1, has1, B0, o0, o0t, o1, v0, v1);
-        wave_jacobi_step<T, 2>(L, lane, isA, i0, j0, i1, j1, has1, B0, o0, o0t, o1, v0, v1);
-        wave_jacobi_step<T, 3>(L, lane, isA, i0, j0, i1, j1, has1, B0, o0, o0t, o1, v0, v1);
-        wave_jacobi_step<T, 4>(L, lane, isA, i0, j0, i1, j1, has1, B0, o0, o0t, o1, v0, v1);
-        wave_jacobi_step<T, 5>(L, lane, isA, i0, j0, i1, j1, has1, B0, o0, o0t, o1, v0, v1);
-        wave_jacobi_step<T, 6>(L, lane, isA, i0, j0, i1, j1, has1, B0, o0, o0t, o1, v0, v1);
-        wave_jacobi_step<T, 7>(L, lane, isA, i0, j0, i1, j1, has1, B0, o0, o0t, o1, v0, v1);
-        wave_jacobi_step<T, 8>(L, lane, isA, i0, j0, i1, j1, has1, B0, o0, o0t, o1, v0, v1);
+        wave_sweep<T, N>(L, lane, el, val);
     }
     /* x = V diag(1/w, w > 1e-12) V^T b, w = |a_jj|: lane j forms entry j of
      * each product */
-    if (lane < 9) {
+    if (lane < N) {
         T sum = 0;
 #pragma unroll
-        for (int k = 0; k < 9; ++k) sum += L.b[k] * L.v[k * 9 + lane];
-        const T w = fabs(L.a[lane * 10]);
+        for (int k = 0; k < N; ++k) sum += L.b[k] * L.v[k * N + lane];
+        const T w = fabs(L.a[lane * (N + 1)]);
         L.aux[lane] = (w > T(1e-12)) ? sum / w : T(0);
     }
     wave_fence();
-    if (lane < 9) {
+    if (lane < N) {
         T sum = 0;
 #pragma unroll
-        for (int k = 0; k < 9; ++k) sum += L.v[lane * 9 + k] * L.aux[k];
+        for (int k = 0; k < N; ++k) sum += L.v[lane * N + k] * L.aux[k];
         L.x[lane] = sum;
     }
     wave_fence();
 #pragma unroll
-    for (int i = 0; i < 9; ++i) x[i] = L.x[i];
+    for (int i = 0; i < N; ++i) x[i] = L.x[i];
     wave_fence();
 }
 
@@ -2372,18 +2455,23 @@ __device__ __forceinline__ void solve11_as10(const double (&a)[66], const double
 
 /* Step solve of the eleven-slot system for the active list, FP64 (the
  * reference's 11-term solve is FP64 svdcmp): LDL^T where it provably equals
- * the pseudo-inverse, the Jacobi SVD otherwise, then AplicaDelta + check. */
+ * the pseudo-inverse, then AplicaDelta + check; the other pixels go to the
+ * fallback list, solved by k_solve_svd<., 11>.  (With the Jacobi inline, a
+ * lane's matrix copy and V held the kernel at 256 VGPRs + 256 AGPRs, 804 B
+ * of spills and one wave per SIMD.) */
+template <bool CORDIC10>
 __global__ __launch_bounds__(64) void k_solve11(const gm_dsetup *__restrict__ S, gm_state st,
                                                 const int32_t *__restrict__ list,
                                                 const int32_t *__restrict__ count,
                                                 int32_t *__restrict__ reset0,
                                                 int32_t *__restrict__ reset1,
-                                                int32_t *__restrict__ hist, int cordic10)
+                                                int32_t *__restrict__ hist,
+                                                int32_t *__restrict__ fb_list,
+                                                int32_t *__restrict__ fb_count)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (hist) {
             hist[0] = *count;
-            hist[1] = 0;
             if (reset1) hist[-1] = *reset1;
         }
         if (reset0) *reset0 = 0;
@@ -2397,14 +2485,13 @@ __global__ __launch_bounds__(64) void k_solve11(const gm_dsetup *__restrict__ S,
         double a[66], b[11], x[11];
         load_damped_n<double, 11>(st.alpha + pix * st.ab_rec, st.beta + pix * st.ab_rec, 1,
                                   st.flambda[pix], a, b);
-        if (cordic10) {
+        if constexpr (CORDIC10) {
             __shared__ float th[cordic::TH_FLOATS];
             cordic::solve11_as10(a, b, x, th + threadIdx.x);
-        } else {
-            double a2[66];
-#pragma unroll
-            for (int e = 0; e < 66; ++e) a2[e] = a[e];
-            if (!solve_ldl_n<double, 11>(a, b, x)) solve_svd_n<double, 11, 14>(a2, b, x);
+        } else if (!solve_ldl_n<double, 11>(a, b, x)) {
+            /* no certificate: the wave Jacobi of k_solve_svd<., 11> */
+            fb_list[atomicAdd(fb_count, 1)] = (int32_t)pix;
+            continue;
         }
         double best[GM_NMODEL], m[GM_NMODEL];
 #pragma unroll
@@ -2426,21 +2513,39 @@ __global__ __launch_bounds__(64) void k_step_solve11(const float *__restrict__ a
                                                      int64_t n, double *__restrict__ delta,
                                                      int cordic10)
 {
-    for (int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * 64) {
-        double a[66], b[11], x[11];
-        load_damped_n<double, 11>(alpha + i * 66, beta + i * 11, 1, flambda[i], a, b);
-        if (cordic10) {
-            __shared__ float th[cordic::TH_FLOATS];
-            cordic::solve11_as10(a, b, x, th + threadIdx.x);
-        } else {
-            double a2[66];
+    for (int64_t base = (int64_t)blockIdx.x * 64; base < n; base += (int64_t)gridDim.x * 64) {
+        const int64_t i = base + threadIdx.x;
+        bool need_svd = false;
+        if (i < n) {
+            double a[66], b[11], x[11];
+            load_damped_n<double, 11>(alpha + i * 66, beta + i * 11, 1, flambda[i], a, b);
+            if (cordic10) {
+                __shared__ float th[cordic::TH_FLOATS];
+                cordic::solve11_as10(a, b, x, th + threadIdx.x);
+            } else {
+                need_svd = !solve_ldl_n<double, 11>(a, b, x);
+            }
+            if (!need_svd) {
 #pragma unroll
-            for (int e = 0; e < 66; ++e) a2[e] = a[e];
-            if (!solve_ldl_n<double, 11>(a, b, x)) solve_svd_n<double, 11, 14>(a2, b, x);
+                for (int p = 0; p < 11; ++p) delta[i * 11 + p] = x[p];
+            }
         }
+        /* the uncertified systems: the Jacobi SVD of the whole wave */
+        uint64_t need = __ballot(need_svd);
+        if (need) {
+            __shared__ WaveSvd<double, 11> L;
+            while (need) {
+                const int l = __builtin_ctzll(need);
+                need &= need - 1;
+                const int64_t fi = base + l;
+                double x[11];
+                wave_svd_solve<double, 11, 14>(L, alpha + fi * 66, beta + fi * 11, 1.0 + flambda[fi], x);
+                if ((int)threadIdx.x == l) {
 #pragma unroll
-        for (int p = 0; p < 11; ++p) delta[i * 11 + p] = x[p];
+                    for (int p = 0; p < 11; ++p) delta[fi * 11 + p] = x[p];
+                }
+            }
+        }
     }
 }
 
@@ -2536,21 +2641,37 @@ __global__ __launch_bounds__(64, SOLVER == GM_SOLVE_JACOBI ? 1 : GM_SOLVE_WAVES)
 /* Jacobi SVD of the fallback list of the two-tier solve, one wave per pixel
  * (wave_svd_solve).  Per 1M-pixel inversion the fallback pixels' Jacobi
  * took 1.5 ms with every lane repeating the rotations of the whole matrix. */
-template <bool MIXED>
+template <bool MIXED, int N>
 __global__ __launch_bounds__(64) void k_solve_svd(const gm_dsetup *__restrict__ S, gm_state st,
                                                   const int32_t *__restrict__ fb_list,
                                                   const int32_t *__restrict__ fb_count)
 {
-    using T = typename std::conditional<MIXED, double, float>::type;
-    __shared__ WaveSvd<T> L;
+    /* the eleven-slot systems are solved in FP64 in both precision modes */
+    using T = typename std::conditional<MIXED || N == 11, double, float>::type;
+    __shared__ WaveSvd<T, N> L;
     const int64_t cnt = *fb_count;
     const int64_t cap = st.cap;
     for (int64_t i = blockIdx.x; i < cnt; i += gridDim.x) {
         const int64_t pix = fb_list[i];
-        T x[9];
-        wave_svd_solve<T, svd_sweeps<T>()>(L, st.alpha + pix * st.ab_rec, st.beta + pix * st.ab_rec,
-                                           T(1) + (T)st.flambda[pix], x);
-        if (threadIdx.x == 0) apply_to_trial<T>(S, st, pix, x);
+        T x[N];
+        wave_svd_solve<T, N, (N == 11 ? 14 : svd_sweeps<T>())>(
+            L, st.alpha + pix * st.ab_rec, st.beta + pix * st.ab_rec, T(1) + (T)st.flambda[pix], x);
+        if (threadIdx.x == 0) {
+            if constexpr (N == 9) {
+                apply_to_trial<T>(S, st, pix, x);
+            } else {
+                double best[GM_NMODEL], m[GM_NMODEL];
+#pragma unroll
+                for (int f = 0; f < GM_NMODEL; ++f) {
+                    best[f] = st.best[pix * GM_MODEL_REC + f];
+                    m[f] = st.trial[pix * GM_MODEL_REC + f];
+                }
+                apply_delta_check<double, 11>(S, best, x, m);
+#pragma unroll
+                for (int f = 0; f < GM_NMODEL; ++f) st.trial[pix * GM_MODEL_REC + f] = m[f];
+                store_pre(st.pre_trial, pix, make_pre(m));
+            }
+        }
     }
 }
 
@@ -2586,14 +2707,14 @@ __global__ __launch_bounds__(64) void k_step_solve(const float *__restrict__ alp
         if constexpr (SOLVER == GM_SOLVE_SVD) {
             uint64_t need = __ballot(need_svd);
             if (need) {
-                __shared__ WaveSvd<T> L;
+                __shared__ WaveSvd<T, 9> L;
                 while (need) {
                     const int l = __builtin_ctzll(need);
                     need &= need - 1;
                     const int64_t fi = base + l;
                     T x[9];
-                    wave_svd_solve<T, svd_sweeps<T>()>(L, alpha + fi * 45, beta + fi * 9,
-                                                       T(1) + (T)flambda[fi], x);
+                    wave_svd_solve<T, 9, svd_sweeps<T>()>(L, alpha + fi * 45, beta + fi * 9,
+                                                          T(1) + (T)flambda[fi], x);
                     if ((int)threadIdx.x == l) {
 #pragma unroll
                         for (int p = 0; p < 9; ++p) delta[fi * 9 + p] = (double)x[p];
@@ -3200,10 +3321,17 @@ void gm_launch_solve(const gm_dsetup *dS, gm_state st, const int32_t *list,
 {
     const int g = grid_for(grid_items, 64, kMaxBlocks);
     if (np == 11) {
-        /* eleven slots: FP64 LDL^T / Jacobi in one kernel, no fallback list;
-         * with mac fixed the reference runs its 10-term CORDIC solve */
-        hipLaunchKernelGGL(k_solve11, dim3(g), dim3(64), 0, s, dS, st, list, count, reset0, reset1,
-                           hist, cordic10);
+        /* eleven slots: FP64 LDL^T, the uncertified systems by the wave
+         * Jacobi; with mac fixed the reference runs its 10-term CORDIC solve */
+        if (cordic10) {
+            hipLaunchKernelGGL((k_solve11<true>), dim3(g), dim3(64), 0, s, dS, st, list, count, reset0,
+                               reset1, hist, fb_list, fb_count);
+        } else {
+            hipLaunchKernelGGL((k_solve11<false>), dim3(g), dim3(64), 0, s, dS, st, list, count, reset0,
+                               reset1, hist, fb_list, fb_count);
+            hipLaunchKernelGGL((k_solve_svd<true, 11>), dim3(grid_for(grid_items, 1, 1024)), dim3(64), 0, s,
+                               dS, st, fb_list, fb_count);
+        }
         return;
     }
 #define GM_SOLVE_CASE(MX, SV)                                                                    \
@@ -3224,9 +3352,9 @@ void gm_launch_solve(const gm_dsetup *dS, gm_state st, const int32_t *list,
     if (solver == GM_SOLVE_SVD) {
         const int gf = grid_for(grid_items, 1, 1024);
         if (mixed)
-            hipLaunchKernelGGL((k_solve_svd<true>), dim3(gf), dim3(64), 0, s, dS, st, fb_list, fb_count);
+            hipLaunchKernelGGL((k_solve_svd<true, 9>), dim3(gf), dim3(64), 0, s, dS, st, fb_list, fb_count);
         else
-            hipLaunchKernelGGL((k_solve_svd<false>), dim3(gf), dim3(64), 0, s, dS, st, fb_list, fb_count);
+            hipLaunchKernelGGL((k_solve_svd<false, 9>), dim3(gf), dim3(64), 0, s, dS, st, fb_list, fb_count);
     }
 }
 
