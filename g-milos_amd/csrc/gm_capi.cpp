@@ -216,6 +216,7 @@ static gm_status build_dsetup(const gm_setup &s, gm_dsetup &d)
         d.freemask |= (1u << 10);
     }
     d.mac_l0 = s.lambda0[0];
+    d.mac_tail_tol = s.psf_tail_tol > 0 ? (float)s.psf_tail_tol : 0.f;
     for (int k = 0; k < n; ++k) d.psf_full[k] = d.use_psf ? (float)s.psf[k] : 0.f;
     if (d.use_slight)
         for (int k = 0; k < 4 * n; ++k) d.slight[k] = s.slight[k];

@@ -77,6 +77,10 @@ struct gm_dsetup {
     int32_t nt_ref;                          /* reference NTERMS: 10 or 11   */
     int32_t use_slight;
     double  mac_l0;                          /* lambda0 of the first line    */
+    float   mac_tail_tol;                    /* combined mac kernel: taps
+                                                outside the window holding
+                                                all those above tol * sum / n
+                                                are skipped (0: every tap) */
     float   psf_full[GM_MAX_LAMBDA];         /* all nlambda PSF taps         */
     float   slight[4 * GM_MAX_LAMBDA];       /* stray light [4][nlambda]     */
     const float *slight_map;                 /* per-pixel [npix][4][nlambda]

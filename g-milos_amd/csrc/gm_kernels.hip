@@ -690,11 +690,15 @@ __device__ float mac_kernel(const gm_dsetup *__restrict__ S, double mac, float *
         if (k < n) {
             float v = 0.f, vd = 0.f;
             if (S->use_psf) {
-                for (int j = 0; j < n; ++j) {
-                    const int i = k + j - c;
+                /* the PSF's applied taps (offsets psf_dlo..psf_dhi: all of
+                 * them with psf_tail_tol 0) */
+                const int dlo = S->psf_dlo, dhi = S->psf_dhi;
+                for (int d = dlo; d <= dhi; ++d) {
+                    const int i = k + d;
                     if (i >= 0 && i < n) {
-                        v += S->psf_full[j] * gk[i];
-                        if constexpr (DER) vd += S->psf_full[j] * gdk[i];
+                        const float tp = S->psf_tap[d - dlo];
+                        v += tp * gk[i];
+                        if constexpr (DER) vd += tp * gdk[i];
                     }
                 }
             } else {
@@ -718,6 +722,39 @@ __device__ __forceinline__ int circ_row(int m, int c, int j, int n)
     r += (r < 0) ? n : 0;
     r -= (r >= n) ? n : 0;
     return r;
+}
+
+/* Tap window [lo, hi] of this lane group's combined mac kernel gc (n taps,
+ * sum `total`) that holds every tap of at least tol * total / n: the taps
+ * outside it, skipped by the circular correlation, carry at most tol of the
+ * kernel's mass (tol = psf_tail_tol as for the PSF; 0 keeps every tap, the
+ * reference's direct correlation).  Called by every lane of the wave. */
+template <int LPP, int NLPL>
+__device__ __forceinline__ void mac_window(const gm_dsetup *__restrict__ S, const float *gc, float total,
+                                           int k0, int h, int &lo, int &hi)
+{
+    const int n = S->nlambda;
+    lo = 0;
+    hi = n - 1;
+    const float tol = S->mac_tail_tol;
+    if (!(tol > 0.f)) return;
+    const float thr = tol * total / (float)n;
+    int l = n, u = -1;
+#pragma unroll
+    for (int s = 0; s < NLPL; ++s) {
+        const int k = k0 + s * LPP;
+        const bool sig = k < n && gc[k] >= thr;
+        const uint64_t m = __ballot(sig);
+        const uint64_t mh = (LPP == 64) ? m : ((m >> (32 * h)) & 0xffffffffull);
+        if (mh) {
+            l = min(l, s * LPP + __builtin_ctzll(mh));
+            u = max(u, s * LPP + 63 - __builtin_clzll(mh));
+        }
+    }
+    if (u >= 0) {
+        lo = l;
+        hi = u;
+    }
 }
 
 /* The 30 of the 36 evaluated values (per Stokes: synth, d/d{eta0, B, vlos,
@@ -999,11 +1036,13 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
         /* EXT: this pixel's macroturbulence and filling factor */
         bool mac_on = false;
         float alv = 1.f, gsum = 1.f;
+        int mlo = 0, mhi = n - 1;
         if (EXT) {
             const double mac = models[9 * mfs + pix * mps];
             alv = (float)models[10 * mfs + pix * mps];
             mac_on = mac > 0.0001;
             gsum = mac_kernel<LPP, NLPL, P11>(S, mac, gk, gc, k0, gdk, gd);
+            mac_window<LPP, NLPL>(S, gc, gsum, k0, h, mlo, mhi);
         }
         const float s0c = (EXT && mac_on) ? gsum : 1.f;
         const float s0rf = (EXT && S->use_slight) ? alv : 1.f;
@@ -1033,7 +1072,7 @@ __global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
 #pragma unroll
                 for (int j = 0; j < 32; ++j) z[j] = 0.f;
                 const int kk = k < n ? k : n - 1;
-                for (int j = 0; j < n; ++j) {
+                for (int j = mlo; j <= mhi; ++j) {
                     const float tap = gc[j];
                     const float4 *row = conv + (padlo + circ_row(kk, n / 2, j, n)) * 9;
 #pragma unroll
@@ -1413,11 +1452,13 @@ __global__ __launch_bounds__(64, GM_SYNTH_WAVES) void k_synth(
         __syncthreads();
         bool mac_on = false;
         float alv = 1.f;
+        int mlo = 0, mhi = n - 1;
         if (EXT) {
             const double mac = models[9 * mfs + pix * mps];
             alv = (float)models[10 * mfs + pix * mps];
             mac_on = mac > 0.0001;
-            mac_kernel<LPP, NLPL>(S, mac, gk, gc, k0);
+            const float gsum = mac_kernel<LPP, NLPL>(S, mac, gk, gc, k0);
+            mac_window<LPP, NLPL>(S, gc, gsum, k0, h, mlo, mhi);
         }
         double chi = 0.0;
         float chif = 0.f;
@@ -1429,7 +1470,7 @@ __global__ __launch_bounds__(64, GM_SYNTH_WAVES) void k_synth(
                 /* macroturbulence (mil_sinrf.cu:101-116): circular, no Ic rule */
                 y = make_float4(0.f, 0.f, 0.f, 0.f);
                 const int kk = k < n ? k : n - 1;
-                for (int j = 0; j < n; ++j) {
+                for (int j = mlo; j <= mhi; ++j) {
                     const float tap = gc[j];
                     const float4 x = conv[padlo + circ_row(kk, n / 2, j, n)];
                     y.x += tap * x.x;
