@@ -102,13 +102,16 @@ def test_ext_normal_equations_match_oracle(gm, orc, cuda):
         assert c[i] == pytest.approx(orc.chisqr(o, obs[i], sp[i]), rel=2e-5)
 
 
-@pytest.mark.parametrize("mac,alfa,with_slight", [(1.2, 1.0, False), (0.8, 0.75, True)])
-def test_ext_inversion_matches_oracle(gm, orc, cuda, mac, alfa, with_slight):
+@pytest.mark.parametrize("mac,alfa,with_slight,tail", [(1.2, 1.0, False, 1e-9), (0.8, 0.75, True, 1e-9),
+                                                       (0.8, 0.75, True, 0.0)])
+def test_ext_inversion_matches_oracle(gm, orc, cuda, mac, alfa, with_slight, tail):
     """Fixed macroturbulence (the reference's 10-term run with mac fixed) and
     a fixed filling factor with stray light, inverted from the classical
-    estimates: medians of |d| against the oracle's inversion."""
+    estimates: medians of |d| against the oracle's inversion, which applies
+    every PSF and mac-kernel tap (tail 0: the product does too; 1e-9: it
+    skips the taps of negligible mass)."""
     L = stray_profile() if with_slight else None
-    p, o = ext_pair(gm, orc, GAUSS60, mac=mac, alfa=alfa, slight=L)
+    p, o = ext_pair(gm, orc, GAUSS60, mac=mac, alfa=alfa, slight=L, psf_tail_tol=tail)
     truth = gm.random_models(512, seed=35)
     truth[:, 9] = mac
     truth[:, 10] = alfa
