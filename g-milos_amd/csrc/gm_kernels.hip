@@ -31,6 +31,12 @@ constexpr double kPI = 3.14159265358979323846264338327950288;
  * kernel still fits 3 (161 VGPRs) but is scheduled more freely: per 1M-pixel
  * inversion 30.1 -> 28.7 ms against 3; the 2- and 4-wavelength-per-lane
  * kernels drop their scratch spills. */
+/* The same for two wavelengths per lane (nlambda 65-128, the two-line
+ * set-up's 112): 184 VGPRs at 2; at 3 (168 VGPRs, 52 B of spills) the
+ * two-line inversion's k_rf takes 32.0 -> 34.0 ms per 262 144 pixels. */
+#ifndef GM_RF_WAVES_2L
+#define GM_RF_WAVES_2L 2
+#endif
 #ifndef GM_RF_WAVES
 #define GM_RF_WAVES 2
 #endif
@@ -871,7 +877,7 @@ __device__ __forceinline__ LmStep<T> lm_step(const gm_dsetup *__restrict__ S, T 
 /* Synthesis + response functions + PSF + normal equations + chi^2 for a list
  * of pixels; wavelength-parallel mapping.                                   */
 template <int LPP, int NLPL, bool MIXED, int XM, bool REC>
-__global__ __launch_bounds__(64, GM_RF_WAVES) void k_rf(
+__global__ __launch_bounds__(64, NLPL == 2 ? GM_RF_WAVES_2L : GM_RF_WAVES) void k_rf(
     const gm_dsetup *__restrict__ S, const double *__restrict__ models,
     const float *__restrict__ pre, int64_t mfs, int64_t mps, const float *__restrict__ obs,
     const int32_t *__restrict__ list, const int32_t *__restrict__ count,
