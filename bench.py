@@ -220,14 +220,18 @@ def accuracy_vs_oracle(got, ref, truth):
     return out
 
 
-def pmc_probe(gm, torch, dev, npix, seed):
+def pmc_probe(gm, torch, dev, npix, seed, which="single"):
     """Counter calibration and probe: a device-to-device copy of 256 MiB (its
     FETCH_SIZE / WRITE_SIZE scale the counters to bytes), then one inversion
     of npix pixels of the 30-lambda PSF set-up (mixed precision, <= 50 LM
     iterations), whose k_rf and k_synth launches are the ones the benchmark
-    times.  Prints the pixel evaluations of each kernel, by which the summed
-    counters of its launches are divided."""
-    setup = gm.setup_single_line(psf_fwhm=60.0)
+    times -- or, with which = "two", of the two-line 112-lambda PSF set-up of
+    the mixed-precision leg.  Prints the pixel evaluations of each kernel, by
+    which the summed counters of its launches are divided."""
+    if which == "two":
+        setup = gm.setup_two_lines(psf_fwhm=60.0)
+    else:
+        setup = gm.setup_single_line(psf_fwhm=60.0)
     ctx = gm.Context(setup)
     n_copy = 64 << 20
     a = torch.ones(n_copy, dtype=torch.float32, device=dev)
@@ -244,8 +248,10 @@ def pmc_probe(gm, torch, dev, npix, seed):
     print(json.dumps({"copy_bytes": 4 * n_copy, "npix": npix,
                       "rf_evals": int(st["rf_evals"]), "synth_evals": int(st["synth_evals"]),
                       "solves": int(st["solves"]), "svd_solves": int(st["svd_solves"]),
-                      "k_rf_algorithmic_bytes_per_eval": 20 * 4 + 4 * 30 * 4 + 54 * 4 + 8,
-                      "k_synth_algorithmic_bytes_per_eval": 20 * 4 + 4 * 30 * 4 + 8}), flush=True)
+                      "nlambda": int(setup.nlambda),
+                      "k_rf_algorithmic_bytes_per_eval": 20 * 4 + 4 * setup.nlambda * 4 + 54 * 4 + 8,
+                      "k_synth_algorithmic_bytes_per_eval": 20 * 4 + 4 * setup.nlambda * 4 + 8}),
+          flush=True)
 
 
 # Serving admission budget: 2 GB of in-flight LM state (about 0.5-1 M
@@ -597,6 +603,8 @@ def main():
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only run a plain device copy of known size and one normal-equation "
                          "(k_rf) and one synthesis (k_synth) launch, for counter runs")
+    ap.add_argument("--pmc-setup", choices=["single", "two"], default="single",
+                    help="the probe's set-up: the headline one-line cube or the two-line one")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -613,7 +621,7 @@ def main():
     torch.cuda.set_device(dev)
 
     if args.pmc_probe:
-        pmc_probe(gm, torch, dev, args.npix, args.seed)
+        pmc_probe(gm, torch, dev, args.npix, args.seed, args.pmc_setup)
         return
 
     prec = gm.GM_PREC_MIXED if args.precision == "mixed" else gm.GM_PREC_FP32
