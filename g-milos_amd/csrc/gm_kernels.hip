@@ -2858,9 +2858,9 @@ __global__ __launch_bounds__(GM_DECIDE_THREADS) void k_decide(const gm_dsetup *_
         }
         __syncthreads();
         /* lanes 0, 1, 2: prefix sums of the next, accepted and done
-         * counts over the waves, and one list reservation each -- the three
-         * atomics are one instruction (from one lane, each atomic's return
-         * was waited for before the next was issued) */
+         * counts over the waves, and one list reservation each; the three
+         * atomics go out as one instruction (issued from a single lane,
+         * each one's return was waited for before the next was issued) */
         if (tid < 3) {
             int *wl = (tid == 0) ? wc : (tid == 1) ? wa : wd;
             int32_t *ctr = (tid == 0) ? next_count : (tid == 1) ? acc_count : done_count;
