@@ -174,7 +174,8 @@ void gm_launch_decide(const gm_dsetup *dS, gm_state st, const int32_t *list,
                       const int32_t *count, int64_t grid_items, int32_t *next_list,
                       int32_t *next_count, int32_t *acc_list, int32_t *acc_count,
                       int32_t *reset_fb, int32_t *done_list, int32_t *done_count, int mixed,
-                      hipStream_t stream, int defer_copy = 0, int32_t *fb_hist = nullptr);
+                      hipStream_t stream, int defer_copy = 0, int32_t *fb_hist = nullptr,
+                      int small_groups = 0);
 void gm_launch_init_state(const gm_dsetup *dS, gm_state st, const float *obs,
                           const double *init, int init_mode, int nlambda, int64_t slot0,
                           int64_t npix, int32_t *list0, hipStream_t stream);

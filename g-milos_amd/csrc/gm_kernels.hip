@@ -2772,10 +2772,6 @@ __global__ __launch_bounds__(64) void k_step_solve(const float *__restrict__ alp
     }
 }
 
-/* Accept / reject and LM bookkeeping (milosUtils.cu:660-698); builds the
- * next active list and the list of accepted pixels that need new normal
- * equations, with one atomic per 1024-pixel workgroup and list.            */
-template <bool MIXED>
 /* Workgroup size and grid cap of k_decide.  Per 1M-pixel inversion the
  * decisions take the same 2.5 ms (with the other small kernels) with caps of
  * 256, 512 or 1024 workgroups of 1024; workgroups of 512 or 256 (more of
@@ -2788,7 +2784,14 @@ template <bool MIXED>
 #ifndef GM_DECIDE_BLOCKS
 #define GM_DECIDE_BLOCKS 512
 #endif
-__global__ __launch_bounds__(GM_DECIDE_THREADS) void k_decide(const gm_dsetup *__restrict__ S, gm_state st,
+/* Accept / reject and LM bookkeeping (milosUtils.cu:660-698); builds the
+ * next active list and the list of accepted pixels that need new normal
+ * equations, with one atomic per workgroup and list.  THREADS: workgroup
+ * size (GM_DECIDE_THREADS for an inversion; the serving loop, whose rounds
+ * run beside other set-ups' kernels, takes 256, which fit partly occupied
+ * CUs). */
+template <bool MIXED, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_decide(const gm_dsetup *__restrict__ S, gm_state st,
                                                  const int32_t *__restrict__ list,
                                                  const int32_t *__restrict__ count,
                                                  int32_t *__restrict__ next_list,
@@ -2801,7 +2804,7 @@ __global__ __launch_bounds__(GM_DECIDE_THREADS) void k_decide(const gm_dsetup *_
                                                  int defer_copy, int32_t *__restrict__ fb_hist)
 {
     using T = typename std::conditional<MIXED, double, float>::type;
-    __shared__ int wc[GM_DECIDE_THREADS / 64], wa[GM_DECIDE_THREADS / 64], wd[GM_DECIDE_THREADS / 64];
+    __shared__ int wc[THREADS / 64], wa[THREADS / 64], wd[THREADS / 64];
     __shared__ int base_c, base_a, base_d;
     /* the solve kernel of this round is complete: its SVD count is final */
     if (blockIdx.x == 0 && threadIdx.x == 0 && reset_fb) {
@@ -2811,8 +2814,8 @@ __global__ __launch_bounds__(GM_DECIDE_THREADS) void k_decide(const gm_dsetup *_
     const int64_t cnt = *count;
     const int64_t cap = st.cap;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int64_t base = (int64_t)blockIdx.x * GM_DECIDE_THREADS; base < cnt;
-         base += (int64_t)gridDim.x * GM_DECIDE_THREADS) {
+    for (int64_t base = (int64_t)blockIdx.x * THREADS; base < cnt;
+         base += (int64_t)gridDim.x * THREADS) {
         const int64_t i = base + tid;
         const bool valid = i < cnt;
         int32_t pix = valid ? list[i] : 0;
@@ -3435,17 +3438,20 @@ void gm_launch_decide(const gm_dsetup *dS, gm_state st, const int32_t *list,
                       const int32_t *count, int64_t grid_items, int32_t *next_list,
                       int32_t *next_count, int32_t *acc_list, int32_t *acc_count,
                       int32_t *reset_fb, int32_t *done_list, int32_t *done_count, int mixed,
-                      hipStream_t s, int defer_copy, int32_t *fb_hist)
+                      hipStream_t s, int defer_copy, int32_t *fb_hist, int small_groups)
 {
-    const int g = grid_for(grid_items, GM_DECIDE_THREADS, GM_DECIDE_BLOCKS);
-    if (mixed)
-        hipLaunchKernelGGL((k_decide<true>), dim3(g), dim3(GM_DECIDE_THREADS), 0, s, dS, st, list,
-                           count, next_list, next_count, acc_list, acc_count, reset_fb, done_list,
-                           done_count, defer_copy, fb_hist);
-    else
-        hipLaunchKernelGGL((k_decide<false>), dim3(g), dim3(GM_DECIDE_THREADS), 0, s, dS, st, list, count,
-                           next_list, next_count, acc_list, acc_count, reset_fb, done_list,
-                           done_count, defer_copy, fb_hist);
+#define GM_DECIDE_CASE(MX, TH)                                                                         \
+    hipLaunchKernelGGL((k_decide<MX, TH>), dim3(grid_for(grid_items, TH, GM_DECIDE_BLOCKS * (GM_DECIDE_THREADS / TH))), \
+                       dim3(TH), 0, s, dS, st, list, count, next_list, next_count, acc_list, acc_count,   \
+                       reset_fb, done_list, done_count, defer_copy, fb_hist)
+    if (small_groups) {
+        if (mixed) GM_DECIDE_CASE(true, 256);
+        else GM_DECIDE_CASE(false, 256);
+    } else {
+        if (mixed) GM_DECIDE_CASE(true, GM_DECIDE_THREADS);
+        else GM_DECIDE_CASE(false, GM_DECIDE_THREADS);
+    }
+#undef GM_DECIDE_CASE
 }
 
 #ifndef GM_INIT_PPB

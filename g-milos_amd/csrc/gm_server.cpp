@@ -297,6 +297,12 @@ static void admit(Engine &e, const Pending &pd, int t, int64_t slot0)
     e.ntiles += 1;
 }
 
+/* k_decide's workgroups in the serving loop: 256 threads (1) or the
+ * inversion's 1024 (0). */
+#ifndef GM_SERVE_DECIDE_SMALL
+#define GM_SERVE_DECIDE_SMALL 1
+#endif
+
 /* One LM round over the engine's active list (cf. gm_invert). */
 static void launch_round(Engine &e)
 {
@@ -313,7 +319,7 @@ static void launch_round(Engine &e)
     gm_launch_synth(e.lc, e.dS, st.trial, st.pre_trial, -GM_MODEL_REC, e.obs, st.list[cur], &cnt[cur], 0, grid,
                     st.chi_trial, e.synth, e.mixed, s);
     gm_launch_decide(e.dS, st, st.list[cur], &cnt[cur], grid, st.list[nxt], &cnt[nxt], st.list[2],
-                     &cnt[2], &cnt[3], e.done, &cnt[5], e.mixed, s, 1);
+                     &cnt[2], &cnt[3], e.done, &cnt[5], e.mixed, s, 1, nullptr, GM_SERVE_DECIDE_SMALL);
     gm_launch_rf(e.lc, e.dS, st.trial, st.pre_trial, -GM_MODEL_REC, e.obs, st.list[2], &cnt[2], 0, grid,
                  st.alpha, st.beta, -st.ab_rec, nullptr, nullptr, nullptr, e.mixed, s, 0, st.best,
                  st.pre_best);
